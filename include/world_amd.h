@@ -1,0 +1,253 @@
+/*
+ * world_amd.h -- C ABI of the MI355X (gfx950) granular-dynamics engine.
+ *
+ * One handle owns one simulation: its device buffers, one HIP stream and the
+ * captured step graph.  Every entry point returns 0 on success and a negative
+ * status on failure; wd_last_error() then returns a message for the calling
+ * thread.  Host buffers are owned by the caller.  All arrays crossing this
+ * boundary are in ORIGINAL grain order (the order the initializer produced or
+ * the caller last wrote), even though the engine keeps grains in cell order
+ * internally.
+ *
+ * State layout is the reference's flat FP32 [x, y, z, vx, vy, vz] per grain
+ * with z up (src/sim_config.h:5-15).  The C++ class façade in
+ * include/world/particle_dynamics.h drives these functions; the ctypes layer
+ * in world_amd/_capi.py binds the same symbols for Python.
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef WORLD_AMD_H
+#define WORLD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WD_OK 0
+#define WD_ERR_ARG (-1)      /* bad argument or configuration */
+#define WD_ERR_HIP (-2)      /* HIP runtime failure */
+#define WD_ERR_IO (-3)       /* file could not be read */
+#define WD_ERR_STATE (-4)    /* call not valid in the handle's current state */
+#define WD_ERR_DEVICE (-5)   /* a kernel reported an internal failure */
+
+#define WD_MAX_MATERIALS 16
+#define WD_PATH_MAX 512
+
+/* Stage indices of wd_stage_ms (reference *_wct getters,
+ * src/particle_dynamics.h:77-80). */
+#define WD_STAGE_FIND_NEIGHBORS 0
+#define WD_STAGE_COMPUTE_RHS 1
+#define WD_STAGE_TAKE_STEP 2
+#define WD_STAGE_DENSITY_GRID 3
+
+enum wd_init_type {
+    WD_INIT_CUBE = 0,
+    WD_INIT_TWO_PARTICLES = 1,
+    WD_INIT_SINGLE_PARTICLE = 2,
+    WD_INIT_FILE = 3
+};
+
+enum wd_integrator { WD_SEMI_IMPLICIT_EULER = 0, WD_BACKWARD_EULER_PICARD = 1 };
+
+/* Extension: tangential contact model.  VISCOUS is the reference's stateless
+ * regularized Coulomb law (src/compute_rhs_kernel.cu:66-78); SPRING_HISTORY
+ * adds a per-contact tangential spring whose displacement persists between
+ * steps and is capped at the Coulomb limit. */
+enum wd_tangential_model { WD_TANGENTIAL_VISCOUS = 0, WD_TANGENTIAL_SPRING_HISTORY = 1 };
+
+/* Extension: distribution of initialization.perturbation. */
+enum wd_perturbation_distribution { WD_PERTURB_UNIFORM = 0, WD_PERTURB_GAUSSIAN = 1 };
+
+/*
+ * Plain-data mirror of the reference SimConfig (src/sim_config.h:64-200),
+ * restricted to what the solver reads, plus opt-in extension keys whose
+ * defaults reproduce the reference.  Field names follow the YAML keys.
+ */
+typedef struct wd_config {
+    /* simulation: */
+    float dt;
+    int collision_grid_size_x, collision_grid_size_y, collision_grid_size_z;
+    int particles_per_cell;
+    int threads_per_block;
+    int density_grid_size_x, density_grid_size_y, density_grid_size_z;
+    /* domain: */
+    float x_min, x_max, y_min, y_max, z_min, z_max;
+    /* physics: (DomainParams / PhysicsParams, src/sim_config.h:28-61) */
+    float gravity, particle_radius, max_force;
+    float floor_z, ceiling_z, wall_x_min, wall_x_max, wall_y_min, wall_y_max;
+    float restitution, friction;
+    /* physics extensions */
+    int tangential_model;             /* wd_tangential_model, default VISCOUS */
+    float tangential_stiffness_ratio; /* k_t / k_n for SPRING_HISTORY, default 2/7 */
+    float normal_stiffness;           /* <= 0: k = max_force / particle_radius (reference) */
+    /* materials: */
+    int num_masses;
+    float masses[WD_MAX_MATERIALS];
+    /* initialization: */
+    int init_type;                    /* wd_init_type */
+    char init_state_path[WD_PATH_MAX];
+    float init_x0, init_y0, init_z0;
+    float init_vx0, init_vy0, init_vz0;
+    float init_jitter;
+    unsigned int init_seed;
+    float init_perturbation;
+    unsigned int init_perturbation_seed;
+    float cube_length_x, cube_length_y, cube_length_z;
+    float two_particle_separation;
+    int particle_material;
+    float lattice_spacing;            /* extension, default 1.01 (diameters) */
+    int perturbation_distribution;    /* extension, wd_perturbation_distribution */
+    /* time_integration: */
+    int time_integrator;              /* wd_integrator */
+    int picard_iterations;
+    /* drivers: */
+    int kernel_timing;
+} wd_config;
+
+typedef struct wd_sim wd_sim;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* wd_last_error(void);
+
+/* Library version string. */
+const char* wd_version(void);
+
+/* sizeof(wd_config) as compiled into the library, so foreign-language
+ * bindings can verify their struct layout. */
+size_t wd_config_size(void);
+
+/* Reference SimConfig{} defaults (src/sim_config.h:64-200). */
+int wd_config_default(wd_config* out);
+
+/* YAML-subset loader with the reference's semantics: starts from defaults,
+ * overrides present keys, warns on unknown keys, returns defaults for a
+ * missing file (load_config, src/sim_config.cpp:153-195). */
+int wd_config_load(const char* path, wd_config* out);
+
+/* Number of grains the configured initializer produces, without touching the
+ * GPU (initialize_to_*, src/particle_dynamics.cu:128-235). */
+int wd_config_grain_count(const wd_config* cfg, int* n_out);
+
+/* Host-side initializer: writes the initial state (6n floats) and material
+ * (n ints) exactly as the constructor would (src/particle_dynamics.cu:20-33).
+ * `capacity` is the number of grains the buffers hold. */
+int wd_initial_state(const wd_config* cfg, float* state_out, int* material_out, int capacity);
+
+/* Constructs a simulation: initializes on the host, uploads, sizes every
+ * buffer (ParticleDynamics::ParticleDynamics, src/particle_dynamics.cu:10-94). */
+int wd_create(const wd_config* cfg, wd_sim** out);
+
+/* Releases the handle (ParticleDynamics::~ParticleDynamics). */
+int wd_destroy(wd_sim* sim);
+
+int wd_num_grains(const wd_sim* sim, int* n_out);
+
+/* Copies the state to the host, original order (unpack_state /
+ * host_state.copy_from_device, src/particle_dynamics.cu:109-124). */
+int wd_get_state(wd_sim* sim, float* state_out);
+
+/* Overwrites the state from the host, original order
+ * (device_state.copy_from_host, src/device_vector.h:59-61). */
+int wd_set_state(wd_sim* sim, const float* state_in);
+
+/* Per-grain material ids (host_material / device_material). */
+int wd_get_material(wd_sim* sim, int* material_out);
+int wd_set_material(wd_sim* sim, const int* material_in);
+
+/* Time step read by the next wd_step (public ParticleDynamics::dt, read per
+ * step at src/particle_dynamics.cu:268,276). */
+int wd_set_dt(wd_sim* sim, float dt);
+int wd_get_time(const wd_sim* sim, float* time_out);
+int wd_set_time(wd_sim* sim, float time);
+
+/* Advances nsteps steps of the configured integrator
+ * (ParticleDynamics::take_step, src/particle_dynamics.cu:264-281).
+ * Asynchronous: returns once the work is queued on the handle's stream. */
+int wd_step(wd_sim* sim, int nsteps);
+
+/* Waits for all queued work on the handle's stream. */
+int wd_sync(wd_sim* sim);
+
+/* Cloud-in-cell mass + momentum deposit, channel-major
+ * [4][nx][ny][nz] float (compute_density_grid,
+ * src/particle_dynamics.cu:289-293; src/density_grid_kernel.cu:25-106).
+ * `out` receives 4 * density nodes floats. */
+int wd_density_grid(wd_sim* sim, float* out);
+
+/* Total kinetic + gravitational + elastic energy of the current state
+ * (compute_total_energy, src/particle_dynamics.cu:296-308;
+ * src/energy_kernel.cu:22-100). */
+int wd_total_energy(wd_sim* sim, float* energy_out);
+
+/* Extension: per-step energy log.  When capacity > 0, every subsequent step
+ * records the total energy of the state it starts from (the value
+ * compute_total_energy would return at that point), fused into the force pass.
+ * wd_energy_log_read copies the first `count` entries recorded since the log
+ * was enabled and returns how many exist in *recorded. */
+int wd_energy_log_enable(wd_sim* sim, int capacity);
+int wd_energy_log_read(wd_sim* sim, double* out, int count, int* recorded);
+
+/* Extension: device-resident CIC checkpoints (the dataset driver's K
+ * recorded grids, src/dataset.cpp:201-214, without a host round trip per
+ * checkpoint).  wd_density_log_enable reserves `capacity` grids;
+ * wd_density_log_record queues one deposit of the current state into the
+ * next slot without waiting; wd_density_log_read copies the first `count`
+ * recorded grids (4 * nodes floats each) and reports how many exist. */
+int wd_density_log_enable(wd_sim* sim, int capacity);
+int wd_density_log_record(wd_sim* sim);
+int wd_density_log_read(wd_sim* sim, float* out, int count, int* recorded);
+
+/* Measurement: runs nsteps steps with one HIP event pair around every kernel
+ * launch on the handle's stream and writes the mean duration (ms) of each
+ * kernel of a step, in launch order, to kernel_ms[0..*count).  `names`
+ * (optional) receives a comma-separated list of the kernel names.
+ * The state advances by nsteps steps. */
+int wd_profile_step(wd_sim* sim, int nsteps, float* kernel_ms, int capacity, int* count,
+                    char* names, size_t names_capacity);
+
+/* Max |acceleration| from the last force pass (compute_max_acceleration,
+ * src/particle_dynamics.cu:328-344). */
+int wd_max_accel(wd_sim* sim, float* out);
+
+/* Finite and inside the domain box (is_stable, src/particle_dynamics.cu:311-325). */
+int wd_is_stable(wd_sim* sim, int* stable_out);
+
+/* Lifetime count of (cell, neighbor pass) over-capacity incidents
+ * (grid_overflow_count / FindNeighborsKernel::overflow_count,
+ * src/find_neighbors_kernel.cu:230-234). */
+int wd_overflow_count(wd_sim* sim, int* count_out);
+
+/* Lifetime per-stage milliseconds (find_neighbors_wct, compute_rhs_wct,
+ * take_step_wct, density_grid_wct; src/particle_dynamics.cu:350-358).
+ * Waits for queued work.  The force pass integrates in the same kernel, so
+ * its time is reported under COMPUTE_RHS and TAKE_STEP stays 0. */
+int wd_stage_ms(wd_sim* sim, float out[4]);
+
+/* ---- parity / inspection entry points ---------------------------------- */
+
+/* Runs one neighbor pass on the current state and returns the cell order:
+ * ids_out[p] = original index of the grain at cell-sorted position p, and
+ * keys_out[p] = its flat collision cell (cx*Gy + cy)*Gz + cz
+ * (flat_cell / particle_cell, src/find_neighbors_kernel.cu:28-40). */
+int wd_cell_order(wd_sim* sim, int* ids_out, uint32_t* keys_out);
+
+/* Neighbor rows in the reference's flat format: n rows of
+ * 27 * particles_per_cell ints, original indices, -1 terminated
+ * (device_neighbors, src/particle_dynamics.h:48-52;
+ * find_neighbors_kernel, src/find_neighbors_kernel.cu:119-161).
+ * `capacity` is the number of ints `out` holds. */
+int wd_neighbor_rows(wd_sim* sim, int* out, size_t capacity);
+
+/* One force evaluation at the current state without integrating:
+ * rhs = [v, F/m] per grain, original order (device_rhs written by
+ * compute_rhs_kernel, src/compute_rhs_kernel.cu:81-212). */
+int wd_rhs(wd_sim* sim, float* rhs_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WORLD_AMD_H */

@@ -1,0 +1,187 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker (and the timed CPU baseline),
+never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+REF_LIB_PATH = HERE / "_ref" / "libref_config.so"
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        from world_amd._capi import WdConfig  # struct layout only
+
+        h = C.CDLL(str(LIB_PATH))
+        P = C.POINTER(WdConfig)
+        vp = C.c_void_p
+        h.or_initial_state.argtypes = [P, vp, vp, C.c_int, C.POINTER(C.c_int)]
+        h.or_initial_state.restype = C.c_int
+        h.or_cell_keys.argtypes = [P, C.c_int, vp, vp]
+        h.or_cell_keys.restype = None
+        h.or_cell_order.argtypes = [P, C.c_int, vp, vp, vp]
+        h.or_cell_order.restype = C.c_longlong
+        h.or_neighbor_rows.argtypes = [P, C.c_int, vp, vp]
+        h.or_neighbor_rows.restype = C.c_longlong
+        h.or_rhs.argtypes = [P, C.c_int, vp, vp, C.c_int, vp, vp]
+        h.or_rhs.restype = None
+        h.or_steps.argtypes = [P, C.c_int, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        h.or_steps.restype = C.c_longlong
+        h.or_energy.argtypes = [P, C.c_int, vp, vp, C.c_int]
+        h.or_energy.restype = C.c_double
+        h.or_grain_energy.argtypes = [P, C.c_int, vp, vp, vp]
+        h.or_grain_energy.restype = None
+        h.or_density_grid.argtypes = [P, C.c_int, vp, vp, C.c_int, vp]
+        h.or_density_grid.restype = None
+        _lib = h
+    return _lib
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def _st(state) -> np.ndarray:
+    return np.ascontiguousarray(state, np.float32).reshape(-1)
+
+
+def initial_state(cfg) -> tuple[np.ndarray, np.ndarray]:
+    n = C.c_int(0)
+    lib().or_initial_state(C.byref(cfg), None, None, 0, C.byref(n))
+    st = np.zeros(6 * max(n.value, 1), np.float32)
+    mat = np.zeros(max(n.value, 1), np.int32)
+    rc = lib().or_initial_state(C.byref(cfg), _p(st), _p(mat), n.value, C.byref(n))
+    assert rc == 0
+    return st[: 6 * n.value], mat[: n.value]
+
+
+def cell_keys(cfg, state) -> np.ndarray:
+    st = _st(state)
+    n = st.size // 6
+    out = np.zeros(max(n, 1), np.int64)
+    lib().or_cell_keys(C.byref(cfg), n, _p(st), _p(out))
+    return out[:n]
+
+
+def cell_order(cfg, state) -> tuple[np.ndarray, np.ndarray, int]:
+    st = _st(state)
+    n = st.size // 6
+    ids = np.zeros(max(n, 1), np.int32)
+    keys = np.zeros(max(n, 1), np.int64)
+    over = lib().or_cell_order(C.byref(cfg), n, _p(st), _p(ids), _p(keys))
+    return ids[:n], keys[:n], int(over)
+
+
+def neighbor_rows(cfg, state) -> tuple[np.ndarray, int]:
+    st = _st(state)
+    n = st.size // 6
+    row = 27 * cfg.particles_per_cell
+    out = np.full(max(n, 1) * row, -1, np.int32)
+    over = lib().or_neighbor_rows(C.byref(cfg), n, _p(st), _p(out))
+    return out[: n * row].reshape(n, row), int(over)
+
+
+def rhs(cfg, state, material, precision: int = 32) -> tuple[np.ndarray, np.ndarray]:
+    st = _st(state)
+    n = st.size // 6
+    mat = np.ascontiguousarray(material, np.int32)
+    out = np.zeros(6 * max(n, 1), np.float64)
+    scale = np.zeros(max(n, 1), np.float64)
+    lib().or_rhs(C.byref(cfg), n, _p(st), _p(mat), precision, _p(out), _p(scale))
+    return out[: 6 * n].reshape(n, 6), scale[:n]
+
+
+def steps(cfg, state, material, nsteps: int, threads: int = 1) -> tuple[np.ndarray, int, float]:
+    st = _st(state).copy()
+    n = st.size // 6
+    mat = np.ascontiguousarray(material, np.int32)
+    amax = C.c_float(0)
+    over = lib().or_steps(C.byref(cfg), n, _p(st), _p(mat), int(nsteps), int(threads), C.byref(amax))
+    return st.reshape(n, 6), int(over), amax.value
+
+
+def energy(cfg, state, material, precision: int = 32) -> float:
+    st = _st(state)
+    mat = np.ascontiguousarray(material, np.int32)
+    return float(lib().or_energy(C.byref(cfg), st.size // 6, _p(st), _p(mat), precision))
+
+
+def grain_energy(cfg, state, material) -> np.ndarray:
+    st = _st(state)
+    n = st.size // 6
+    mat = np.ascontiguousarray(material, np.int32)
+    out = np.zeros(max(n, 1), np.float32)
+    lib().or_grain_energy(C.byref(cfg), n, _p(st), _p(mat), _p(out))
+    return out[:n]
+
+
+def density_grid(cfg, state, material, precision: int = 32) -> np.ndarray:
+    st = _st(state)
+    n = st.size // 6
+    mat = np.ascontiguousarray(material, np.int32)
+    nodes = cfg.density_grid_size_x * cfg.density_grid_size_y * cfg.density_grid_size_z
+    out = np.zeros(4 * nodes, np.float64)
+    lib().or_density_grid(C.byref(cfg), n, _p(st), _p(mat), precision, _p(out))
+    return out
+
+
+# -- the reference's own YAML loader (built from the reference sources) ------
+
+class RefConfigView(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("grid", C.c_int * 3),
+        ("particles_per_cell", C.c_int),
+        ("threads_per_block", C.c_int),
+        ("density", C.c_int * 3),
+        ("domain", C.c_float * 6),
+        ("physics", C.c_float * 11),
+        ("num_masses", C.c_int),
+        ("masses", C.c_float * 16),
+        ("init_type", C.c_char * 64),
+        ("init_state_path", C.c_char * 512),
+        ("init", C.c_float * 10),
+        ("seeds", C.c_uint * 2),
+        ("cube", C.c_float * 3),
+        ("two_particle_separation", C.c_float),
+        ("particle_material", C.c_int),
+        ("time_integrator", C.c_char * 64),
+        ("picard_iterations", C.c_int),
+        ("kernel_timing", C.c_int),
+        ("profiling", C.c_int * 2),
+        ("dataset_sim_time", C.c_float),
+        ("dataset_checkpoints", C.c_int),
+        ("model_problem_sim_time", C.c_float),
+    ]
+
+
+def ref_loader_available() -> bool:
+    return REF_LIB_PATH.exists()
+
+
+def ref_load_config(path: str) -> RefConfigView:
+    h = C.CDLL(str(REF_LIB_PATH))
+    h.ref_load_config.argtypes = [C.c_char_p, C.POINTER(RefConfigView)]
+    h.ref_load_config.restype = C.c_int
+    out = RefConfigView()
+    h.ref_load_config(str(path).encode(), C.byref(out))
+    return out

@@ -1,0 +1,634 @@
+// CPU ORACLE -- test infrastructure only.
+//
+// A plain restatement of the reference's per-step algorithm, used by tests/,
+// by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the thing
+// the GPU engine is checked (and timed) against.  It is never linked into,
+// loaded by or dispatched to from the product library.
+//
+// Each function names the reference code it follows.  The FP32 paths keep
+// the reference's operation order exactly (compile with -ffp-contract=off).
+// Where the reference is nondeterministic (the in-cell order produced by its
+// atomics, src/find_neighbors_kernel.cu:61-112, and its float atomic sums,
+// src/density_grid_kernel.cu:72-80, src/energy_kernel.cu:80) the oracle uses
+// the canonical form: grains of a cell in ascending original index, sums in
+// double in ascending grain order.
+//
+// How each function is pinned (tests/test_oracle.py):
+//   or_initial_state  golden lattice fixtures; closed-form lattice spacing
+//   or_cell_*         brute-force recomputation; closed-form cell of a point
+//   or_neighbor_rows  O(N^2) brute-force stencil sets; overflow closed form
+//   or_rhs            two-body closed forms (spring, dashpot restitution,
+//                     friction cap), Newton's third law, FP64 variant
+//   or_steps          free-fall closed form, constant-velocity transport,
+//                     Picard == SIE under constant force
+//   or_energy         closed-form free-fall energy, piecewise contact energy
+//   or_density_grid   node / cell-centre / momentum closed forms, mass
+//                     conservation
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../include/world_amd.h"
+
+namespace {
+
+struct Grid {
+    int gx, gy, gz;
+    float lo[3], hi[3];
+};
+
+Grid grid_of(const wd_config& c) {
+    Grid g;
+    g.gx = c.collision_grid_size_x;
+    g.gy = c.collision_grid_size_y;
+    g.gz = c.collision_grid_size_z;
+    g.lo[0] = c.x_min; g.lo[1] = c.y_min; g.lo[2] = c.z_min;
+    g.hi[0] = c.x_max; g.hi[1] = c.y_max; g.hi[2] = c.z_max;
+    return g;
+}
+
+// static_cast<int>(float) as the GPU performs it: truncate, saturate, NaN -> 0.
+int to_int_sat(float t) {
+    if (std::isnan(t)) return 0;
+    if (t >= 2147483648.0f) return 2147483647;
+    if (t <= -2147483648.0f) return -2147483647 - 1;
+    return static_cast<int>(t);
+}
+
+// src/find_neighbors_kernel.cu:11-22
+int cell_of(float x, float lo, float hi, int g) {
+    int c = to_int_sat((x - lo) / (hi - lo) * static_cast<float>(g));
+    if (c < 0) c = 0;
+    if (c > g - 1) c = g - 1;
+    return c;
+}
+
+void cell3(const Grid& g, const float* p, int c[3]) {
+    c[0] = cell_of(p[0], g.lo[0], g.hi[0], g.gx);
+    c[1] = cell_of(p[1], g.lo[1], g.hi[1], g.gy);
+    c[2] = cell_of(p[2], g.lo[2], g.hi[2], g.gz);
+}
+
+// src/find_neighbors_kernel.cu:28-31
+long long flat(const Grid& g, int x, int y, int z) {
+    return (static_cast<long long>(x) * g.gy + y) * g.gz + z;
+}
+
+// Canonical cell table: members of each occupied cell in ascending index.
+struct Bins {
+    Grid g;
+    std::vector<int> first;   // dense per cell: offset into members, or -1
+    std::vector<int> count;   // dense per cell
+    std::vector<int> members;
+    long long over = 0;       // cells holding more than k grains
+};
+
+void build_bins(const wd_config& c, int n, const float* st, Bins& b) {
+    b.g = grid_of(c);
+    const size_t cells = static_cast<size_t>(b.g.gx) * b.g.gy * b.g.gz;
+    b.first.assign(cells, -1);
+    b.count.assign(cells, 0);
+    std::vector<long long> key(n);
+    for (int i = 0; i < n; ++i) {
+        int cc[3];
+        cell3(b.g, st + 6 * static_cast<size_t>(i), cc);
+        key[i] = flat(b.g, cc[0], cc[1], cc[2]);
+        b.count[key[i]]++;
+    }
+    int off = 0;
+    for (size_t k = 0; k < cells; ++k) {
+        if (b.count[k]) {
+            b.first[k] = off;
+            off += b.count[k];
+            if (b.count[k] > c.particles_per_cell) b.over++;
+        }
+    }
+    b.members.assign(n, 0);
+    std::vector<int> fill(cells, 0);
+    for (int i = 0; i < n; ++i) {  // ascending i -> ascending index in a cell
+        b.members[b.first[key[i]] + fill[key[i]]++] = i;
+    }
+}
+
+// Visits the canonical neighbor row of grain i: the 27 stencil cells in
+// dx -> dy -> dz order (src/find_neighbors_kernel.cu:130-158), each cell's
+// first k members, skipping i itself.
+template <typename F>
+void for_neighbors(const Bins& b, int k, const float* p, int i, F&& fn) {
+    int cc[3];
+    cell3(b.g, p, cc);
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int x = cc[0] + dx, y = cc[1] + dy, z = cc[2] + dz;
+                if (x < 0 || x >= b.g.gx || y < 0 || y >= b.g.gy || z < 0 || z >= b.g.gz) continue;
+                const long long f = flat(b.g, x, y, z);
+                const int cnt = b.count[f] < k ? b.count[f] : k;
+                for (int s = 0; s < cnt; ++s) {
+                    const int j = b.members[b.first[f] + s];
+                    if (j != i) fn(j);
+                }
+            }
+}
+
+// src/compute_rhs_kernel.cu:41-47 (evaluated in FP32 on the host).
+float damping32(float e, float k, float m) {
+    const float pi = 3.14159265358979323846f;
+    if (e <= 0.f) return 2.f * std::sqrt(k * m);
+    if (e >= 1.f) return 0.f;
+    const float ln_e = std::log(e);
+    return -2.f * std::sqrt(m * k) * ln_e / std::sqrt(pi * pi + ln_e * ln_e);
+}
+double damping64(double e, double k, double m) {
+    const double pi = 3.14159265358979323846;
+    if (e <= 0.0) return 2.0 * std::sqrt(k * m);
+    if (e >= 1.0) return 0.0;
+    const double ln_e = std::log(e);
+    return -2.0 * std::sqrt(m * k) * ln_e / std::sqrt(pi * pi + ln_e * ln_e);
+}
+
+struct Phys {
+    float g, r, fmax, kneg, k, e, mu;
+    float lower[3], upper[3];
+    std::vector<float> mass;
+};
+
+Phys phys_of(const wd_config& c) {
+    Phys p;
+    p.g = c.gravity;
+    p.r = c.particle_radius;
+    p.fmax = c.max_force;
+    if (c.normal_stiffness > 0.f) {
+        p.k = c.normal_stiffness;
+        p.kneg = -c.normal_stiffness;
+    } else {
+        p.k = c.max_force / c.particle_radius;
+        p.kneg = -c.max_force / c.particle_radius;
+    }
+    p.e = c.restitution;
+    p.mu = c.friction;
+    p.lower[0] = c.wall_x_min; p.lower[1] = c.wall_y_min; p.lower[2] = c.floor_z;
+    p.upper[0] = c.wall_x_max; p.upper[1] = c.wall_y_max; p.upper[2] = c.ceiling_z;
+    p.mass.assign(c.masses, c.masses + c.num_masses);
+    return p;
+}
+
+// True when fminf(a, b) returns a: b is NaN, or a < b with a not NaN.
+template <typename T>
+bool pick_first_min(T a, T b) {
+    if (std::isnan(b)) return true;
+    if (std::isnan(a)) return false;
+    return a < b;
+}
+
+// Branch decisions of one FP32 evaluation, replayed by the FP64 one so that
+// a contact / clamp / friction branch flip does not count as a discrepancy.
+struct Branches {
+    std::vector<uint8_t> bits;
+    size_t pos = 0;
+    bool recording = true;
+    bool take(bool v) {
+        if (recording) {
+            bits.push_back(v ? 1 : 0);
+            return v;
+        }
+        return bits[pos++] != 0;
+    }
+};
+
+// src/compute_rhs_kernel.cu:66-78
+template <typename T>
+void friction(T f[3], const T vt[3], T limit, T c, Branches* br) {
+    T s = 0;
+    for (int a = 0; a < 3; ++a) s += vt[a] * vt[a];
+    const T n = std::sqrt(s);
+    const bool stop = br ? br->take(n < T(1e-8f)) : (n < T(1e-8f));
+    if (stop) return;
+    // fminf(limit, c*n): a NaN operand yields the other one.
+    const T cn = c * n;
+    const bool lim = br ? br->take(pick_first_min(limit, cn)) : pick_first_min(limit, cn);
+    const T mag = lim ? limit : cn;
+    for (int a = 0; a < 3; ++a) f[a] -= mag * vt[a] / n;
+}
+
+// max(0, min(fmax, kneg * d)) with fminf/fmaxf semantics (a NaN operand
+// yields the other one) and the branch choices recorded / replayed.
+template <typename T>
+T spring(T fmax, T kneg, T d, Branches* br) {
+    const T raw = kneg * d;
+    const bool capped = br ? br->take(pick_first_min(fmax, raw)) : pick_first_min(fmax, raw);
+    const T inner = capped ? fmax : raw;
+    const bool neg = br ? br->take(!(inner > T(0))) : !(inner > T(0));
+    return neg ? T(0) : inner;
+}
+
+// Force on grain i (src/compute_rhs_kernel.cu:81-212).  T = float follows the
+// reference's FP32 order; T = double replays the FP32 branch decisions.
+// `scale` receives sum |f_term| over every force term (tolerance scale).
+template <typename T>
+void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const float* st,
+                 const int* mat, int i, T out_a[3], Branches* br, double* scale) {
+    (void)n;
+    const float* si = st + 6 * static_cast<size_t>(i);
+    T pos[3], vel[3];
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = static_cast<T>(si[a]);
+        vel[a] = static_cast<T>(si[3 + a]);
+    }
+    const int mi = mat[i];
+    const T m_i = static_cast<T>(ph.mass[mi]);
+    const T kk = static_cast<T>(ph.k);
+    const T fmax = static_cast<T>(ph.fmax);
+    const T kneg = static_cast<T>(ph.kneg);
+    const T r = static_cast<T>(ph.r);
+    const T mu = static_cast<T>(ph.mu);
+    T c_wall;
+    if (sizeof(T) == 4) c_wall = static_cast<T>(damping32(ph.e, ph.k, ph.mass[mi]));
+    else c_wall = static_cast<T>(damping64(ph.e, kk, m_i));
+    T f[3] = {0, 0, 0};
+    double sc = 0.0;
+    f[2] -= m_i * static_cast<T>(ph.g);
+    sc += std::fabs(static_cast<double>(m_i * static_cast<T>(ph.g)));
+    for (int a = 0; a < 3; ++a) {
+        T vt[3];
+        for (int q = 0; q < 3; ++q) vt[q] = (q == a) ? T(0) : vel[q];
+        const T dl = pos[a] - static_cast<T>(ph.lower[a]) - r;
+        const T fl = spring(fmax, kneg, dl, br);
+        f[a] += fl;
+        sc += std::fabs(static_cast<double>(fl));
+        const bool cl = br ? br->take(dl < T(0)) : (dl < T(0));
+        if (cl) {
+            f[a] -= c_wall * vel[a];
+            friction(f, vt, mu * fl, c_wall, br);
+            sc += std::fabs(static_cast<double>(c_wall * vel[a])) + std::fabs(static_cast<double>(mu * fl));
+        }
+        const T du = static_cast<T>(ph.upper[a]) - pos[a] - r;
+        const T fu = spring(fmax, kneg, du, br);
+        f[a] -= fu;
+        sc += std::fabs(static_cast<double>(fu));
+        const bool cu = br ? br->take(du < T(0)) : (du < T(0));
+        if (cu) {
+            f[a] -= c_wall * vel[a];
+            friction(f, vt, mu * fu, c_wall, br);
+            sc += std::fabs(static_cast<double>(c_wall * vel[a])) + std::fabs(static_cast<double>(mu * fu));
+        }
+    }
+    for_neighbors(b, c.particles_per_cell, si, i, [&](int j) {
+        const float* sj = st + 6 * static_cast<size_t>(j);
+        T d[3];
+        T s = 0;
+        for (int a = 0; a < 3; ++a) {
+            d[a] = pos[a] - static_cast<T>(sj[a]);
+            s += d[a] * d[a];
+        }
+        const T norm = std::sqrt(s);
+        const T dist = norm - r - r;
+        T nrm[3];
+        for (int a = 0; a < 3; ++a) nrm[a] = d[a] / norm;
+        const T fp = spring(fmax, kneg, dist, br);
+        for (int a = 0; a < 3; ++a) f[a] += fp * nrm[a];
+        sc += std::fabs(static_cast<double>(fp));
+        const bool contact = br ? br->take(dist < T(0)) : (dist < T(0));
+        if (contact) {
+            const int mj = mat[j];
+            T cp;
+            if (sizeof(T) == 4) {
+                const float msum = ph.mass[mi] + ph.mass[mj];
+                const float mred = (msum > 0.f) ? (ph.mass[mi] * ph.mass[mj]) / msum : 0.f;
+                cp = static_cast<T>(damping32(ph.e, ph.k, mred));
+            } else {
+                const double a_ = ph.mass[mi], b_ = ph.mass[mj];
+                const double msum = a_ + b_;
+                cp = static_cast<T>(damping64(ph.e, kk, (msum > 0) ? a_ * b_ / msum : 0.0));
+            }
+            T vr[3], vn = 0;
+            for (int a = 0; a < 3; ++a) {
+                vr[a] = vel[a] - static_cast<T>(sj[3 + a]);
+                vn += vr[a] * nrm[a];
+            }
+            T vt[3];
+            for (int a = 0; a < 3; ++a) {
+                f[a] -= cp * vn * nrm[a];
+                vt[a] = vr[a] - vn * nrm[a];
+            }
+            friction(f, vt, mu * fp, cp, br);
+            sc += std::fabs(static_cast<double>(cp * vn)) + std::fabs(static_cast<double>(mu * fp));
+        }
+    });
+    for (int a = 0; a < 3; ++a) out_a[a] = f[a] / static_cast<T>(ph.mass[mi]);
+    if (scale) *scale = sc / static_cast<double>(ph.mass[mi]);
+}
+
+// src/energy_kernel.cu:22-28
+template <typename T>
+T contact_u(T delta, T r, T fmax, T k, T clamp) {
+    if (delta <= T(0)) return T(0);
+    if (delta <= clamp) return T(0.5f) * k * delta * delta;
+    return T(0.5f) * fmax * clamp + fmax * (delta - clamp);
+    (void)r;
+}
+
+// src/energy_kernel.cu:30-81 for one grain; pairs counted from the lower index.
+template <typename T>
+T grain_energy(const wd_config& c, const Phys& ph, const Bins& b, const float* st, const int* mat,
+               int i) {
+    const float* si = st + 6 * static_cast<size_t>(i);
+    const T r = static_cast<T>(ph.r), fmax = static_cast<T>(ph.fmax), k = static_cast<T>(ph.k);
+    const T clamp = c.normal_stiffness > 0.f ? static_cast<T>(c.max_force / c.normal_stiffness) : r;
+    T total = 0;
+    T s = 0;
+    for (int a = 0; a < 3; ++a) s += static_cast<T>(si[3 + a]) * static_cast<T>(si[3 + a]);
+    const T m = static_cast<T>(ph.mass[mat[i]]);
+    total += T(0.5f) * m * s;
+    total += m * static_cast<T>(ph.g) * (static_cast<T>(si[2]) - static_cast<T>(ph.lower[2]));
+    for (int a = 0; a < 3; ++a) {
+        const T p = static_cast<T>(si[a]);
+        total += contact_u<T>(r - (p - static_cast<T>(ph.lower[a])), r, fmax, k, clamp);
+        total += contact_u<T>(r - (static_cast<T>(ph.upper[a]) - p), r, fmax, k, clamp);
+    }
+    for_neighbors(b, c.particles_per_cell, si, i, [&](int j) {
+        if (j <= i) return;
+        const float* sj = st + 6 * static_cast<size_t>(j);
+        T d2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            const T d = static_cast<T>(si[a]) - static_cast<T>(sj[a]);
+            d2 += d * d;
+        }
+        total += contact_u<T>(T(2.f) * r - std::sqrt(d2), r, fmax, k, clamp);
+    });
+    return total;
+}
+
+// src/density_grid_kernel.cu:25-41
+template <typename T>
+void axis_w(T p, T lo, T hi, int size, int& i0, int& i1, T& w0, T& w1) {
+    if (size <= 1) {
+        i0 = 0; i1 = 0; w0 = 1; w1 = 0;
+        return;
+    }
+    const T h = (hi - lo) / static_cast<T>(size - 1);
+    const T u = (p - lo) / h;
+    const T base = std::floor(u);
+    const T frac = u - base;
+    const long long bb = sizeof(T) == 4 ? to_int_sat(static_cast<float>(base))
+                                        : static_cast<long long>(std::isnan(base) ? 0 : base);
+    auto clampi = [size](long long v) { return static_cast<int>(v < 0 ? 0 : (v > size - 1 ? size - 1 : v)); };
+    i0 = clampi(bb);
+    i1 = clampi(bb + 1);
+    w0 = T(1) - frac;
+    w1 = frac;
+}
+
+}  // namespace
+
+extern "C" {
+
+// initialize_to_cube / two_particles / single_particle
+// (src/particle_dynamics.cu:128-209); file loading is the caller's business.
+int or_initial_state(const wd_config* c, float* st, int* mat, int cap, int* n_out) {
+    const float r = c->particle_radius;
+    if (c->init_type == WD_INIT_TWO_PARTICLES) {
+        if (cap < 2) return -1;
+        const float s[12] = {c->init_x0, c->init_y0, c->init_z0, 0, 0, 0,
+                             c->init_x0, c->init_y0, c->init_z0 + c->two_particle_separation, 0, 0, 0};
+        std::memcpy(st, s, sizeof(s));
+        mat[0] = mat[1] = c->particle_material;
+        *n_out = 2;
+        return 0;
+    }
+    if (c->init_type == WD_INIT_SINGLE_PARTICLE) {
+        if (cap < 1) return -1;
+        const float s[6] = {c->init_x0, c->init_y0, c->init_z0, c->init_vx0, c->init_vy0, c->init_vz0};
+        std::memcpy(st, s, sizeof(s));
+        mat[0] = c->particle_material;
+        *n_out = 1;
+        return 0;
+    }
+    const float L[3] = {c->cube_length_x, c->cube_length_y, c->cube_length_z};
+    int side[3];
+    for (int a = 0; a < 3; ++a) side[a] = static_cast<int>(L[a] / (2 * r));
+    const int n = side[0] * side[1] * side[2];
+    *n_out = n;
+    if (n > cap) return -1;
+    std::mt19937 rj(c->init_seed), rp(c->init_perturbation_seed);
+    std::uniform_real_distribution<float> uj(-c->init_jitter * r, c->init_jitter * r);
+    std::uniform_real_distribution<float> up(-c->init_perturbation * r, c->init_perturbation * r);
+    std::normal_distribution<float> gp(0.f, c->init_perturbation * r);
+    const float o[3] = {c->init_x0, c->init_y0, c->init_z0};
+    const float v[3] = {c->init_vx0, c->init_vy0, c->init_vz0};
+    int p = 0;
+    for (int i = 0; i < side[0]; ++i)
+        for (int j = 0; j < side[1]; ++j)
+            for (int k = 0; k < side[2]; ++k, ++p) {
+                const int l[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    const float dj = c->init_jitter > 0.f ? uj(rj) : 0.f;
+                    float dp = 0.f;
+                    if (c->init_perturbation > 0.f) {
+                        dp = c->perturbation_distribution == WD_PERTURB_GAUSSIAN ? gp(rp) : up(rp);
+                    }
+                    st[6 * p + a] = o[a] + static_cast<float>(l[a]) * 2.f * c->lattice_spacing * r + dj + dp;
+                    st[6 * p + 3 + a] = v[a];
+                }
+                mat[p] = c->particle_material;
+            }
+    return 0;
+}
+
+// Flat cell index of every grain (particle_cell, src/find_neighbors_kernel.cu:35-40).
+void or_cell_keys(const wd_config* c, int n, const float* st, long long* keys) {
+    const Grid g = grid_of(*c);
+    for (int i = 0; i < n; ++i) {
+        int cc[3];
+        cell3(g, st + 6 * static_cast<size_t>(i), cc);
+        keys[i] = flat(g, cc[0], cc[1], cc[2]);
+    }
+}
+
+// Canonical cell order: ids sorted by (flat cell, original index).
+long long or_cell_order(const wd_config* c, int n, const float* st, int* ids, long long* keys) {
+    Bins b;
+    build_bins(*c, n, st, b);
+    for (int p = 0; p < n; ++p) {
+        ids[p] = b.members[p];
+        int cc[3];
+        cell3(b.g, st + 6 * static_cast<size_t>(ids[p]), cc);
+        keys[p] = flat(b.g, cc[0], cc[1], cc[2]);
+    }
+    return b.over;
+}
+
+// Neighbor rows in the reference's flat -1-terminated format
+// (src/find_neighbors_kernel.cu:119-161); returns the over-capacity cell count.
+long long or_neighbor_rows(const wd_config* c, int n, const float* st, int* out) {
+    Bins b;
+    build_bins(*c, n, st, b);
+    const size_t row = static_cast<size_t>(27) * c->particles_per_cell;
+    for (int i = 0; i < n; ++i) {
+        size_t w = 0;
+        int* r = out + row * i;
+        for_neighbors(b, c->particles_per_cell, st + 6 * static_cast<size_t>(i), i,
+                      [&](int j) { if (w + 1 < row) r[w++] = j; });
+        r[w] = -1;
+    }
+    return b.over;
+}
+
+// rhs = [v, F/m] (src/compute_rhs_kernel.cu:81-212).  precision 32: FP32 in
+// the reference order.  precision 64: FP64 with the FP32 branch decisions;
+// `scale` (may be null) gets the per-grain sum of |force term| / m.
+void or_rhs(const wd_config* c, int n, const float* st, const int* mat, int precision,
+            double* rhs, double* scale) {
+    Bins b;
+    build_bins(*c, n, st, b);
+    const Phys ph = phys_of(*c);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        double* r = rhs + 6 * static_cast<size_t>(i);
+        for (int a = 0; a < 3; ++a) r[a] = st[6 * static_cast<size_t>(i) + 3 + a];
+        if (precision == 32) {
+            float acc[3];
+            grain_force<float>(*c, ph, b, n, st, mat, i, acc, nullptr, nullptr);
+            for (int a = 0; a < 3; ++a) r[3 + a] = acc[a];
+        } else {
+            Branches br;
+            float acc32[3];
+            double sc = 0.0;
+            grain_force<float>(*c, ph, b, n, st, mat, i, acc32, &br, &sc);
+            br.recording = false;
+            double acc[3];
+            grain_force<double>(*c, ph, b, n, st, mat, i, acc, &br, nullptr);
+            for (int a = 0; a < 3; ++a) r[3 + a] = acc[a];
+            if (scale) scale[i] = sc;
+        }
+    }
+}
+
+// nsteps of take_step (src/particle_dynamics.cu:264-281) in FP32 on the
+// original-order state, in place.  Returns the over-capacity cell count
+// summed over every neighbor pass; *max_accel gets max |a| of the last force
+// pass (src/particle_dynamics.cu:328-344).
+long long or_steps(const wd_config* c, int n, float* st, const int* mat, int nsteps, int threads,
+                   float* max_accel) {
+    const Phys ph = phys_of(*c);
+    const float dt = c->dt;
+    long long over = 0;
+    std::vector<float> acc(3 * static_cast<size_t>(n));
+    std::vector<float> base;
+    const bool picard = c->time_integrator == WD_BACKWARD_EULER_PICARD;
+    float amax = 0.f;
+    auto force_pass = [&]() {
+        Bins b;
+        build_bins(*c, n, st, b);
+        over += b.over;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+        for (int i = 0; i < n; ++i) {
+            grain_force<float>(*c, ph, b, n, st, mat, i, &acc[3 * static_cast<size_t>(i)], nullptr, nullptr);
+        }
+        amax = 0.f;
+        for (int i = 0; i < n; ++i) {
+            float s = 0.f;
+            for (int a = 0; a < 3; ++a) s += acc[3 * i + a] * acc[3 * i + a];
+            const float m = std::sqrt(s);
+            if (m > amax) amax = m;
+        }
+    };
+    for (int step = 0; step < nsteps; ++step) {
+        if (!picard) {
+            force_pass();
+            for (int i = 0; i < n; ++i) {
+                float* s = st + 6 * static_cast<size_t>(i);
+                for (int a = 0; a < 3; ++a) s[3 + a] += dt * acc[3 * i + a];
+                for (int a = 0; a < 3; ++a) s[a] += dt * s[3 + a];
+            }
+        } else {
+            base.assign(st, st + 6 * static_cast<size_t>(n));
+            for (int k = 0; k < c->picard_iterations; ++k) {
+                force_pass();
+                for (int i = 0; i < n; ++i) {
+                    float* s = st + 6 * static_cast<size_t>(i);
+                    const float* s0 = base.data() + 6 * static_cast<size_t>(i);
+                    for (int a = 0; a < 3; ++a) {
+                        const float vn = s0[3 + a] + dt * acc[3 * i + a];
+                        s[3 + a] = vn;
+                        s[a] = s0[a] + dt * vn;
+                    }
+                }
+            }
+        }
+    }
+    if (max_accel) *max_accel = amax;
+    return over;
+}
+
+// Total energy (src/energy_kernel.cu:30-81).  precision 32: per-grain FP32
+// totals in the reference order, summed in double by ascending index.
+// precision 64: every term in double.
+double or_energy(const wd_config* c, int n, const float* st, const int* mat, int precision) {
+    Bins b;
+    build_bins(*c, n, st, b);
+    const Phys ph = phys_of(*c);
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) {
+        total += precision == 32 ? static_cast<double>(grain_energy<float>(*c, ph, b, st, mat, i))
+                                 : grain_energy<double>(*c, ph, b, st, mat, i);
+    }
+    return total;
+}
+
+// Energy of each grain (FP32 reference order), for per-grain checks.
+void or_grain_energy(const wd_config* c, int n, const float* st, const int* mat, float* out) {
+    Bins b;
+    build_bins(*c, n, st, b);
+    const Phys ph = phys_of(*c);
+    for (int i = 0; i < n; ++i) out[i] = grain_energy<float>(*c, ph, b, st, mat, i);
+}
+
+// CIC mass + momentum deposit (src/density_grid_kernel.cu:43-106), channel
+// major [4][nx][ny][nz].  precision 32: FP32 weights and products as the
+// reference forms them, summed in double.  precision 64: all in double.
+void or_density_grid(const wd_config* c, int n, const float* st, const int* mat, int precision,
+                     double* out) {
+    const int sx = c->density_grid_size_x, sy = c->density_grid_size_y, sz = c->density_grid_size_z;
+    const size_t nodes = static_cast<size_t>(sx) * sy * sz;
+    std::memset(out, 0, sizeof(double) * 4 * nodes);
+    const float lo[3] = {c->x_min, c->y_min, c->z_min}, hi[3] = {c->x_max, c->y_max, c->z_max};
+    const int size[3] = {sx, sy, sz};
+    for (int i = 0; i < n; ++i) {
+        const float* s = st + 6 * static_cast<size_t>(i);
+        const float m = c->masses[mat[i]];
+        if (m == 0.f) continue;
+        int idx[3][2];
+        if (precision == 32) {
+            float w[3][2];
+            for (int a = 0; a < 3; ++a) axis_w<float>(s[a], lo[a], hi[a], size[a], idx[a][0], idx[a][1], w[a][0], w[a][1]);
+            const float mom[3] = {m * s[3], m * s[4], m * s[5]};
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y)
+                    for (int z = 0; z < 2; ++z) {
+                        const float ww = w[0][x] * w[1][y] * w[2][z];
+                        if (ww == 0.f) continue;
+                        const size_t node = (static_cast<size_t>(idx[0][x]) * sy + idx[1][y]) * sz + idx[2][z];
+                        out[node] += static_cast<double>(m * ww);
+                        for (int a = 0; a < 3; ++a) out[(a + 1) * nodes + node] += static_cast<double>(mom[a] * ww);
+                    }
+        } else {
+            double w[3][2];
+            for (int a = 0; a < 3; ++a) axis_w<double>(s[a], lo[a], hi[a], size[a], idx[a][0], idx[a][1], w[a][0], w[a][1]);
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y)
+                    for (int z = 0; z < 2; ++z) {
+                        const double ww = w[0][x] * w[1][y] * w[2][z];
+                        if (ww == 0.0) continue;
+                        const size_t node = (static_cast<size_t>(idx[0][x]) * sy + idx[1][y]) * sz + idx[2][z];
+                        out[node] += static_cast<double>(m) * ww;
+                        for (int a = 0; a < 3; ++a) out[(a + 1) * nodes + node] += static_cast<double>(m) * s[3 + a] * ww;
+                    }
+        }
+    }
+}
+
+}  // extern "C"

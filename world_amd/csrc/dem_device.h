@@ -1,0 +1,215 @@
+// Per-grain physics shared by every gfx950 kernel of the engine.
+//
+// Each function restates one piece of the reference's per-thread arithmetic
+// in the same FP32 operation order (the build compiles with
+// -ffp-contract=off and correctly rounded division and square root, so the
+// results are reproducible bit for bit by a CPU evaluation of the same order):
+//   cell_axis          src/find_neighbors_kernel.cu:11-22
+//   wall_contacts      src/compute_rhs_kernel.cu:110-141
+//   pair_contact       src/compute_rhs_kernel.cu:145-197
+//   viscous_friction   src/compute_rhs_kernel.cu:66-78
+//   contact_energy     src/energy_kernel.cu:22-28
+// The restitution-to-damping constants (src/compute_rhs_kernel.cu:41-47)
+// depend only on material pairs, so they are evaluated once on the host and
+// read here from a table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/world_amd.h"
+
+namespace wd {
+
+constexpr int kMaxMat = WD_MAX_MATERIALS;
+
+// Device-resident solver constants (one copy per handle, read through a
+// pointer so that dt can change without re-capturing the step graph).
+struct Params {
+    int n;
+    int gx, gy, gz;        // collision grid
+    int kcap;              // particles_per_cell
+    int nmat;
+    float lo[3], hi[3];    // domain (cell mapping, CIC)
+    float wlo[3], whi[3];  // walls: x_min, y_min, floor / x_max, y_max, ceiling
+    float gravity, radius, max_force;
+    float kneg;            // -max_force / radius (or -normal_stiffness)
+    float spring_k;        // k of the energy's quadratic branch
+    float clamp_delta;     // overlap where the force clamp engages
+    float mu;
+    float dt;
+    int dgx, dgy, dgz;     // density grid
+    float mass[kMaxMat];
+    float c_wall[kMaxMat];
+    float c_pair[kMaxMat * kMaxMat];
+};
+
+// Saturating float -> int truncation (NaN -> 0), the conversion the
+// reference's static_cast<int> performs on the GPU.
+__device__ __forceinline__ int sat_f2i(float t) {
+    if (!(t == t)) return 0;
+    if (t >= 2147483648.0f) return 2147483647;
+    if (t <= -2147483648.0f) return (-2147483647 - 1);
+    return static_cast<int>(t);
+}
+
+__device__ __forceinline__ int cell_axis(float x, float lo, float hi, int g) {
+    const float t = (x - lo) / (hi - lo) * static_cast<float>(g);
+    const int c = sat_f2i(t);
+    return min(g - 1, max(0, c));
+}
+
+// Packed cell key: cx in bits 21..31, cy in 10..20, cz in 0..9.
+__device__ __forceinline__ uint32_t pack_cell(int cx, int cy, int cz) {
+    return (static_cast<uint32_t>(cx) << 21) | (static_cast<uint32_t>(cy) << 10) |
+           static_cast<uint32_t>(cz);
+}
+__device__ __forceinline__ void unpack_cell(uint32_t k, int& cx, int& cy, int& cz) {
+    cx = static_cast<int>(k >> 21);
+    cy = static_cast<int>((k >> 10) & 0x7FFu);
+    cz = static_cast<int>(k & 0x3FFu);
+}
+
+__device__ __forceinline__ void grain_cell(const Params& P, float x, float y, float z,
+                                           int& cx, int& cy, int& cz) {
+    cx = cell_axis(x, P.lo[0], P.hi[0], P.gx);
+    cy = cell_axis(y, P.lo[1], P.hi[1], P.gy);
+    cz = cell_axis(z, P.lo[2], P.hi[2], P.gz);
+}
+
+// Sub-grid covering the occupied cells' bounding box with one empty cell of
+// padding on every side, so every 27-cell stencil of an occupied cell is
+// addressable without bounds checks.
+struct SubGrid {
+    int ox, oy, oz;
+    int bx, by, bz;
+    __device__ __forceinline__ int index(int cx, int cy, int cz) const {
+        return ((cx - ox) * by + (cy - oy)) * bz + (cz - oz);
+    }
+    __device__ __forceinline__ int cells() const { return bx * by * bz; }
+};
+
+__device__ __forceinline__ SubGrid make_subgrid(const int* bbox) {
+    SubGrid s;
+    s.ox = bbox[0] - 1;
+    s.oy = bbox[1] - 1;
+    s.oz = bbox[2] - 1;
+    s.bx = bbox[3] - bbox[0] + 3;
+    s.by = bbox[4] - bbox[1] + 3;
+    s.bz = bbox[5] - bbox[2] + 3;
+    return s;
+}
+
+// Reference viscous Coulomb friction: oppose the tangential velocity,
+// magnitude min(limit, c * |v_t|); no force below |v_t| = 1e-8.
+__device__ __forceinline__ void viscous_friction(float f[3], const float vt[3], float limit,
+                                                 float c) {
+    float s = 0.f;
+    s += vt[0] * vt[0];
+    s += vt[1] * vt[1];
+    s += vt[2] * vt[2];
+    const float n = sqrtf(s);
+    if (n < 1e-8f) return;
+    const float mag = fminf(limit, c * n);
+    f[0] -= mag * vt[0] / n;
+    f[1] -= mag * vt[1] / n;
+    f[2] -= mag * vt[2] / n;
+}
+
+// Gravity and the six axis-aligned walls, lower then upper per axis.
+__device__ __forceinline__ void wall_contacts(const Params& P, const float x[3], const float v[3],
+                                              float m_i, float c_wall, float f[3]) {
+    f[2] -= m_i * P.gravity;
+    for (int a = 0; a < 3; ++a) {
+        float vt[3];
+        for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
+        const float dl = x[a] - P.wlo[a] - P.radius;
+        const float fl = fmaxf(0.f, fminf(P.max_force, P.kneg * dl));
+        f[a] += fl;
+        if (dl < 0.f) {
+            f[a] -= c_wall * v[a];
+            viscous_friction(f, vt, P.mu * fl, c_wall);
+        }
+        const float du = P.whi[a] - x[a] - P.radius;
+        const float fu = fmaxf(0.f, fminf(P.max_force, P.kneg * du));
+        f[a] -= fu;
+        if (du < 0.f) {
+            f[a] -= c_wall * v[a];
+            viscous_friction(f, vt, P.mu * fu, c_wall);
+        }
+    }
+}
+
+// Centre distance |x_i - x_j| with the reference's summation order; also
+// the separation vector.
+__device__ __forceinline__ float pair_norm(const float xi[3], float xj0, float xj1, float xj2,
+                                           float& d0, float& d1, float& d2) {
+    d0 = xi[0] - xj0;
+    d1 = xi[1] - xj1;
+    d2 = xi[2] - xj2;
+    float s = 0.f;
+    s += d0 * d0;
+    s += d1 * d1;
+    s += d2 * d2;
+    return sqrtf(s);
+}
+
+// One neighbor j (cell-sorted slot q) acting on grain i, given the
+// separation d and its norm.  A pair beyond contact adds a zero spring force,
+// as in the reference.
+__device__ __forceinline__ void pair_contact(const Params& P, const float vi[3], int mat_i,
+                                             float d0, float d1, float d2, float norm,
+                                             const float4* __restrict__ vel, int q, float f[3]) {
+    const float dist = norm - P.radius - P.radius;
+    const float n0 = d0 / norm, n1 = d1 / norm, n2 = d2 / norm;
+    const float fp = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
+    f[0] += fp * n0;
+    f[1] += fp * n1;
+    f[2] += fp * n2;
+    if (dist < 0.f) {
+        const float4 vj = vel[q];
+        const int mat_j = __float_as_int(vj.w);
+        const float c = P.c_pair[mat_i * kMaxMat + mat_j];
+        const float r0 = vi[0] - vj.x, r1 = vi[1] - vj.y, r2 = vi[2] - vj.z;
+        float vn = 0.f;
+        vn += r0 * n0;
+        vn += r1 * n1;
+        vn += r2 * n2;
+        float vt[3];
+        f[0] -= c * vn * n0;
+        vt[0] = r0 - vn * n0;
+        f[1] -= c * vn * n1;
+        vt[1] = r1 - vn * n1;
+        f[2] -= c * vn * n2;
+        vt[2] = r2 - vn * n2;
+        viscous_friction(f, vt, P.mu * fp, c);
+    }
+}
+
+// Elastic energy of one contact of overlap delta: quadratic up to the clamp,
+// linear beyond it.
+__device__ __forceinline__ float contact_energy(const Params& P, float delta) {
+    if (delta <= 0.f) return 0.f;
+    if (delta <= P.clamp_delta) return 0.5f * P.spring_k * delta * delta;
+    return 0.5f * P.max_force * P.clamp_delta + P.max_force * (delta - P.clamp_delta);
+}
+
+// Kinetic + gravitational + wall elastic energy of one grain (pair terms are
+// added by the caller in neighbor order).
+__device__ __forceinline__ float grain_energy_self(const Params& P, const float x[3],
+                                                   const float v[3], float m) {
+    float total = 0.f;
+    float s = 0.f;
+    s += v[0] * v[0];
+    s += v[1] * v[1];
+    s += v[2] * v[2];
+    total += 0.5f * m * s;
+    total += m * P.gravity * (x[2] - P.wlo[2]);
+    for (int a = 0; a < 3; ++a) {
+        total += contact_energy(P, P.radius - (x[a] - P.wlo[a]));
+        total += contact_energy(P, P.radius - (P.whi[a] - x[a]));
+    }
+    return total;
+}
+
+}  // namespace wd

@@ -1,0 +1,843 @@
+// Host side of the engine: one wd_sim per simulation, its device buffers,
+// its HIP stream, graph replay of the step, and the C-ABI entry points.
+//
+// Construction, state access and diagnostics follow the reference solver
+// object (src/particle_dynamics.cu:10-360); the step pipeline is the one
+// described in kernels.hip.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/world_amd.h"
+#include "capi_util.h"
+#include "kernels.h"
+
+// The kernels are compiled in this translation unit.
+#include "kernels.hip"
+
+using namespace wd;
+
+namespace {
+
+#define WD_HIP(expr)                                                                        \
+    do {                                                                                    \
+        const hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                             \
+            return wd::fail(WD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                   \
+    } while (0)
+
+// Host evaluation of the restitution -> dashpot relation
+// (src/compute_rhs_kernel.cu:41-47), in the reference's FP32 order.
+float damping(float e, float k, float m) {
+    const float pi = 3.14159265358979323846f;
+    if (e <= 0.f) return 2.f * std::sqrt(k * m);
+    if (e >= 1.f) return 0.f;
+    const float ln_e = std::log(e);
+    return -2.f * std::sqrt(m * k) * ln_e / std::sqrt(pi * pi + ln_e * ln_e);
+}
+
+struct TimedStage {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct wd_sim {
+    wd_config cfg{};
+    int n = 0;
+    float time = 0.f;
+    hipStream_t stream = nullptr;
+    Params hp{};
+    Params* dp = nullptr;
+    Ctl* ctl = nullptr;
+    float4 *Tp = nullptr, *Tv = nullptr, *Sp = nullptr, *Sv = nullptr;
+    float4* NP[2] = {nullptr, nullptr};
+    float4* NV[2] = {nullptr, nullptr};
+    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
+    int* tmp_id = nullptr;
+    unsigned long long* status = nullptr;
+    size_t scan_cells = 0, ntiles_max = 0;
+    float* st6 = nullptr;
+    int* mat_d = nullptr;
+    int nblocks = 0;
+    double* partials = nullptr;
+    double* energy_log = nullptr;
+    int energy_cap = 0;
+    bool energy_on = false;
+    unsigned long long* cic_acc = nullptr;
+    float* cic_out = nullptr;
+    size_t cic_count = 0;
+    float* rhs_buf = nullptr;
+    float* cic_log = nullptr;
+    int cic_log_cap = 0, cic_log_count = 0;
+    struct ProfRec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<ProfRec>* prof = nullptr;  // set while wd_profile_step runs
+    hipGraphExec_t graph_one = nullptr, graph_chunk = nullptr;
+    int chunk = 16;
+    bool timing = false;
+    float stage_ms[4] = {0.f, 0.f, 0.f, 0.f};
+    std::vector<TimedStage> pending;
+    std::vector<hipEvent_t> spare;
+
+    ~wd_sim() { release(); }
+
+    void release() {
+        if (graph_one) (void)hipGraphExecDestroy(graph_one);
+        if (graph_chunk) (void)hipGraphExecDestroy(graph_chunk);
+        graph_one = graph_chunk = nullptr;
+        for (auto& t : pending) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        pending.clear();
+        for (auto e : spare) (void)hipEventDestroy(e);
+        spare.clear();
+        void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], key, slot, count,
+                        start, tmp_id, status, st6, mat_d, partials, energy_log, cic_acc,
+                        cic_out, rhs_buf, cic_log};
+        for (void* b : bufs) {
+            if (b) (void)hipFree(b);
+        }
+        dp = nullptr; ctl = nullptr; Tp = Tv = Sp = Sv = nullptr;
+        NP[0] = NP[1] = NV[0] = NV[1] = nullptr;
+        key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
+        st6 = nullptr; mat_d = nullptr; partials = nullptr; energy_log = nullptr;
+        cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    bool picard() const { return cfg.time_integrator == WD_BACKWARD_EULER_PICARD; }
+
+    hipEvent_t take_event() {
+        if (!spare.empty()) {
+            hipEvent_t e = spare.back();
+            spare.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+
+    // Opens a timed stage on the stream (only when kernel timing is on).
+    hipEvent_t stage_begin() {
+        if (!timing) return nullptr;
+        hipEvent_t a = take_event();
+        (void)hipEventRecord(a, stream);
+        return a;
+    }
+    void stage_end(int stage, hipEvent_t a) {
+        if (!timing || !a) return;
+        hipEvent_t b = take_event();
+        (void)hipEventRecord(b, stream);
+        pending.push_back({stage, a, b});
+        if (pending.size() > 8192) resolve_timing();
+    }
+    void resolve_timing() {
+        if (pending.empty()) return;
+        (void)hipStreamSynchronize(stream);
+        for (auto& t : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) stage_ms[t.stage] += ms;
+            spare.push_back(t.a);
+            spare.push_back(t.b);
+        }
+        pending.clear();
+    }
+
+    // Runs one launch, bracketed by an event pair when profiling.
+    template <typename F>
+    void timed(const char* name, F&& launch) {
+        if (!prof) {
+            launch();
+            return;
+        }
+        hipEvent_t a = take_event(), b = take_event();
+        (void)hipEventRecord(a, stream);
+        launch();
+        (void)hipEventRecord(b, stream);
+        prof->push_back({name, a, b});
+    }
+
+    // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
+    void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
+                     float4* nout_v) {
+        const int gb = grain_blocks(n);
+        const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
+        timed("k_count", [&] {
+            hipLaunchKernelGGL(k_count, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, count, n);
+        });
+        timed("k_scan", [&] {
+            hipLaunchKernelGGL(k_scan, dim3(scan_grid), dim3(kScanThreads), 0, stream, dp, ctl,
+                               count, start, status);
+        });
+        timed("k_scatter", [&] {
+            hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, Tp,
+                               start, tmp_id, n);
+        });
+        timed("k_reorder", [&] {
+            if (nmode == 0) {
+                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
+            } else if (nmode == 1) {
+                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
+            } else {
+                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
+            }
+        });
+    }
+
+    template <int MODE, bool ENERGY>
+    void launch_force(const float4* np, const float4* nv, float* rhs) {
+        timed("k_force", [&] {
+            hipLaunchKernelGGL((k_force<MODE, ENERGY>), dim3(grain_blocks(n)), dim3(kBlock), 0,
+                               stream, dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, partials,
+                               energy_log, n);
+        });
+    }
+
+    // CIC deposit of the current state into `out` (4 * nodes floats), queued.
+    void enqueue_cic(float* out) {
+        const int zb = static_cast<int>(std::min<size_t>((cic_count + kBlock - 1) / kBlock, 4096));
+        timed("k_cic_zero", [&] {
+            hipLaunchKernelGGL(k_cic_zero, dim3(zb), dim3(kBlock), 0, stream, cic_acc, cic_count, ctl);
+        });
+        if (n > 0) {
+            timed("k_cic_scale", [&] {
+                hipLaunchKernelGGL(k_cic_scale, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                                   stream, dp, ctl, Tp, Tv, n);
+            });
+            timed("k_cic_deposit", [&] {
+                hipLaunchKernelGGL(k_cic_deposit, dim3(grain_blocks(n)), dim3(kBlock), 0, stream, dp,
+                                   ctl, Tp, Tv, cic_acc, n);
+            });
+        }
+        timed("k_cic_convert", [&] {
+            hipLaunchKernelGGL(k_cic_convert, dim3(zb), dim3(kBlock), 0, stream, ctl, cic_acc, out,
+                               cic_count, std::max(n, 1));
+        });
+    }
+
+    void enqueue_step() {
+        if (n == 0) return;
+        if (!picard()) {
+            hipEvent_t a = stage_begin();
+            enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+            stage_end(WD_STAGE_FIND_NEIGHBORS, a);
+            a = stage_begin();
+            if (energy_on) launch_force<kModeSIE, true>(nullptr, nullptr, nullptr);
+            else launch_force<kModeSIE, false>(nullptr, nullptr, nullptr);
+            stage_end(WD_STAGE_COMPUTE_RHS, a);
+            return;
+        }
+        for (int k = 0; k < cfg.picard_iterations; ++k) {
+            hipEvent_t a = stage_begin();
+            if (k == 0) {
+                enqueue_bin(1, nullptr, nullptr, NP[0], NV[0]);
+            } else {
+                enqueue_bin(2, NP[(k - 1) & 1], NV[(k - 1) & 1], NP[k & 1], NV[k & 1]);
+            }
+            stage_end(WD_STAGE_FIND_NEIGHBORS, a);
+            a = stage_begin();
+            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[k & 1], NV[k & 1], nullptr);
+            else launch_force<kModeBEP, false>(NP[k & 1], NV[k & 1], nullptr);
+            stage_end(WD_STAGE_COMPUTE_RHS, a);
+        }
+    }
+
+    void drop_graphs() {
+        if (graph_one) (void)hipGraphExecDestroy(graph_one);
+        if (graph_chunk) (void)hipGraphExecDestroy(graph_chunk);
+        graph_one = graph_chunk = nullptr;
+    }
+
+    int capture(int steps, hipGraphExec_t* out) {
+        hipGraph_t g = nullptr;
+        WD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < steps; ++s) enqueue_step();
+        WD_HIP(hipStreamEndCapture(stream, &g));
+        const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(WD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return WD_OK;
+    }
+
+    int step(int nsteps) {
+        if (n > 0) {
+            if (timing) {
+                for (int s = 0; s < nsteps; ++s) enqueue_step();
+            } else {
+                if (!graph_one) {
+                    int rc = capture(1, &graph_one);
+                    if (rc != WD_OK) return rc;
+                    rc = capture(chunk, &graph_chunk);
+                    if (rc != WD_OK) return rc;
+                }
+                int left = nsteps;
+                while (left >= chunk) {
+                    WD_HIP(hipGraphLaunch(graph_chunk, stream));
+                    left -= chunk;
+                }
+                while (left-- > 0) WD_HIP(hipGraphLaunch(graph_one, stream));
+            }
+            WD_HIP(hipGetLastError());
+        }
+        // The reference accumulates time in FP32, one dt per step.
+        for (int s = 0; s < nsteps; ++s) time += hp.dt;
+        return WD_OK;
+    }
+
+    int check_device() {
+        int err = 0;
+        WD_HIP(hipMemcpy(&err, &ctl->error, sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return fail(WD_ERR_DEVICE, "device: scan look-back wait exceeded its bound");
+        return WD_OK;
+    }
+
+    int sync() {
+        WD_HIP(hipStreamSynchronize(stream));
+        return check_device();
+    }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    WD_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+    return WD_OK;
+}
+
+int validate(const wd_config& c) {
+    if (c.collision_grid_size_x < 1 || c.collision_grid_size_y < 1 || c.collision_grid_size_z < 1) {
+        return fail(WD_ERR_ARG, "collision grid sizes must be positive");
+    }
+    if (c.collision_grid_size_x > 2046 || c.collision_grid_size_y > 2046 ||
+        c.collision_grid_size_z > 1022) {
+        return fail(WD_ERR_ARG, "collision grid exceeds 2046 x 2046 x 1022 cells");
+    }
+    const double cells = (c.collision_grid_size_x + 2.0) * (c.collision_grid_size_y + 2.0) *
+                         (c.collision_grid_size_z + 2.0);
+    if (cells > 1.0e9) return fail(WD_ERR_ARG, "collision grid exceeds 1e9 cells");
+    if (c.particles_per_cell < 1) return fail(WD_ERR_ARG, "particles_per_cell must be positive");
+    if (c.density_grid_size_x < 1 || c.density_grid_size_y < 1 || c.density_grid_size_z < 1) {
+        return fail(WD_ERR_ARG, "density grid sizes must be positive");
+    }
+    if (c.num_masses < 1 || c.num_masses > WD_MAX_MATERIALS) {
+        return fail(WD_ERR_ARG, "materials.masses must hold 1..16 entries");
+    }
+    if (c.particle_material < 0 || c.particle_material >= c.num_masses) {
+        return fail(WD_ERR_ARG, "initialization.particle_material has no entry in materials.masses");
+    }
+    if (c.tangential_model != WD_TANGENTIAL_VISCOUS) {
+        return fail(WD_ERR_ARG, "tangential_model spring_history is not available in this build");
+    }
+    return WD_OK;
+}
+
+void fill_params(const wd_config& c, int n, Params& P) {
+    std::memset(&P, 0, sizeof(P));
+    P.n = n;
+    P.gx = c.collision_grid_size_x;
+    P.gy = c.collision_grid_size_y;
+    P.gz = c.collision_grid_size_z;
+    P.kcap = c.particles_per_cell;
+    P.nmat = c.num_masses;
+    P.lo[0] = c.x_min; P.lo[1] = c.y_min; P.lo[2] = c.z_min;
+    P.hi[0] = c.x_max; P.hi[1] = c.y_max; P.hi[2] = c.z_max;
+    P.wlo[0] = c.wall_x_min; P.wlo[1] = c.wall_y_min; P.wlo[2] = c.floor_z;
+    P.whi[0] = c.wall_x_max; P.whi[1] = c.wall_y_max; P.whi[2] = c.ceiling_z;
+    P.gravity = c.gravity;
+    P.radius = c.particle_radius;
+    P.max_force = c.max_force;
+    float k = c.max_force / c.particle_radius;
+    if (c.normal_stiffness > 0.f) {
+        k = c.normal_stiffness;
+        P.kneg = -c.normal_stiffness;
+        P.clamp_delta = c.max_force / c.normal_stiffness;
+    } else {
+        P.kneg = -c.max_force / c.particle_radius;
+        P.clamp_delta = c.particle_radius;
+    }
+    P.spring_k = k;
+    P.mu = c.friction;
+    P.dt = c.dt;
+    P.dgx = c.density_grid_size_x;
+    P.dgy = c.density_grid_size_y;
+    P.dgz = c.density_grid_size_z;
+    for (int i = 0; i < kMaxMat; ++i) {
+        const float mi = i < c.num_masses ? c.masses[i] : 0.f;
+        P.mass[i] = mi;
+        P.c_wall[i] = damping(c.restitution, k, mi);
+        for (int j = 0; j < kMaxMat; ++j) {
+            const float mj = j < c.num_masses ? c.masses[j] : 0.f;
+            const float msum = mi + mj;
+            const float mred = (msum > 0.f) ? (mi * mj) / msum : 0.f;
+            P.c_pair[i * kMaxMat + j] = damping(c.restitution, k, mred);
+        }
+    }
+}
+
+int upload_params(wd_sim* s) {
+    WD_HIP(hipMemcpy(s->dp, &s->hp, sizeof(Params), hipMemcpyHostToDevice));
+    return WD_OK;
+}
+
+int allocate(wd_sim* s) {
+    const int n = s->n;
+    const wd_config& c = s->cfg;
+    int rc;
+#define WD_TRY(x) do { rc = (x); if (rc != WD_OK) return rc; } while (0)
+    WD_TRY(dalloc(&s->dp, 1));
+    WD_TRY(dalloc(&s->ctl, 1));
+    WD_TRY(dalloc(&s->Tp, n)); WD_TRY(dalloc(&s->Tv, n));
+    WD_TRY(dalloc(&s->Sp, n)); WD_TRY(dalloc(&s->Sv, n));
+    if (s->picard()) {
+        for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->NP[b], n)); WD_TRY(dalloc(&s->NV[b], n)); }
+    }
+    WD_TRY(dalloc(&s->key, n)); WD_TRY(dalloc(&s->slot, n)); WD_TRY(dalloc(&s->tmp_id, n));
+    const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
+                         (c.collision_grid_size_y + 2) * (c.collision_grid_size_z + 2);
+    s->ntiles_max = (cells + 1 + kScanTile - 1) / kScanTile;
+    s->scan_cells = s->ntiles_max * kScanTile;
+    WD_TRY(dalloc(&s->count, s->scan_cells));
+    WD_TRY(dalloc(&s->start, s->scan_cells));
+    WD_TRY(dalloc(&s->status, s->ntiles_max));
+    WD_TRY(dalloc(&s->st6, static_cast<size_t>(6) * n));
+    WD_TRY(dalloc(&s->mat_d, n));
+    s->nblocks = grain_blocks(n);
+    WD_TRY(dalloc(&s->partials, s->nblocks));
+    s->cic_count = static_cast<size_t>(4) * c.density_grid_size_x * c.density_grid_size_y *
+                   c.density_grid_size_z;
+    WD_TRY(dalloc(&s->cic_acc, s->cic_count));
+    WD_TRY(dalloc(&s->cic_out, s->cic_count));
+#undef WD_TRY
+    WD_HIP(hipMemset(s->ctl, 0, sizeof(Ctl)));
+    WD_HIP(hipMemset(s->count, 0, s->scan_cells * sizeof(uint32_t)));
+    WD_HIP(hipMemset(s->status, 0, s->ntiles_max * sizeof(unsigned long long)));
+    Ctl init;
+    std::memset(&init, 0, sizeof(init));
+    for (int k = 0; k < 3; ++k) { init.bbox_acc[k] = 0x7FFFFFFF; init.bbox_acc[k + 3] = -0x7FFFFFFF - 1; }
+    WD_HIP(hipMemcpy(s->ctl, &init, sizeof(Ctl), hipMemcpyHostToDevice));
+    return upload_params(s);
+}
+
+// Uploads an original-order state (and optionally materials) and rebuilds
+// the working layout.
+int ingest(wd_sim* s, const float* st, const int* mat) {
+    if (s->n == 0) return WD_OK;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    if (mat) WD_HIP(hipMemcpy(s->mat_d, mat, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_ingest, dim3(grain_blocks(s->n)), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
+                       s->st6, s->mat_d, s->Tp, s->Tv, s->key, s->n);
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+int export_state(wd_sim* s) {
+    if (s->n == 0) return WD_OK;
+    const int blocks = (s->n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_export, dim3(blocks), dim3(kBlock), 0, s->stream, s->Tp, s->Tv, s->st6,
+                       s->mat_d, s->n);
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+}  // namespace
+
+#define WD_CHECK_SIM(s) \
+    if (!(s)) return fail(WD_ERR_ARG, std::string(__func__) + ": null handle")
+
+extern "C" int wd_create(const wd_config* cfg, wd_sim** out) {
+    if (!cfg || !out) return fail(WD_ERR_ARG, "wd_create: null argument");
+    *out = nullptr;
+    int rc = validate(*cfg);
+    if (rc != WD_OK) return rc;
+    int n = 0;
+    rc = wd_config_grain_count(cfg, &n);
+    if (rc != WD_OK) return rc;
+    std::vector<float> st(static_cast<size_t>(6) * std::max(n, 1));
+    std::vector<int> mat(static_cast<size_t>(std::max(n, 1)));
+    rc = wd_initial_state(cfg, st.data(), mat.data(), n);
+    if (rc != WD_OK) return rc;
+    wd_sim* s = new wd_sim();
+    s->cfg = *cfg;
+    s->n = n;
+    s->timing = cfg->kernel_timing != 0;
+    fill_params(*cfg, n, s->hp);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(WD_ERR_HIP, "wd_create: hipStreamCreate failed");
+    }
+    rc = allocate(s);
+    if (rc == WD_OK) rc = ingest(s, st.data(), mat.data());
+    if (rc != WD_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return WD_OK;
+}
+
+extern "C" int wd_destroy(wd_sim* s) {
+    if (s) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        delete s;
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_num_grains(const wd_sim* s, int* n_out) {
+    WD_CHECK_SIM(s);
+    if (!n_out) return fail(WD_ERR_ARG, "wd_num_grains: null output");
+    *n_out = s->n;
+    return WD_OK;
+}
+
+extern "C" int wd_get_state(wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_get_state: null output");
+    int rc = export_state(s);
+    if (rc != WD_OK) return rc;
+    if (s->n) WD_HIP(hipMemcpy(out, s->st6, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+extern "C" int wd_set_state(wd_sim* s, const float* in) {
+    WD_CHECK_SIM(s);
+    if (!in) return fail(WD_ERR_ARG, "wd_set_state: null input");
+    return ingest(s, in, nullptr);
+}
+
+extern "C" int wd_get_material(wd_sim* s, int* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_get_material: null output");
+    int rc = export_state(s);
+    if (rc != WD_OK) return rc;
+    if (s->n) WD_HIP(hipMemcpy(out, s->mat_d, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+extern "C" int wd_set_material(wd_sim* s, const int* in) {
+    WD_CHECK_SIM(s);
+    if (!in) return fail(WD_ERR_ARG, "wd_set_material: null input");
+    for (int i = 0; i < s->n; ++i) {
+        if (in[i] < 0 || in[i] >= s->cfg.num_masses) return fail(WD_ERR_ARG, "wd_set_material: material id out of range");
+    }
+    if (s->n == 0) return WD_OK;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(s->mat_d, in, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_set_material, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
+                       s->Tp, s->Tv, s->mat_d, s->n);
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+extern "C" int wd_set_dt(wd_sim* s, float dt) {
+    WD_CHECK_SIM(s);
+    if (s->hp.dt == dt) return WD_OK;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->hp.dt = dt;
+    s->cfg.dt = dt;
+    return upload_params(s);
+}
+
+extern "C" int wd_get_time(const wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_get_time: null output");
+    *out = s->time;
+    return WD_OK;
+}
+
+extern "C" int wd_set_time(wd_sim* s, float t) {
+    WD_CHECK_SIM(s);
+    s->time = t;
+    return WD_OK;
+}
+
+extern "C" int wd_step(wd_sim* s, int nsteps) {
+    WD_CHECK_SIM(s);
+    if (nsteps < 0) return fail(WD_ERR_ARG, "wd_step: negative step count");
+    return s->step(nsteps);
+}
+
+extern "C" int wd_sync(wd_sim* s) {
+    WD_CHECK_SIM(s);
+    return s->sync();
+}
+
+extern "C" int wd_density_grid(wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_density_grid: null output");
+    hipEvent_t a = s->stage_begin();
+    s->enqueue_cic(s->cic_out);
+    s->stage_end(WD_STAGE_DENSITY_GRID, a);
+    WD_HIP(hipGetLastError());
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    WD_HIP(hipMemcpy(out, s->cic_out, sizeof(float) * s->cic_count, hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+extern "C" int wd_total_energy(wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_total_energy: null output");
+    if (s->n == 0) {
+        *out = 0.f;
+        return WD_OK;
+    }
+    hipEvent_t a = s->stage_begin();
+    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->stage_end(WD_STAGE_FIND_NEIGHBORS, a);
+    s->launch_force<kModeRHS, true>(nullptr, nullptr, nullptr);
+    WD_HIP(hipGetLastError());
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    double e = 0.0;
+    WD_HIP(hipMemcpy(&e, &s->ctl->energy_last, sizeof(double), hipMemcpyDeviceToHost));
+    *out = static_cast<float>(e);
+    return WD_OK;
+}
+
+extern "C" int wd_energy_log_enable(wd_sim* s, int capacity) {
+    WD_CHECK_SIM(s);
+    if (capacity < 0) return fail(WD_ERR_ARG, "wd_energy_log_enable: negative capacity");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    if (s->energy_log) {
+        (void)hipFree(s->energy_log);
+        s->energy_log = nullptr;
+    }
+    s->energy_cap = capacity;
+    if (capacity > 0) {
+        rc = dalloc(&s->energy_log, capacity);
+        if (rc != WD_OK) return rc;
+    }
+    const int zero = 0;
+    WD_HIP(hipMemcpy(&s->ctl->energy_count, &zero, sizeof(int), hipMemcpyHostToDevice));
+    WD_HIP(hipMemcpy(&s->ctl->energy_capacity, &capacity, sizeof(int), hipMemcpyHostToDevice));
+    const bool on = capacity > 0;
+    if (on != s->energy_on) {
+        s->energy_on = on;
+        s->drop_graphs();
+    }
+    if (on) s->drop_graphs();  // the log pointer is baked into captured launches
+    return WD_OK;
+}
+
+extern "C" int wd_energy_log_read(wd_sim* s, double* out, int count, int* recorded) {
+    WD_CHECK_SIM(s);
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    int have = 0;
+    WD_HIP(hipMemcpy(&have, &s->ctl->energy_count, sizeof(int), hipMemcpyDeviceToHost));
+    if (recorded) *recorded = have;
+    const int m = std::min(count, have);
+    if (m > 0 && out) WD_HIP(hipMemcpy(out, s->energy_log, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+extern "C" int wd_max_accel(wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_max_accel: null output");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    unsigned bits = 0;
+    WD_HIP(hipMemcpy(&bits, &s->ctl->accel_last, sizeof(unsigned), hipMemcpyDeviceToHost));
+    std::memcpy(out, &bits, sizeof(float));
+    return WD_OK;
+}
+
+extern "C" int wd_is_stable(wd_sim* s, int* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_is_stable: null output");
+    std::vector<float> st(static_cast<size_t>(6) * s->n);
+    int rc = s->n ? wd_get_state(s, st.data()) : WD_OK;
+    if (rc != WD_OK) return rc;
+    const float lo[3] = {s->cfg.x_min, s->cfg.y_min, s->cfg.z_min};
+    const float hi[3] = {s->cfg.x_max, s->cfg.y_max, s->cfg.z_max};
+    *out = 1;
+    for (int i = 0; i < s->n && *out; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float p = st[6 * static_cast<size_t>(i) + a];
+            if (!std::isfinite(p) || p < lo[a] || p > hi[a]) {
+                *out = 0;
+                break;
+            }
+        }
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_overflow_count(wd_sim* s, int* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_overflow_count: null output");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    unsigned long long v = 0;
+    WD_HIP(hipMemcpy(&v, &s->ctl->overflow_lifetime, sizeof(v), hipMemcpyDeviceToHost));
+    *out = static_cast<int>(std::min<unsigned long long>(v, 0x7FFFFFFFull));
+    return WD_OK;
+}
+
+extern "C" int wd_stage_ms(wd_sim* s, float out[4]) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_stage_ms: null output");
+    s->resolve_timing();
+    for (int k = 0; k < 4; ++k) out[k] = s->stage_ms[k];
+    return WD_OK;
+}
+
+extern "C" int wd_cell_order(wd_sim* s, int* ids_out, uint32_t* keys_out) {
+    WD_CHECK_SIM(s);
+    if (!ids_out || !keys_out) return fail(WD_ERR_ARG, "wd_cell_order: null output");
+    if (s->n == 0) return WD_OK;
+    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    int* ids = nullptr;
+    uint32_t* keys = nullptr;
+    int rc = dalloc(&ids, s->n);
+    if (rc == WD_OK) rc = dalloc(&keys, s->n);
+    if (rc == WD_OK) {
+        hipLaunchKernelGGL(k_export_order, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
+                           s->dp, s->Sp, ids, keys, s->n);
+        rc = s->sync();
+    }
+    if (rc == WD_OK && (hipMemcpy(ids_out, ids, sizeof(int) * s->n, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(keys_out, keys, sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost) != hipSuccess)) {
+        rc = fail(WD_ERR_HIP, "wd_cell_order: copy failed");
+    }
+    if (ids) (void)hipFree(ids);
+    if (keys) (void)hipFree(keys);
+    return rc;
+}
+
+extern "C" int wd_neighbor_rows(wd_sim* s, int* out, size_t capacity) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_neighbor_rows: null output");
+    const size_t need = static_cast<size_t>(s->n) * 27 * s->cfg.particles_per_cell;
+    if (capacity < need) return fail(WD_ERR_ARG, "wd_neighbor_rows: output too small");
+    if (s->n == 0) return WD_OK;
+    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    int* rows = nullptr;
+    int rc = dalloc(&rows, need);
+    if (rc == WD_OK) {
+        hipLaunchKernelGGL(k_neighbor_rows, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
+                           s->dp, s->ctl, s->Sp, s->start, rows, s->n);
+        rc = s->sync();
+    }
+    if (rc == WD_OK && hipMemcpy(out, rows, sizeof(int) * need, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = fail(WD_ERR_HIP, "wd_neighbor_rows: copy failed");
+    }
+    if (rows) (void)hipFree(rows);
+    return rc;
+}
+
+extern "C" int wd_rhs(wd_sim* s, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out) return fail(WD_ERR_ARG, "wd_rhs: null output");
+    if (s->n == 0) return WD_OK;
+    if (!s->rhs_buf) {
+        int rc = dalloc(&s->rhs_buf, static_cast<size_t>(6) * s->n);
+        if (rc != WD_OK) return rc;
+    }
+    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->launch_force<kModeRHS, false>(nullptr, nullptr, s->rhs_buf);
+    WD_HIP(hipGetLastError());
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    WD_HIP(hipMemcpy(out, s->rhs_buf, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+extern "C" int wd_density_log_enable(wd_sim* s, int capacity) {
+    WD_CHECK_SIM(s);
+    if (capacity < 0) return fail(WD_ERR_ARG, "wd_density_log_enable: negative capacity");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    if (s->cic_log) {
+        (void)hipFree(s->cic_log);
+        s->cic_log = nullptr;
+    }
+    s->cic_log_cap = capacity;
+    s->cic_log_count = 0;
+    if (capacity > 0) return dalloc(&s->cic_log, s->cic_count * static_cast<size_t>(capacity));
+    return WD_OK;
+}
+
+extern "C" int wd_density_log_record(wd_sim* s) {
+    WD_CHECK_SIM(s);
+    if (s->cic_log_count >= s->cic_log_cap) return fail(WD_ERR_STATE, "wd_density_log_record: log full");
+    s->enqueue_cic(s->cic_log + s->cic_count * static_cast<size_t>(s->cic_log_count));
+    WD_HIP(hipGetLastError());
+    s->cic_log_count++;
+    return WD_OK;
+}
+
+extern "C" int wd_density_log_read(wd_sim* s, float* out, int count, int* recorded) {
+    WD_CHECK_SIM(s);
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    if (recorded) *recorded = s->cic_log_count;
+    const int m = std::min(count, s->cic_log_count);
+    if (m > 0 && out) {
+        WD_HIP(hipMemcpy(out, s->cic_log, sizeof(float) * s->cic_count * static_cast<size_t>(m),
+                         hipMemcpyDeviceToHost));
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_profile_step(wd_sim* s, int nsteps, float* kernel_ms, int capacity, int* count,
+                               char* names, size_t names_capacity) {
+    WD_CHECK_SIM(s);
+    if (nsteps <= 0 || !kernel_ms || !count) return fail(WD_ERR_ARG, "wd_profile_step: bad argument");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    std::vector<wd_sim::ProfRec> recs;
+    const bool timing = s->timing;
+    s->timing = false;
+    s->prof = &recs;
+    for (int k = 0; k < nsteps; ++k) s->enqueue_step();
+    s->prof = nullptr;
+    s->timing = timing;
+    for (int k = 0; k < nsteps; ++k) s->time += s->hp.dt;
+    rc = s->sync();
+    // Group by position within a step: every step launches the same sequence.
+    const size_t per = nsteps > 0 ? recs.size() / static_cast<size_t>(nsteps) : 0;
+    std::vector<double> acc(per, 0.0);
+    for (size_t r = 0; r < recs.size(); ++r) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, recs[r].a, recs[r].b);
+        if (per) acc[r % per] += ms;
+        s->spare.push_back(recs[r].a);
+        s->spare.push_back(recs[r].b);
+    }
+    if (rc != WD_OK) return rc;
+    *count = static_cast<int>(per);
+    std::string list;
+    for (size_t k = 0; k < per; ++k) {
+        if (static_cast<int>(k) < capacity) kernel_ms[k] = static_cast<float>(acc[k] / nsteps);
+        if (k) list += ",";
+        list += recs[k].name;
+    }
+    if (names && names_capacity) {
+        std::strncpy(names, list.c_str(), names_capacity - 1);
+        names[names_capacity - 1] = '\0';
+    }
+    return WD_OK;
+}

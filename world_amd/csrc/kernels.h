@@ -1,0 +1,52 @@
+// Kernel declarations and the device control block of one simulation handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_device.h"
+
+namespace wd {
+
+constexpr int kBlock = 256;          // threads per block of the per-grain kernels
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
+constexpr int kXcds = 8;             // MI355X accelerator complex dies
+
+// Small device-side control block: counters, the occupied-cell bounding box
+// and per-pass results.  Reset points are documented at each field.
+struct Ctl {
+    unsigned tile_counter;     // scan tile dispenser; zeroed by k_count
+    unsigned epoch;            // scan look-back tag; bumped by k_count
+    unsigned done_force;       // last-block detection; self-resetting
+    unsigned done_ingest;      // last-block detection; self-resetting
+    unsigned done_energy;      // last-block detection; self-resetting
+    int bbox_cur[6];           // occupied cells (min xyz, max xyz) of the current state
+    int bbox_acc[6];           // accumulated by the force pass for the next state
+    unsigned accel_acc;        // float bits, max |a| of the running force pass
+    unsigned accel_last;       // float bits, max |a| of the last completed force pass
+    unsigned long long overflow_lifetime;  // over-capacity cells, summed over passes
+    int error;                 // nonzero: a bounded wait gave up
+    int energy_count;          // entries written to the energy log
+    int energy_capacity;
+    double energy_last;        // last standalone energy
+    unsigned cic_scale_bits;   // float bits, max per-grain deposit magnitude
+};
+
+// Launch geometry helpers: grid rounded up to a multiple of the XCD count,
+// logical block b maps to a contiguous slab so neighbouring grains share an
+// XCD's L2.
+inline int grain_blocks(int n) {
+    int b = (n + kBlock - 1) / kBlock;
+    return ((b + kXcds - 1) / kXcds) * kXcds;
+}
+
+__device__ __forceinline__ int logical_block() {
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+enum ForceMode { kModeSIE = 0, kModeBEP = 1, kModeRHS = 2 };
+
+}  // namespace wd

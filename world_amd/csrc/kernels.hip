@@ -1,0 +1,689 @@
+// gfx950 kernels of the per-step granular pipeline.
+//
+// Grain data lives in two float4 streams per grain: position with the
+// original grain index in .w, velocity with the material id in .w (32 B per
+// grain, one dwordx4 load each).  A step is
+//
+//   k_count   -> slot of every grain in its cell (atomic per-cell counter)
+//   k_scan    -> exclusive scan of the counts over the occupied sub-grid,
+//                one pass with decoupled look-back; counts re-zeroed here
+//   k_scatter -> original ids into cell-sorted slots
+//   k_reorder -> rank inside the cell by original id, move the grain to its
+//                cell-sorted position (coalesced float4 streams)
+//   k_force   -> walls + 27-cell pair contacts + integration, fused; writes
+//                the next state and its cell keys, reduces the next bounding
+//                box, max |a| and (optionally) the total energy
+//
+// The cell order (flat cell, then original index inside a cell) is a
+// deterministic instance of the reference's grid (src/find_neighbors_kernel.cu:
+// 43-161), whose in-cell order depends on atomic races.  Pair forces are
+// summed in stencil order dx -> dy -> dz as the reference's neighbor rows are.
+// A cell holding more than particles_per_cell grains contributes only its
+// first particles_per_cell grains to every neighbor list, and counts once
+// per pass towards the overflow total (src/find_neighbors_kernel.cu:81-112).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace wd {
+
+namespace {
+
+constexpr int kEmptyMin = 0x7FFFFFFF;
+constexpr int kEmptyMax = (-0x7FFFFFFF - 1);
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ unsigned wave_max_u(unsigned v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64)));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+    for (int o = 32; o > 0; o >>= 1) v += static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int atomic_load_i(int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned atomic_load_u(unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide reduction of the occupied-cell box, applied to ctl->bbox_acc
+// with atomics that are skipped when they cannot change the value.
+__device__ void reduce_bbox(Ctl* ctl, int lo[3], int hi[3]) {
+    __shared__ int s_box[16][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    int v[6] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
+                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2])};
+    if (lane == 0) for (int k = 0; k < 6; ++k) s_box[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; ++w) {
+            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
+            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (v[k] < atomic_load_i(&ctl->bbox_acc[k])) atomicMin(&ctl->bbox_acc[k], v[k]);
+        }
+        for (int k = 3; k < 6; ++k) {
+            if (v[k] > atomic_load_i(&ctl->bbox_acc[k])) atomicMax(&ctl->bbox_acc[k], v[k]);
+        }
+    }
+}
+
+// Called by the last block of a pass: publish the accumulated box as the
+// current one and re-arm the accumulator.
+__device__ void publish_bbox(Ctl* ctl) {
+    for (int k = 0; k < 6; ++k) ctl->bbox_cur[k] = atomic_load_i(&ctl->bbox_acc[k]);
+    for (int k = 0; k < 3; ++k) ctl->bbox_acc[k] = kEmptyMin;
+    for (int k = 3; k < 6; ++k) ctl->bbox_acc[k] = kEmptyMax;
+}
+
+// Returns true in every thread of the block that finished last.  Thread 0's
+// prior global writes are released before the counter increments.
+__device__ bool last_block(unsigned* done) {
+    __shared__ bool s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(done, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) __threadfence();
+    return s_last;
+}
+
+// Deterministic block sum of one double per thread (fixed butterfly and wave
+// order); the result is valid in thread 0.
+__device__ double block_sum(double v) {
+    __shared__ double s_part[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) s_part[wave] = v;
+    __syncthreads();
+    double total = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < nw; ++w) total += s_part[w];
+    }
+    __syncthreads();
+    return total;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Ingest an original-order AoS state: grain i gets id i.
+__global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp, Ctl* ctl,
+                                                   const float* __restrict__ st6,
+                                                   const int* __restrict__ mat,
+                                                   float4* __restrict__ Tp, float4* __restrict__ Tv,
+                                                   uint32_t* __restrict__ key, int n) {
+    const Params& P = *Pp;
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    if (i < n) {
+        const float* s = st6 + 6 * static_cast<size_t>(i);
+        const float x = s[0], y = s[1], z = s[2];
+        Tp[i] = make_float4(x, y, z, __int_as_float(i));
+        Tv[i] = make_float4(s[3], s[4], s[5], __int_as_float(mat[i]));
+        int cx, cy, cz;
+        grain_cell(P, x, y, z, cx, cy, cz);
+        key[i] = pack_cell(cx, cy, cz);
+        lo[0] = hi[0] = cx;
+        lo[1] = hi[1] = cy;
+        lo[2] = hi[2] = cz;
+    }
+    reduce_bbox(ctl, lo, hi);
+    if (last_block(&ctl->done_ingest) && threadIdx.x == 0) {
+        publish_bbox(ctl);
+        ctl->done_ingest = 0;
+    }
+}
+
+// Original-order material ids pushed into the working layout.
+__global__ __launch_bounds__(kBlock) void k_set_material(const float4* __restrict__ Tp,
+                                                         float4* __restrict__ Tv,
+                                                         const int* __restrict__ mat, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Tv[i].w = __int_as_float(mat[__float_as_int(Tp[i].w)]);
+}
+
+// Working layout -> original-order AoS state.
+__global__ __launch_bounds__(kBlock) void k_export(const float4* __restrict__ Tp,
+                                                   const float4* __restrict__ Tv,
+                                                   float* __restrict__ st6, int* __restrict__ mat,
+                                                   int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float4 p = Tp[i], v = Tv[i];
+        const int id = __float_as_int(p.w);
+        float* s = st6 + 6 * static_cast<size_t>(id);
+        s[0] = p.x; s[1] = p.y; s[2] = p.z;
+        s[3] = v.x; s[4] = v.y; s[5] = v.z;
+        mat[id] = __float_as_int(v.w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
+__global__ __launch_bounds__(kBlock) void k_count(Ctl* ctl, const uint32_t* __restrict__ key,
+                                                  uint32_t* __restrict__ slot,
+                                                  uint32_t* __restrict__ count, int n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->tile_counter = 0;
+        ctl->epoch = ctl->epoch + 1;
+    }
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        int cx, cy, cz;
+        unpack_cell(key[i], cx, cy, cz);
+        slot[i] = atomicAdd(&count[g.index(cx, cy, cz)], 1u);
+    }
+}
+
+// Pass 2: start[c] = sum of count[0..c) over the padded sub-grid, written up
+// to and including index `cells`; counts are zeroed for the next pass and
+// over-capacity cells are added to the lifetime overflow total.  Tiles are
+// handed out by an atomic counter, so a tile only waits on tiles already
+// owned by running blocks (forward progress without co-residency limits).
+__global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict__ Pp, Ctl* ctl,
+                                                       uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ start,
+                                                       unsigned long long* __restrict__ status) {
+    const SubGrid g = make_subgrid(ctl->bbox_cur);
+    const int cells = g.cells();
+    const int ntiles = (cells + 1 + kScanTile - 1) / kScanTile;
+    const unsigned long long tag = static_cast<unsigned long long>(ctl->epoch & 0x3FFFFFFFu) << 34;
+    const uint32_t kcap = static_cast<uint32_t>(Pp->kcap);
+    __shared__ int s_tile;
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_over[kScanThreads / 64];
+    __shared__ uint32_t s_prefix;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(&ctl->tile_counter, 1u));
+        __syncthreads();
+        const int tile = s_tile;
+        if (tile >= ntiles) break;
+        const size_t base = static_cast<size_t>(tile) * kScanTile + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems];
+        const uint4* src = reinterpret_cast<const uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            const uint4 q = src[k];
+            v[4 * k + 0] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+        uint32_t sum = 0, over = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            over += v[k] > kcap ? 1u : 0u;
+            const uint32_t c = v[k];
+            v[k] = sum;  // exclusive within the thread
+            sum += c;
+        }
+        // Wave-level exclusive scan of the thread sums.
+        uint32_t incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
+            if (lane >= o) incl += t;
+        }
+        const uint32_t wave_total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+        over = wave_sum_u(over);
+        if (lane == 0) {
+            s_wave[wave] = wave_total;
+            s_over[wave] = over;
+        }
+        __syncthreads();
+        uint32_t wave_off = 0, agg = 0, over_total = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) {
+            if (w < wave) wave_off += s_wave[w];
+            agg += s_wave[w];
+            over_total += s_over[w];
+        }
+        if (threadIdx.x == 0) {
+            if (over_total) atomicAdd(&ctl->overflow_lifetime, static_cast<unsigned long long>(over_total));
+            uint32_t prefix = 0;
+            if (tile == 0) {
+                __hip_atomic_store(&status[0], tag | (2ull << 32) | agg, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&status[tile], tag | (1ull << 32) | agg, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                int j = tile - 1;
+                unsigned spins = 0;
+                while (j >= 0) {
+                    const unsigned long long w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned flag = static_cast<unsigned>((w >> 32) & 3u);
+                    if ((w & ~((1ull << 34) - 1)) != tag || flag == 0) {
+                        if (++spins > (1u << 22)) {
+                            ctl->error = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    prefix += static_cast<uint32_t>(w);
+                    if (flag == 2) break;
+                    --j;
+                }
+                __hip_atomic_store(&status[tile], tag | (2ull << 32) | (prefix + agg),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_prefix = prefix;
+        }
+        __syncthreads();
+        const uint32_t off = s_prefix + wave_off + (incl - sum);
+        uint4* dst = reinterpret_cast<uint4*>(start + base);
+        uint4* cz = reinterpret_cast<uint4*>(count + base);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            dst[k] = make_uint4(off + v[4 * k + 0], off + v[4 * k + 1], off + v[4 * k + 2],
+                                off + v[4 * k + 3]);
+            const uint4 q = src[k];
+            if (q.x | q.y | q.z | q.w) cz[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+    }
+}
+
+// Pass 3: original ids into cell-sorted slots (order inside a cell is still
+// arbitrary here).
+__global__ __launch_bounds__(kBlock) void k_scatter(const Ctl* __restrict__ ctl,
+                                                    const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ slot,
+                                                    const float4* __restrict__ Tp,
+                                                    const uint32_t* __restrict__ start,
+                                                    int* __restrict__ tmp_id, int n) {
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        int cx, cy, cz;
+        unpack_cell(key[i], cx, cy, cz);
+        tmp_id[start[g.index(cx, cy, cz)] + slot[i]] = __float_as_int(Tp[i].w);
+    }
+}
+
+// Pass 4: final position = cell start + rank of the original id among the
+// cell's ids.  NMODE 1 also copies the grain into the Picard base buffers,
+// NMODE 2 carries the Picard base along the same permutation.
+template <int NMODE>
+__global__ __launch_bounds__(kBlock) void k_reorder(
+        const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
+        const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
+        const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
+        float4* __restrict__ Sv, const float4* __restrict__ NinP, const float4* __restrict__ NinV,
+        float4* __restrict__ NoutP, float4* __restrict__ NoutV, int n) {
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SubGrid g = make_subgrid(ctl->bbox_cur);
+    int cx, cy, cz;
+    unpack_cell(key[i], cx, cy, cz);
+    const int c = g.index(cx, cy, cz);
+    const uint32_t b = start[c], e = start[c + 1];
+    const float4 p = Tp[i];
+    const float4 v = Tv[i];
+    const int id = __float_as_int(p.w);
+    uint32_t rank = 0;
+    if (e - b > 1) {
+        for (uint32_t q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1u : 0u;
+    }
+    const uint32_t dst = b + rank;
+    Sp[dst] = p;
+    Sv[dst] = v;
+    if (NMODE == 1) {
+        NoutP[dst] = p;
+        NoutV[dst] = v;
+    } else if (NMODE == 2) {
+        NoutP[dst] = NinP[i];
+        NoutV[dst] = NinV[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 5: forces at the cell-sorted state S, fused with the integrator.
+// MODE kModeSIE / kModeBEP write the next state to T (same slot), its cell
+// key and the bounding-box / max |a| reductions; kModeRHS only writes
+// rhs = [v, F/m] in original order.  ENERGY adds the per-grain energy of S
+// (pair terms counted once, from the lower original index) to a
+// deterministic grid-wide sum: appended to the energy log for the
+// integrating modes, stored in ctl->energy_last for kModeRHS.
+template <int MODE, bool ENERGY>
+__global__ __launch_bounds__(kBlock) void k_force(
+        const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
+        const float4* __restrict__ Sv, const float4* __restrict__ Np,
+        const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
+        float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
+        float* __restrict__ rhs_out, double* __restrict__ partials,
+        double* __restrict__ energy_log, int n) {
+    const Params& P = *Pp;
+    const int lb = logical_block();
+    const int p = lb * blockDim.x + threadIdx.x;
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    unsigned accel_bits = 0;
+    float e_grain = 0.f;
+    if (p < n) {
+        const float4 pi = Sp[p];
+        const float4 vi4 = Sv[p];
+        const float x[3] = {pi.x, pi.y, pi.z};
+        const float v[3] = {vi4.x, vi4.y, vi4.z};
+        const int id = __float_as_int(pi.w);
+        const int mat = __float_as_int(vi4.w);
+        const float m = P.mass[mat];
+        float f[3] = {0.f, 0.f, 0.f};
+        wall_contacts(P, x, v, m, P.c_wall[mat], f);
+        if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
+
+        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        int cx, cy, cz;
+        grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
+        const uint32_t kcap = static_cast<uint32_t>(P.kcap);
+        for (int dx = -1; dx <= 1; ++dx) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int base = g.index(cx + dx, cy + dy, cz - 1);
+                uint32_t s[4];
+                s[0] = start[base];
+                s[1] = start[base + 1];
+                s[2] = start[base + 2];
+                s[3] = start[base + 3];
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t b = s[c];
+                    const uint32_t e = min(s[c + 1], b + kcap);
+                    for (uint32_t q = b; q < e; ++q) {
+                        if (q == static_cast<uint32_t>(p)) continue;
+                        const float4 pj = Sp[q];
+                        float d0, d1, d2;
+                        const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
+                        pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
+                        if (ENERGY && __float_as_int(pj.w) > id) {
+                            e_grain += contact_energy(P, 2.f * P.radius - norm);
+                        }
+                    }
+                }
+            }
+        }
+        const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
+        if (MODE == kModeRHS) {
+            if (rhs_out) {
+                float* r = rhs_out + 6 * static_cast<size_t>(id);
+                r[0] = v[0]; r[1] = v[1]; r[2] = v[2];
+                r[3] = a[0]; r[4] = a[1]; r[5] = a[2];
+            }
+        } else {
+            float vn[3], xn[3];
+            if (MODE == kModeSIE) {
+                for (int k = 0; k < 3; ++k) vn[k] = v[k] + P.dt * a[k];
+                for (int k = 0; k < 3; ++k) xn[k] = x[k] + P.dt * vn[k];
+            } else {
+                const float4 bp = Np[p], bv = Nv[p];
+                const float x0[3] = {bp.x, bp.y, bp.z}, v0[3] = {bv.x, bv.y, bv.z};
+                for (int k = 0; k < 3; ++k) {
+                    vn[k] = v0[k] + P.dt * a[k];
+                    xn[k] = x0[k] + P.dt * vn[k];
+                }
+            }
+            Tp[p] = make_float4(xn[0], xn[1], xn[2], pi.w);
+            Tv[p] = make_float4(vn[0], vn[1], vn[2], vi4.w);
+            int nx, ny, nz;
+            grain_cell(P, xn[0], xn[1], xn[2], nx, ny, nz);
+            key_out[p] = pack_cell(nx, ny, nz);
+            lo[0] = hi[0] = nx;
+            lo[1] = hi[1] = ny;
+            lo[2] = hi[2] = nz;
+            float s2 = 0.f;
+            s2 += a[0] * a[0];
+            s2 += a[1] * a[1];
+            s2 += a[2] * a[2];
+            const float acc = sqrtf(s2);
+            if (acc == acc) accel_bits = __float_as_uint(acc);
+        }
+    }
+    if (MODE != kModeRHS) {
+        reduce_bbox(ctl, lo, hi);
+        const unsigned am = wave_max_u(accel_bits);
+        if ((threadIdx.x & 63) == 0 && am > atomic_load_u(&ctl->accel_acc)) {
+            atomicMax(&ctl->accel_acc, am);
+            __threadfence();  // performed before this block reports completion
+        }
+    }
+    if (ENERGY) {
+        const double bs = block_sum(static_cast<double>(e_grain));
+        if (threadIdx.x == 0) partials[lb] = bs;
+    }
+    if (MODE == kModeRHS && !ENERGY) return;
+    if (last_block(&ctl->done_force)) {
+        if (ENERGY) {
+            double acc = 0.0;
+            for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x) acc += partials[b];
+            const double total = block_sum(acc);
+            if (threadIdx.x == 0) {
+                if (MODE == kModeRHS) {
+                    ctl->energy_last = total;
+                } else if (ctl->energy_count < ctl->energy_capacity) {
+                    energy_log[ctl->energy_count] = total;
+                    ctl->energy_count = ctl->energy_count + 1;
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            if (MODE != kModeRHS) {
+                publish_bbox(ctl);
+                ctl->accel_last = atomic_load_u(&ctl->accel_acc);
+                ctl->accel_acc = 0u;
+            }
+            ctl->done_force = 0;
+        }
+    }
+}
+
+template __global__ void k_reorder<0>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
+                                      const float4*, const float4*, float4*, float4*,
+                                      const float4*, const float4*, float4*, float4*, int);
+template __global__ void k_reorder<1>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
+                                      const float4*, const float4*, float4*, float4*,
+                                      const float4*, const float4*, float4*, float4*, int);
+template __global__ void k_reorder<2>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
+                                      const float4*, const float4*, float4*, float4*,
+                                      const float4*, const float4*, float4*, float4*, int);
+
+#define WD_INST_FORCE(M, E)                                                                    \
+    template __global__ void k_force<M, E>(const Params*, Ctl*, const float4*, const float4*,  \
+                                           const float4*, const float4*, const uint32_t*,      \
+                                           float4*, float4*, uint32_t*, float*, double*,       \
+                                           double*, int);
+WD_INST_FORCE(kModeSIE, false)
+WD_INST_FORCE(kModeSIE, true)
+WD_INST_FORCE(kModeBEP, false)
+WD_INST_FORCE(kModeBEP, true)
+WD_INST_FORCE(kModeRHS, false)
+WD_INST_FORCE(kModeRHS, true)
+#undef WD_INST_FORCE
+
+// ---------------------------------------------------------------------------
+// Inspection: cell order and reference-format neighbor rows from S.
+__global__ __launch_bounds__(kBlock) void k_export_order(const Params* __restrict__ Pp,
+                                                         const float4* __restrict__ Sp,
+                                                         int* __restrict__ ids,
+                                                         uint32_t* __restrict__ keys, int n) {
+    const Params& P = *Pp;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) {
+        const float4 q = Sp[p];
+        int cx, cy, cz;
+        grain_cell(P, q.x, q.y, q.z, cx, cy, cz);
+        ids[p] = __float_as_int(q.w);
+        keys[p] = (static_cast<uint32_t>(cx) * static_cast<uint32_t>(P.gy) + static_cast<uint32_t>(cy)) *
+                          static_cast<uint32_t>(P.gz) + static_cast<uint32_t>(cz);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restrict__ Pp,
+                                                          const Ctl* __restrict__ ctl,
+                                                          const float4* __restrict__ Sp,
+                                                          const uint32_t* __restrict__ start,
+                                                          int* __restrict__ out, int n) {
+    const Params& P = *Pp;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 pi = Sp[p];
+    const size_t row_len = static_cast<size_t>(27) * P.kcap;
+    int* row = out + static_cast<size_t>(__float_as_int(pi.w)) * row_len;
+    const SubGrid g = make_subgrid(ctl->bbox_cur);
+    int cx, cy, cz;
+    grain_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+    const uint32_t kcap = static_cast<uint32_t>(P.kcap);
+    size_t w = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int c = g.index(cx + dx, cy + dy, cz + dz);
+                const uint32_t b = start[c];
+                const uint32_t e = min(start[c + 1], b + kcap);
+                for (uint32_t q = b; q < e; ++q) {
+                    // A grain dropped from its own full cell can see 27 * k
+                    // candidates; the row keeps the first 27 * k - 1 so the
+                    // terminator stays inside it.
+                    if (q != static_cast<uint32_t>(p) && w + 1 < row_len) row[w++] = __float_as_int(Sp[q].w);
+                }
+            }
+        }
+    }
+    row[w] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Cloud-in-cell deposit (src/density_grid_kernel.cu:25-83), made
+// order-independent: every FP32 contribution (m*w, m*v_a*w, formed as in the
+// reference) is rounded to a 64-bit fixed-point integer whose scale 2^S is
+// chosen from the largest per-grain magnitude so no node sum can overflow;
+// integer sums do not depend on the order of the atomics, so the grid is
+// bitwise reproducible.
+__global__ __launch_bounds__(kBlock) void k_cic_zero(unsigned long long* __restrict__ acc,
+                                                     size_t count, Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cic_scale_bits = 0u;
+    for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        acc[k] = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__ Pp, Ctl* ctl,
+                                                      const float4* __restrict__ Tp,
+                                                      const float4* __restrict__ Tv, int n) {
+    const Params& P = *Pp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned bits = 0u;
+    if (i < n) {
+        const float4 p = Tp[i], v = Tv[i];
+        const float m = P.mass[__float_as_int(v.w)];
+        float mag = fabsf(m);
+        mag = fmaxf(mag, fabsf(m * v.x));
+        mag = fmaxf(mag, fabsf(m * v.y));
+        mag = fmaxf(mag, fabsf(m * v.z));
+        const bool finite = isfinite(m * v.x) && isfinite(m * v.y) && isfinite(m * v.z) &&
+                            isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(m);
+        bits = (m == 0.f) ? 0u : (finite ? __float_as_uint(mag) : 0x7F800000u);
+    }
+    const unsigned bm = wave_max_u(bits);
+    if ((threadIdx.x & 63) == 0 && bm > atomic_load_u(&ctl->cic_scale_bits)) {
+        atomicMax(&ctl->cic_scale_bits, bm);
+    }
+}
+
+// Fixed-point exponent S for n grains of largest magnitude M: n*M*2^S < 2^62.
+__device__ __forceinline__ int cic_shift(float M, int n) {
+    const double nm = static_cast<double>(n) * static_cast<double>(M);
+    if (!(nm > 0.0)) return 0;
+    return 61 - ilogb(nm);
+}
+
+__device__ __forceinline__ void axis_weights(float p, float lo, float hi, int size, int& i0,
+                                             int& i1, float& w0, float& w1) {
+    if (size <= 1) {
+        i0 = 0; i1 = 0; w0 = 1.f; w1 = 0.f;
+        return;
+    }
+    const float h = (hi - lo) / static_cast<float>(size - 1);
+    const float u = (p - lo) / h;
+    const float base = floorf(u);
+    const float frac = u - base;
+    const int b = sat_f2i(base);
+    i0 = min(size - 1, max(0, b));
+    i1 = (b == 2147483647) ? (size - 1) : min(size - 1, max(0, b + 1));
+    w0 = 1.f - frac;
+    w1 = frac;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict__ Pp,
+                                                        const Ctl* __restrict__ ctl,
+                                                        const float4* __restrict__ Tp,
+                                                        const float4* __restrict__ Tv,
+                                                        unsigned long long* __restrict__ acc,
+                                                        int n) {
+    const Params& P = *Pp;
+    const unsigned sb = ctl->cic_scale_bits;
+    if (sb >= 0x7F800000u) return;  // non-finite input: the grid is reported as NaN
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double scale = ldexp(1.0, cic_shift(__uint_as_float(sb), n));
+    const float4 p = Tp[i], v = Tv[i];
+    const float m = P.mass[__float_as_int(v.w)];
+    if (m == 0.f) return;
+    int ix[2], iy[2], iz[2];
+    float wx[2], wy[2], wz[2];
+    axis_weights(p.x, P.lo[0], P.hi[0], P.dgx, ix[0], ix[1], wx[0], wx[1]);
+    axis_weights(p.y, P.lo[1], P.hi[1], P.dgy, iy[0], iy[1], wy[0], wy[1]);
+    axis_weights(p.z, P.lo[2], P.hi[2], P.dgz, iz[0], iz[1], wz[0], wz[1]);
+    const float mom[3] = {m * v.x, m * v.y, m * v.z};
+    const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
+    for (int a = 0; a < 2; ++a) {
+        for (int b = 0; b < 2; ++b) {
+            for (int c = 0; c < 2; ++c) {
+                const float w = wx[a] * wy[b] * wz[c];
+                if (w == 0.f) continue;
+                const size_t node = (static_cast<size_t>(ix[a]) * P.dgy + iy[b]) * P.dgz + iz[c];
+                atomicAdd(&acc[node], static_cast<unsigned long long>(
+                                              llrint(static_cast<double>(m * w) * scale)));
+                for (int k = 0; k < 3; ++k) {
+                    atomicAdd(&acc[(k + 1) * nodes + node],
+                              static_cast<unsigned long long>(
+                                      llrint(static_cast<double>(mom[k] * w) * scale)));
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cic_convert(const Ctl* __restrict__ ctl,
+                                                        const unsigned long long* __restrict__ acc,
+                                                        float* __restrict__ out, size_t count,
+                                                        int n) {
+    const unsigned sb = ctl->cic_scale_bits;
+    const bool bad = sb >= 0x7F800000u;
+    const double inv = bad ? 0.0 : ldexp(1.0, -cic_shift(__uint_as_float(sb), n));
+    for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+         k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        out[k] = bad ? __int_as_float(0x7FC00000)
+                     : static_cast<float>(static_cast<double>(static_cast<long long>(acc[k])) * inv);
+    }
+}
+
+}  // namespace wd

@@ -1,0 +1,224 @@
+"""Python mirror of the reference's ``ParticleDynamics`` class
+(src/particle_dynamics.h:17-141) over the C ABI.
+
+Same member names and semantics: public ``host_state`` / ``host_material`` /
+``host_mass`` arrays, ``device_state.copy_from_host(host_state)`` to push a
+host edit, ``dt`` re-read at every ``take_step``, FP32 ``time`` accumulated
+one ``dt`` per step, and the on-demand diagnostics.  Used by the tests the way
+the reference's GoogleTest suite uses the C++ class.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import WdConfig, check, lib, ptr
+
+K_DIM = 3
+K_STATE_STRIDE = 6
+K_STENCIL_CELLS = 27
+K_CHANNELS = 1 + K_DIM
+
+
+class _DeviceVector:
+    """Stand-in for ``DeviceVector<T>``: pushes a host array into the engine."""
+
+    def __init__(self, sim: "ParticleDynamics", kind: str):
+        self._sim = sim
+        self._kind = kind
+
+    def copy_from_host(self, host: np.ndarray) -> None:
+        s = self._sim
+        if self._kind == "state":
+            a = np.ascontiguousarray(host, np.float32).reshape(-1)
+            if a.size != K_STATE_STRIDE * s.n:
+                raise ValueError("state must hold 6n floats")
+            check(lib().wd_set_state(s._h, ptr(a)))
+        else:
+            a = np.ascontiguousarray(host, np.int32).reshape(-1)
+            if a.size != s.n:
+                raise ValueError("material must hold n ints")
+            check(lib().wd_set_material(s._h, ptr(a)))
+
+
+class ParticleDynamics:
+    def __init__(self, config: WdConfig | None = None):
+        self.config = (config or _capi.default_config()).copy()
+        self._h = C.c_void_p()
+        check(lib().wd_create(C.byref(self.config), C.byref(self._h)))
+        n = C.c_int(0)
+        check(lib().wd_num_grains(self._h, C.byref(n)))
+        self.n = n.value
+        self.dt = float(self.config.dt)
+        self._dt_sent = self.dt
+        self.particles_per_cell = self.config.particles_per_cell
+        self.collision_grid_size_x = self.config.collision_grid_size_x
+        self.collision_grid_size_y = self.config.collision_grid_size_y
+        self.collision_grid_size_z = self.config.collision_grid_size_z
+        self.density_grid_size_x = self.config.density_grid_size_x
+        self.density_grid_size_y = self.config.density_grid_size_y
+        self.density_grid_size_z = self.config.density_grid_size_z
+        self.host_state = np.zeros(K_STATE_STRIDE * self.n, np.float32)
+        self.host_material = np.zeros(self.n, np.int32)
+        self.host_mass = np.array(self.config.get_masses(), np.float32)
+        self.positions = np.zeros(K_DIM * self.n, np.float32)
+        self.host_density_grid = np.zeros(K_CHANNELS * self.density_grid_nodes(), np.float32)
+        self.device_state = _DeviceVector(self, "state")
+        self.device_material = _DeviceVector(self, "material")
+        if self.n:
+            check(lib().wd_get_material(self._h, ptr(self.host_material)))
+        self.unpack_state()
+
+    # -- lifetime -------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            lib().wd_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping -------------------------------------------------------
+    def _push_dt(self) -> None:
+        if self.dt != self._dt_sent:
+            check(lib().wd_set_dt(self._h, C.c_float(self.dt)))
+            self._dt_sent = self.dt
+
+    def take_step(self, nsteps: int = 1) -> None:
+        self._push_dt()
+        check(lib().wd_step(self._h, int(nsteps)))
+
+    @property
+    def time(self) -> float:
+        t = C.c_float(0)
+        check(lib().wd_get_time(self._h, C.byref(t)))
+        return t.value
+
+    def sync(self) -> None:
+        check(lib().wd_sync(self._h))
+
+    # -- state access ---------------------------------------------------
+    def unpack_state(self) -> None:
+        if self.n:
+            check(lib().wd_get_state(self._h, ptr(self.host_state)))
+        self.positions = self.host_state.reshape(-1, K_STATE_STRIDE)[:, :K_DIM].reshape(-1).copy()
+
+    def state(self) -> np.ndarray:
+        self.unpack_state()
+        return self.host_state.reshape(-1, K_STATE_STRIDE).copy()
+
+    def set_state(self, state: np.ndarray) -> None:
+        self.host_state[:] = np.asarray(state, np.float32).reshape(-1)
+        self.device_state.copy_from_host(self.host_state)
+
+    # -- diagnostics ----------------------------------------------------
+    def density_grid_nodes(self) -> int:
+        return self.density_grid_size_x * self.density_grid_size_y * self.density_grid_size_z
+
+    def compute_density_grid(self) -> int:
+        check(lib().wd_density_grid(self._h, ptr(self.host_density_grid)))
+        return self.host_density_grid.size
+
+    def compute_total_energy(self) -> float:
+        e = C.c_float(0)
+        check(lib().wd_total_energy(self._h, C.byref(e)))
+        return e.value
+
+    def enable_energy_log(self, capacity: int) -> None:
+        check(lib().wd_energy_log_enable(self._h, int(capacity)))
+
+    def energy_log(self) -> np.ndarray:
+        rec = C.c_int(0)
+        check(lib().wd_energy_log_read(self._h, None, 0, C.byref(rec)))
+        out = np.zeros(max(rec.value, 1), np.float64)
+        check(lib().wd_energy_log_read(self._h, ptr(out), rec.value, C.byref(rec)))
+        return out[: rec.value]
+
+    def enable_density_log(self, capacity: int) -> None:
+        check(lib().wd_density_log_enable(self._h, int(capacity)))
+
+    def record_density_grid(self) -> None:
+        check(lib().wd_density_log_record(self._h))
+
+    def density_log(self) -> np.ndarray:
+        rec = C.c_int(0)
+        check(lib().wd_density_log_read(self._h, None, 0, C.byref(rec)))
+        out = np.zeros(max(rec.value, 1) * self.host_density_grid.size, np.float32)
+        check(lib().wd_density_log_read(self._h, ptr(out), rec.value, C.byref(rec)))
+        return out[: rec.value * self.host_density_grid.size].reshape(rec.value, -1)
+
+    def profile_step(self, nsteps: int) -> dict[str, float]:
+        """Mean ms per launch of each kernel of a step (HIP events on the
+        engine's stream); advances the state by nsteps steps."""
+        self._push_dt()
+        ms = np.zeros(64, np.float32)
+        cnt = C.c_int(0)
+        names = C.create_string_buffer(2048)
+        check(lib().wd_profile_step(self._h, int(nsteps), ptr(ms), ms.size, C.byref(cnt), names,
+                                    len(names)))
+        keys = names.value.decode().split(",") if cnt.value else []
+        out: dict[str, float] = {}
+        for k, v in zip(keys, ms[: cnt.value]):
+            out[k] = out.get(k, 0.0) + float(v)
+        return out
+
+    def is_stable(self) -> bool:
+        v = C.c_int(0)
+        check(lib().wd_is_stable(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def compute_max_acceleration(self) -> float:
+        v = C.c_float(0)
+        check(lib().wd_max_accel(self._h, C.byref(v)))
+        return v.value
+
+    def grid_overflow_count(self) -> int:
+        v = C.c_int(0)
+        check(lib().wd_overflow_count(self._h, C.byref(v)))
+        return v.value
+
+    def _stage(self, k: int) -> float:
+        out = np.zeros(4, np.float32)
+        check(lib().wd_stage_ms(self._h, ptr(out)))
+        return float(out[k])
+
+    def find_neighbors_wct(self) -> float:
+        return self._stage(0)
+
+    def compute_rhs_wct(self) -> float:
+        return self._stage(1)
+
+    def take_step_wct(self) -> float:
+        return self._stage(2)
+
+    def density_grid_wct(self) -> float:
+        return self._stage(3)
+
+    # -- inspection (parity tests) ----------------------------------------
+    def cell_order(self) -> tuple[np.ndarray, np.ndarray]:
+        ids = np.zeros(max(self.n, 1), np.int32)
+        keys = np.zeros(max(self.n, 1), np.uint32)
+        check(lib().wd_cell_order(self._h, ptr(ids), ptr(keys)))
+        return ids[: self.n], keys[: self.n]
+
+    def neighbor_rows(self) -> np.ndarray:
+        row = K_STENCIL_CELLS * self.particles_per_cell
+        out = np.full(max(self.n, 1) * row, -1, np.int32)
+        check(lib().wd_neighbor_rows(self._h, ptr(out), out.size))
+        return out[: self.n * row].reshape(self.n, row)
+
+    def rhs(self) -> np.ndarray:
+        out = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
+        check(lib().wd_rhs(self._h, ptr(out)))
+        return out[: K_STATE_STRIDE * self.n].reshape(self.n, K_STATE_STRIDE)

@@ -1,0 +1,115 @@
+"""Granular column-collapse workloads, built as engine configurations.
+
+Material and box shared by all of them: monodisperse spheres of diameter
+d = 0.01 m and density 2500 kg/m^3 (m = 1.309e-3 kg), g = 9.81, a linear
+spring-dashpot normal contact of stiffness k_n = 150 N/m (the reference
+parameterizes k = max_force / radius, so max_force = k_n * r = 0.75 N) with
+restitution 0.5, collision cells one diameter wide, 8 grains per cell, and
+a walled box whose lowest lattice layer rests one radius above the floor.
+Lattice spacing is 1.02 d with a seeded uniform jitter of +-0.02 d.
+"""
+from __future__ import annotations
+
+import math
+
+from ._capi import (WD_BACKWARD_EULER_PICARD, WD_INIT_CUBE, WD_PERTURB_GAUSSIAN,
+                    WD_SEMI_IMPLICIT_EULER, WdConfig, default_config)
+
+DIAMETER = 0.01
+RADIUS = DIAMETER / 2
+DENSITY = 2500.0
+GRAIN_MASS = DENSITY * 4.0 / 3.0 * math.pi * RADIUS ** 3
+K_N = 150.0
+SPACING = 1.02
+
+
+def column(nx: int, ny: int, nz: int, *, half_width: float = 0.9, ceiling: float = 0.5,
+           friction: float = 0.5, dt: float = 1e-3, density_nodes: int = 32,
+           integrator: int = WD_SEMI_IMPLICIT_EULER, picard_iterations: int = 3,
+           seed: int = 12345) -> WdConfig:
+    """A jittered nx x ny x nz lattice column centred on the z axis, resting on
+    the floor of a (2*half_width)^2 box of height `ceiling`."""
+    c = default_config()
+    c.dt = dt
+    cells_xy = int(round(2 * half_width / DIAMETER))
+    cells_z = int(round(ceiling / DIAMETER))
+    c.collision_grid_size_x = cells_xy
+    c.collision_grid_size_y = cells_xy
+    c.collision_grid_size_z = cells_z
+    c.particles_per_cell = 8
+    c.threads_per_block = 256
+    c.density_grid_size_x = c.density_grid_size_y = c.density_grid_size_z = density_nodes
+    c.x_min, c.x_max = -half_width, half_width
+    c.y_min, c.y_max = -half_width, half_width
+    c.z_min, c.z_max = 0.0, ceiling
+    c.gravity = 9.81
+    c.particle_radius = RADIUS
+    c.max_force = K_N * RADIUS
+    c.floor_z, c.ceiling_z = 0.0, ceiling
+    c.wall_x_min, c.wall_x_max = -half_width, half_width
+    c.wall_y_min, c.wall_y_max = -half_width, half_width
+    c.restitution = 0.5
+    c.friction = friction
+    c.set_masses([0.0, GRAIN_MASS])
+    c.init_type = WD_INIT_CUBE
+    pitch = 2 * SPACING * RADIUS
+    c.init_x0 = -0.5 * (nx - 1) * pitch
+    c.init_y0 = -0.5 * (ny - 1) * pitch
+    c.init_z0 = RADIUS
+    c.init_vx0 = c.init_vy0 = c.init_vz0 = 0.0
+    c.init_jitter = 0.04
+    c.init_seed = seed
+    c.init_perturbation = 0.0
+    # n_a = int(L_a / d): pick L_a in the middle of [n_a d, (n_a + 1) d).
+    c.cube_length_x = (nx + 0.5) * DIAMETER
+    c.cube_length_y = (ny + 0.5) * DIAMETER
+    c.cube_length_z = (nz + 0.5) * DIAMETER
+    c.particle_material = 1
+    c.lattice_spacing = SPACING
+    c.time_integrator = integrator
+    c.picard_iterations = picard_iterations
+    c.kernel_timing = 0
+    return c
+
+
+def column_collapse_4k(**kw) -> WdConfig:
+    """4,096 grains (16^3), 1.8 m box, mu 0.5, SIE dt 1e-3, 32^3 CIC."""
+    return column(16, 16, 16, **kw)
+
+
+def column_collapse_32k(friction: float = 0.5, picard: bool = False, **kw) -> WdConfig:
+    """32,768 grains (32^3), 1.8 m box, 64^3 CIC; SIE or Picard (3 iterations)."""
+    return column(32, 32, 32, friction=friction, density_nodes=64,
+                  integrator=WD_BACKWARD_EULER_PICARD if picard else WD_SEMI_IMPLICIT_EULER,
+                  **kw)
+
+
+def column_8k(friction: float = 0.3, **kw) -> WdConfig:
+    """8,192 grains (16 x 16 x 32), the adjoint window's column."""
+    return column(16, 16, 32, friction=friction, **kw)
+
+
+def column_collapse_1m(**kw) -> WdConfig:
+    """1,048,576 grains (128 x 128 x 64) in a 4 x 4 m box, 128^3 CIC."""
+    return column(128, 128, 64, half_width=2.0, ceiling=0.8, density_nodes=128, **kw)
+
+
+def sliding_grain(friction: float = 0.5, vx0: float = 1.0, dt: float = 1e-5) -> WdConfig:
+    """One grain launched along the floor at vx0 (analytic deceleration -mu g)."""
+    from ._capi import WD_INIT_SINGLE_PARTICLE
+
+    c = column(1, 1, 1, friction=friction, dt=dt)
+    c.init_type = WD_INIT_SINGLE_PARTICLE
+    c.init_x0, c.init_y0 = -0.5, 0.0
+    c.init_z0 = RADIUS - 1e-5
+    c.init_vx0 = vx0
+    return c
+
+
+def perturbed(c: WdConfig, sigma_m: float, seed: int) -> WdConfig:
+    """Adds an isotropic Gaussian offset of standard deviation sigma_m to every grain."""
+    out = c.copy()
+    out.init_perturbation = sigma_m / RADIUS
+    out.perturbation_distribution = WD_PERTURB_GAUSSIAN
+    out.init_perturbation_seed = seed
+    return out
