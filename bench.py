@@ -37,9 +37,10 @@ B_STEP = 132.0
 B_FORCE = 56.0
 
 
-def cic_bytes_per_step(n: int, nodes: int, every: int) -> float:
-    # read position 12 + velocity 12 + material 4 per grain, write 16 B per node
-    return (28.0 * n + 16.0 * nodes) / every
+def cic_bytes_per_grain_step(n: int, nodes: int, every: int) -> float:
+    # read position 12 + velocity 12 + material 4 per grain, write 16 B per
+    # node, once every `every` steps
+    return (28.0 + 16.0 * nodes / n) / every
 
 
 def run_gpu(cfg, steps: int, warmup: int, cic_every: int, energy: bool, profile_steps: int):
@@ -135,7 +136,7 @@ def main() -> int:
     n = main_run["n"]
     value = n * args.steps / main_run["elapsed"]
     nodes = 64 ** 3
-    b_total = B_STEP + cic_bytes_per_step(n, nodes, 40)
+    b_total = B_STEP + cic_bytes_per_grain_step(n, nodes, 40)
     out = {
         "metric": "grain-steps/s (32,768-grain column collapse, SIE, mu 0.5, energy every step, "
                   "64^3 CIC every 40 steps)",
@@ -165,7 +166,7 @@ def main() -> int:
         big = workloads.column_collapse_1m()
         lr = run_gpu(big, args.large_steps, 10, 100, False, 20)
         lv = lr["n"] * args.large_steps / lr["elapsed"]
-        lb = B_STEP + cic_bytes_per_step(lr["n"], 128 ** 3, 100)
+        lb = B_STEP + cic_bytes_per_grain_step(lr["n"], 128 ** 3, 100)
         out["large"] = {
             "workload": "column_collapse_1m", "grains": lr["n"], "steps": args.large_steps,
             "value": lv, "unit": "grain-steps/s", "ms_per_step": 1e3 * lr["elapsed"] / args.large_steps,

@@ -140,6 +140,13 @@ int wd_initial_state(const wd_config* cfg, float* state_out, int* material_out, 
  * buffer (ParticleDynamics::ParticleDynamics, src/particle_dynamics.cu:10-94). */
 int wd_create(const wd_config* cfg, wd_sim** out);
 
+/* Constructs a simulation from caller-supplied grains instead of the
+ * configured initializer (state: 6n floats, material: n ids, original
+ * order); the file initializer's path (src/particle_dynamics.cu:212-235)
+ * and ParticleDynamics::resize (src/particle_dynamics.cu:97-106) use it. */
+int wd_create_from_state(const wd_config* cfg, int n, const float* state, const int* material,
+                         wd_sim** out);
+
 /* Releases the handle (ParticleDynamics::~ParticleDynamics). */
 int wd_destroy(wd_sim* sim);
 

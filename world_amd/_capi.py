@@ -97,6 +97,8 @@ _SIGNATURES = [
     ("wd_config_grain_count", C.c_int, [C.POINTER(WdConfig), C.POINTER(C.c_int)]),
     ("wd_initial_state", C.c_int, [C.POINTER(WdConfig), C.c_void_p, C.c_void_p, C.c_int]),
     ("wd_create", C.c_int, [C.POINTER(WdConfig), C.POINTER(C.c_void_p)]),
+    ("wd_create_from_state", C.c_int, [C.POINTER(WdConfig), C.c_int, C.c_void_p, C.c_void_p,
+                                       C.POINTER(C.c_void_p)]),
     ("wd_destroy", C.c_int, [C.c_void_p]),
     ("wd_num_grains", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_get_state", C.c_int, [C.c_void_p, C.c_void_p]),

@@ -36,6 +36,7 @@ struct Params {
     float kneg;            // -max_force / radius (or -normal_stiffness)
     float spring_k;        // k of the energy's quadratic branch
     float clamp_delta;     // overlap where the force clamp engages
+    float cut2;            // squared centre distance above which a pair cannot touch
     float mu;
     float dt;
     int dgx, dgy, dgz;     // density grid

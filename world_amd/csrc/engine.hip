@@ -67,7 +67,7 @@ struct wd_sim {
     float* st6 = nullptr;
     int* mat_d = nullptr;
     int nblocks = 0;
-    double* partials = nullptr;
+    BlockPart* parts = nullptr;
     double* energy_log = nullptr;
     int energy_cap = 0;
     bool energy_on = false;
@@ -103,7 +103,7 @@ struct wd_sim {
         for (auto e : spare) (void)hipEventDestroy(e);
         spare.clear();
         void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], key, slot, count,
-                        start, tmp_id, status, st6, mat_d, partials, energy_log, cic_acc,
+                        start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
@@ -111,7 +111,7 @@ struct wd_sim {
         dp = nullptr; ctl = nullptr; Tp = Tv = Sp = Sv = nullptr;
         NP[0] = NP[1] = NV[0] = NV[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
-        st6 = nullptr; mat_d = nullptr; partials = nullptr; energy_log = nullptr;
+        st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -202,10 +202,21 @@ struct wd_sim {
 
     template <int MODE, bool ENERGY>
     void launch_force(const float4* np, const float4* nv, float* rhs) {
+        const int gb = grain_blocks(n);
         timed("k_force", [&] {
-            hipLaunchKernelGGL((k_force<MODE, ENERGY>), dim3(grain_blocks(n)), dim3(kBlock), 0,
-                               stream, dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, partials,
-                               energy_log, n);
+            hipLaunchKernelGGL((k_force<MODE, ENERGY>), dim3(gb), dim3(kBlock), 0, stream, dp, ctl,
+                               Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, n);
+        });
+        int flags = 0;
+        if (MODE != kModeRHS) flags = kFinBox | kFinAccel | (ENERGY ? kFinEnergyLog : 0);
+        else if (ENERGY) flags = kFinEnergyLast;
+        if (flags) finalize(gb, flags);
+    }
+
+    void finalize(int nparts, int flags) {
+        timed("k_finalize", [&] {
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, stream, ctl, parts, nparts,
+                               energy_log, flags);
         });
     }
 
@@ -213,13 +224,14 @@ struct wd_sim {
     void enqueue_cic(float* out) {
         const int zb = static_cast<int>(std::min<size_t>((cic_count + kBlock - 1) / kBlock, 4096));
         timed("k_cic_zero", [&] {
-            hipLaunchKernelGGL(k_cic_zero, dim3(zb), dim3(kBlock), 0, stream, cic_acc, cic_count, ctl);
+            hipLaunchKernelGGL(k_cic_zero, dim3(zb), dim3(kBlock), 0, stream, cic_acc, cic_count);
         });
         if (n > 0) {
+            const int sb = (n + kBlock - 1) / kBlock;
             timed("k_cic_scale", [&] {
-                hipLaunchKernelGGL(k_cic_scale, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                                   stream, dp, ctl, Tp, Tv, n);
+                hipLaunchKernelGGL(k_cic_scale, dim3(sb), dim3(kBlock), 0, stream, dp, parts, Tp, Tv, n);
             });
+            finalize(sb, kFinCicScale);
             timed("k_cic_deposit", [&] {
                 hipLaunchKernelGGL(k_cic_deposit, dim3(grain_blocks(n)), dim3(kBlock), 0, stream, dp,
                                    ctl, Tp, Tv, cic_acc, n);
@@ -375,6 +387,10 @@ void fill_params(const wd_config& c, int n, Params& P) {
         P.clamp_delta = c.particle_radius;
     }
     P.spring_k = k;
+    // (2r)^2 with a 1e-5 relative margin: a pair whose FP32 squared distance
+    // exceeds it has norm - r - r > 0 after rounding, so its spring, dashpot,
+    // friction and energy terms are all exactly zero.
+    P.cut2 = static_cast<float>(4.0 * c.particle_radius * static_cast<double>(c.particle_radius) * (1.0 + 1e-5));
     P.mu = c.friction;
     P.dt = c.dt;
     P.dgx = c.density_grid_size_x;
@@ -421,7 +437,7 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->st6, static_cast<size_t>(6) * n));
     WD_TRY(dalloc(&s->mat_d, n));
     s->nblocks = grain_blocks(n);
-    WD_TRY(dalloc(&s->partials, s->nblocks));
+    WD_TRY(dalloc(&s->parts, s->nblocks));
     s->cic_count = static_cast<size_t>(4) * c.density_grid_size_x * c.density_grid_size_y *
                    c.density_grid_size_z;
     WD_TRY(dalloc(&s->cic_acc, s->cic_count));
@@ -430,10 +446,7 @@ int allocate(wd_sim* s) {
     WD_HIP(hipMemset(s->ctl, 0, sizeof(Ctl)));
     WD_HIP(hipMemset(s->count, 0, s->scan_cells * sizeof(uint32_t)));
     WD_HIP(hipMemset(s->status, 0, s->ntiles_max * sizeof(unsigned long long)));
-    Ctl init;
-    std::memset(&init, 0, sizeof(init));
-    for (int k = 0; k < 3; ++k) { init.bbox_acc[k] = 0x7FFFFFFF; init.bbox_acc[k + 3] = -0x7FFFFFFF - 1; }
-    WD_HIP(hipMemcpy(s->ctl, &init, sizeof(Ctl), hipMemcpyHostToDevice));
+
     return upload_params(s);
 }
 
@@ -444,8 +457,10 @@ int ingest(wd_sim* s, const float* st, const int* mat) {
     WD_HIP(hipStreamSynchronize(s->stream));
     WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
     if (mat) WD_HIP(hipMemcpy(s->mat_d, mat, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_ingest, dim3(grain_blocks(s->n)), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
-                       s->st6, s->mat_d, s->Tp, s->Tv, s->key, s->n);
+    const int gb = grain_blocks(s->n);
+    hipLaunchKernelGGL(k_ingest, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->parts, s->st6,
+                       s->mat_d, s->Tp, s->Tv, s->key, s->n);
+    s->finalize(gb, kFinBox);
     WD_HIP(hipGetLastError());
     return s->sync();
 }
@@ -464,18 +479,15 @@ int export_state(wd_sim* s) {
 #define WD_CHECK_SIM(s) \
     if (!(s)) return fail(WD_ERR_ARG, std::string(__func__) + ": null handle")
 
-extern "C" int wd_create(const wd_config* cfg, wd_sim** out) {
-    if (!cfg || !out) return fail(WD_ERR_ARG, "wd_create: null argument");
+namespace {
+
+int create_with(const wd_config* cfg, int n, const float* st, const int* mat, wd_sim** out) {
     *out = nullptr;
     int rc = validate(*cfg);
     if (rc != WD_OK) return rc;
-    int n = 0;
-    rc = wd_config_grain_count(cfg, &n);
-    if (rc != WD_OK) return rc;
-    std::vector<float> st(static_cast<size_t>(6) * std::max(n, 1));
-    std::vector<int> mat(static_cast<size_t>(std::max(n, 1)));
-    rc = wd_initial_state(cfg, st.data(), mat.data(), n);
-    if (rc != WD_OK) return rc;
+    for (int i = 0; i < n; ++i) {
+        if (mat[i] < 0 || mat[i] >= cfg->num_masses) return fail(WD_ERR_ARG, "material id out of range");
+    }
     wd_sim* s = new wd_sim();
     s->cfg = *cfg;
     s->n = n;
@@ -486,13 +498,36 @@ extern "C" int wd_create(const wd_config* cfg, wd_sim** out) {
         return fail(WD_ERR_HIP, "wd_create: hipStreamCreate failed");
     }
     rc = allocate(s);
-    if (rc == WD_OK) rc = ingest(s, st.data(), mat.data());
+    if (rc == WD_OK) rc = ingest(s, st, mat);
     if (rc != WD_OK) {
         delete s;
         return rc;
     }
     *out = s;
     return WD_OK;
+}
+
+}  // namespace
+
+extern "C" int wd_create(const wd_config* cfg, wd_sim** out) {
+    if (!cfg || !out) return fail(WD_ERR_ARG, "wd_create: null argument");
+    *out = nullptr;
+    int n = 0;
+    int rc = wd_config_grain_count(cfg, &n);
+    if (rc != WD_OK) return rc;
+    std::vector<float> st(static_cast<size_t>(6) * std::max(n, 1));
+    std::vector<int> mat(static_cast<size_t>(std::max(n, 1)));
+    rc = wd_initial_state(cfg, st.data(), mat.data(), n);
+    if (rc != WD_OK) return rc;
+    return create_with(cfg, n, st.data(), mat.data(), out);
+}
+
+extern "C" int wd_create_from_state(const wd_config* cfg, int n, const float* state,
+                                    const int* material, wd_sim** out) {
+    if (!cfg || !out || n < 0 || (n > 0 && (!state || !material))) {
+        return fail(WD_ERR_ARG, "wd_create_from_state: bad argument");
+    }
+    return create_with(cfg, n, state, material, out);
 }
 
 extern "C" int wd_destroy(wd_sim* s) {

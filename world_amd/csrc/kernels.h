@@ -15,23 +15,37 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 
 // Small device-side control block: counters, the occupied-cell bounding box
-// and per-pass results.  Reset points are documented at each field.
+// and per-pass results.  Only k_count (block 0) and k_finalize write it
+// outside of atomics, so no pass needs a grid-wide fence.
 struct Ctl {
     unsigned tile_counter;     // scan tile dispenser; zeroed by k_count
     unsigned epoch;            // scan look-back tag; bumped by k_count
-    unsigned done_force;       // last-block detection; self-resetting
-    unsigned done_ingest;      // last-block detection; self-resetting
-    unsigned done_energy;      // last-block detection; self-resetting
     int bbox_cur[6];           // occupied cells (min xyz, max xyz) of the current state
-    int bbox_acc[6];           // accumulated by the force pass for the next state
-    unsigned accel_acc;        // float bits, max |a| of the running force pass
     unsigned accel_last;       // float bits, max |a| of the last completed force pass
+    unsigned cic_scale_bits;   // float bits, max per-grain deposit magnitude
     unsigned long long overflow_lifetime;  // over-capacity cells, summed over passes
     int error;                 // nonzero: a bounded wait gave up
     int energy_count;          // entries written to the energy log
     int energy_capacity;
+    int pad;
     double energy_last;        // last standalone energy
-    unsigned cic_scale_bits;   // float bits, max per-grain deposit magnitude
+};
+
+// Per-block partial results of a grain pass, reduced by k_finalize.
+struct BlockPart {
+    int box[6];       // min cx, cy, cz / max cx, cy, cz of the block's grains
+    unsigned accel;   // float bits of the block's max |a| (or deposit magnitude)
+    unsigned pad;
+    double energy;    // deterministic block sum of per-grain energies
+};
+
+// What k_finalize publishes.
+enum FinalizeFlags {
+    kFinBox = 1,        // bbox_cur <- reduced box
+    kFinAccel = 2,      // accel_last <- reduced max
+    kFinEnergyLog = 4,  // append the energy sum to the log
+    kFinEnergyLast = 8, // energy_last <- energy sum
+    kFinCicScale = 16   // cic_scale_bits <- reduced max
 };
 
 // Launch geometry helpers: grid rounded up to a multiple of the XCD count,

@@ -55,60 +55,6 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
     return v;
 }
 
-__device__ __forceinline__ int atomic_load_i(int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned atomic_load_u(unsigned* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Block-wide reduction of the occupied-cell box, applied to ctl->bbox_acc
-// with atomics that are skipped when they cannot change the value.
-__device__ void reduce_bbox(Ctl* ctl, int lo[3], int hi[3]) {
-    __shared__ int s_box[16][6];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    int v[6] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
-                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2])};
-    if (lane == 0) for (int k = 0; k < 6; ++k) s_box[wave][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < nw; ++w) {
-            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
-            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
-        }
-        for (int k = 0; k < 3; ++k) {
-            if (v[k] < atomic_load_i(&ctl->bbox_acc[k])) atomicMin(&ctl->bbox_acc[k], v[k]);
-        }
-        for (int k = 3; k < 6; ++k) {
-            if (v[k] > atomic_load_i(&ctl->bbox_acc[k])) atomicMax(&ctl->bbox_acc[k], v[k]);
-        }
-    }
-}
-
-// Called by the last block of a pass: publish the accumulated box as the
-// current one and re-arm the accumulator.
-__device__ void publish_bbox(Ctl* ctl) {
-    for (int k = 0; k < 6; ++k) ctl->bbox_cur[k] = atomic_load_i(&ctl->bbox_acc[k]);
-    for (int k = 0; k < 3; ++k) ctl->bbox_acc[k] = kEmptyMin;
-    for (int k = 3; k < 6; ++k) ctl->bbox_acc[k] = kEmptyMax;
-}
-
-// Returns true in every thread of the block that finished last.  Thread 0's
-// prior global writes are released before the counter increments.
-__device__ bool last_block(unsigned* done) {
-    __shared__ bool s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(done, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) __threadfence();
-    return s_last;
-}
-
 // Deterministic block sum of one double per thread (fixed butterfly and wave
 // order); the result is valid in thread 0.
 __device__ double block_sum(double v) {
@@ -126,11 +72,43 @@ __device__ double block_sum(double v) {
     return total;
 }
 
+// Reduces the block's box, max and energy and stores them, with plain
+// stores, in parts[slot] (thread 0).
+__device__ void store_block_part(BlockPart* parts, int slot, const int lo[3], const int hi[3],
+                                 unsigned accel, double energy, bool with_energy) {
+    __shared__ int s_box[16][7];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    int v[7] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
+                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]),
+                static_cast<int>(wave_max_u(accel))};
+    if (lane == 0) {
+        for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
+    }
+    const double e = with_energy ? block_sum(energy) : 0.0;  // contains __syncthreads
+    if (!with_energy) __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned am = static_cast<unsigned>(s_box[0][6]);
+        for (int w = 1; w < nw; ++w) {
+            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
+            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
+            am = max(am, static_cast<unsigned>(s_box[w][6]));
+        }
+        BlockPart bp;
+        for (int k = 0; k < 6; ++k) bp.box[k] = v[k];
+        bp.accel = am;
+        bp.pad = 0;
+        bp.energy = e;
+        parts[slot] = bp;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Ingest an original-order AoS state: grain i gets id i.
-__global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp, Ctl* ctl,
+__global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp,
+                                                   BlockPart* __restrict__ parts,
                                                    const float* __restrict__ st6,
                                                    const int* __restrict__ mat,
                                                    float4* __restrict__ Tp, float4* __restrict__ Tv,
@@ -150,11 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
         lo[1] = hi[1] = cy;
         lo[2] = hi[2] = cz;
     }
-    reduce_bbox(ctl, lo, hi);
-    if (last_block(&ctl->done_ingest) && threadIdx.x == 0) {
-        publish_bbox(ctl);
-        ctl->done_ingest = 0;
-    }
+    store_block_part(parts, logical_block(), lo, hi, 0u, 0.0, false);
 }
 
 // Original-order material ids pushed into the working layout.
@@ -218,6 +192,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     __shared__ uint32_t s_over[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (static_cast<int>(blockIdx.x) >= ntiles) return;  // never takes a tile
     for (;;) {
         if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(&ctl->tile_counter, 1u));
         __syncthreads();
@@ -372,8 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
-        float* __restrict__ rhs_out, double* __restrict__ partials,
-        double* __restrict__ energy_log, int n) {
+        float* __restrict__ rhs_out, BlockPart* __restrict__ parts, int n) {
     const Params& P = *Pp;
     const int lb = logical_block();
     const int p = lb * blockDim.x + threadIdx.x;
@@ -396,26 +370,37 @@ __global__ __launch_bounds__(kBlock) void k_force(
         int cx, cy, cz;
         grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
         const uint32_t kcap = static_cast<uint32_t>(P.kcap);
-        for (int dx = -1; dx <= 1; ++dx) {
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int base = g.index(cx + dx, cy + dy, cz - 1);
-                uint32_t s[4];
-                s[0] = start[base];
-                s[1] = start[base + 1];
-                s[2] = start[base + 2];
-                s[3] = start[base + 3];
-                for (int c = 0; c < 3; ++c) {
-                    const uint32_t b = s[c];
-                    const uint32_t e = min(s[c + 1], b + kcap);
-                    for (uint32_t q = b; q < e; ++q) {
-                        if (q == static_cast<uint32_t>(p)) continue;
-                        const float4 pj = Sp[q];
-                        float d0, d1, d2;
-                        const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
-                        pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
-                        if (ENERGY && __float_as_int(pj.w) > id) {
-                            e_grain += contact_energy(P, 2.f * P.radius - norm);
-                        }
+        // Cell starts of the 9 stencil columns (cz-1 .. cz+2 each), issued
+        // together so their latencies overlap.
+        uint4 col[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int base = g.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1);
+            __builtin_memcpy(&col[c], start + base, sizeof(uint4));
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const uint32_t s[4] = {col[c].x, col[c].y, col[c].z, col[c].w};
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+                const uint32_t b = s[z];
+                const uint32_t e = min(s[z + 1], b + kcap);
+                for (uint32_t q = b; q < e; ++q) {
+                    const float4 pj = Sp[q];
+                    float d0, d1, d2;
+                    d0 = x[0] - pj.x;
+                    d1 = x[1] - pj.y;
+                    d2 = x[2] - pj.z;
+                    float s2 = 0.f;
+                    s2 += d0 * d0;
+                    s2 += d1 * d1;
+                    s2 += d2 * d2;
+                    // Beyond contact every term is an exact zero; NaN falls through.
+                    if (s2 > P.cut2 || q == static_cast<uint32_t>(p)) continue;
+                    const float norm = sqrtf(s2);
+                    pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
+                    if (ENERGY && __float_as_int(pj.w) > id) {
+                        e_grain += contact_energy(P, 2.f * P.radius - norm);
                     }
                 }
             }
@@ -456,41 +441,53 @@ __global__ __launch_bounds__(kBlock) void k_force(
             if (acc == acc) accel_bits = __float_as_uint(acc);
         }
     }
-    if (MODE != kModeRHS) {
-        reduce_bbox(ctl, lo, hi);
-        const unsigned am = wave_max_u(accel_bits);
-        if ((threadIdx.x & 63) == 0 && am > atomic_load_u(&ctl->accel_acc)) {
-            atomicMax(&ctl->accel_acc, am);
-            __threadfence();  // performed before this block reports completion
-        }
+    if (MODE != kModeRHS || ENERGY) {
+        store_block_part(parts, lb, lo, hi, accel_bits, static_cast<double>(e_grain), ENERGY);
     }
-    if (ENERGY) {
-        const double bs = block_sum(static_cast<double>(e_grain));
-        if (threadIdx.x == 0) partials[lb] = bs;
+}
+
+// One block: reduces the per-block partials of the pass that just ran, in a
+// fixed order, and publishes what `flags` asks for.
+__global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* __restrict__ parts,
+                                                     int nparts, double* __restrict__ energy_log,
+                                                     int flags) {
+    __shared__ int s_box[16][7];
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    unsigned am = 0u;
+    double e = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        const BlockPart& bp = parts[b];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], bp.box[k]);
+            hi[k] = max(hi[k], bp.box[k + 3]);
+        }
+        am = max(am, bp.accel);
+        e += bp.energy;
     }
-    if (MODE == kModeRHS && !ENERGY) return;
-    if (last_block(&ctl->done_force)) {
-        if (ENERGY) {
-            double acc = 0.0;
-            for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += blockDim.x) acc += partials[b];
-            const double total = block_sum(acc);
-            if (threadIdx.x == 0) {
-                if (MODE == kModeRHS) {
-                    ctl->energy_last = total;
-                } else if (ctl->energy_count < ctl->energy_capacity) {
-                    energy_log[ctl->energy_count] = total;
-                    ctl->energy_count = ctl->energy_count + 1;
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            if (MODE != kModeRHS) {
-                publish_bbox(ctl);
-                ctl->accel_last = atomic_load_u(&ctl->accel_acc);
-                ctl->accel_acc = 0u;
-            }
-            ctl->done_force = 0;
-        }
+    const double total = block_sum(e);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int v[7] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
+                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]), static_cast<int>(wave_max_u(am))};
+    if (lane == 0) {
+        for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    unsigned amax = static_cast<unsigned>(s_box[0][6]);
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
+        for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
+        amax = max(amax, static_cast<unsigned>(s_box[w][6]));
+    }
+    if (flags & kFinBox) {
+        for (int k = 0; k < 6; ++k) ctl->bbox_cur[k] = v[k];
+    }
+    if (flags & kFinAccel) ctl->accel_last = amax;
+    if (flags & kFinCicScale) ctl->cic_scale_bits = amax;
+    if (flags & kFinEnergyLast) ctl->energy_last = total;
+    if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
+        energy_log[ctl->energy_count] = total;
+        ctl->energy_count = ctl->energy_count + 1;
     }
 }
 
@@ -507,8 +504,8 @@ template __global__ void k_reorder<2>(const Ctl*, const uint32_t*, const uint32_
 #define WD_INST_FORCE(M, E)                                                                    \
     template __global__ void k_force<M, E>(const Params*, Ctl*, const float4*, const float4*,  \
                                            const float4*, const float4*, const uint32_t*,      \
-                                           float4*, float4*, uint32_t*, float*, double*,       \
-                                           double*, int);
+                                           float4*, float4*, uint32_t*, float*, BlockPart*,    \
+                                           int);
 WD_INST_FORCE(kModeSIE, false)
 WD_INST_FORCE(kModeSIE, true)
 WD_INST_FORCE(kModeBEP, false)
@@ -577,15 +574,15 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
 // integer sums do not depend on the order of the atomics, so the grid is
 // bitwise reproducible.
 __global__ __launch_bounds__(kBlock) void k_cic_zero(unsigned long long* __restrict__ acc,
-                                                     size_t count, Ctl* ctl) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->cic_scale_bits = 0u;
+                                                     size_t count) {
     for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
          k += static_cast<size_t>(gridDim.x) * blockDim.x) {
         acc[k] = 0ull;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__ Pp, Ctl* ctl,
+__global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__ Pp,
+                                                      BlockPart* __restrict__ parts,
                                                       const float4* __restrict__ Tp,
                                                       const float4* __restrict__ Tv, int n) {
     const Params& P = *Pp;
@@ -602,10 +599,8 @@ __global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__
                             isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(m);
         bits = (m == 0.f) ? 0u : (finite ? __float_as_uint(mag) : 0x7F800000u);
     }
-    const unsigned bm = wave_max_u(bits);
-    if ((threadIdx.x & 63) == 0 && bm > atomic_load_u(&ctl->cic_scale_bits)) {
-        atomicMax(&ctl->cic_scale_bits, bm);
-    }
+    const int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    store_block_part(parts, blockIdx.x, lo, hi, bits, 0.0, false);
 }
 
 // Fixed-point exponent S for n grains of largest magnitude M: n*M*2^S < 2^62.

@@ -2,10 +2,15 @@
 
 Material and box shared by all of them: monodisperse spheres of diameter
 d = 0.01 m and density 2500 kg/m^3 (m = 1.309e-3 kg), g = 9.81, a linear
-spring-dashpot normal contact of stiffness k_n = 150 N/m (the reference
-parameterizes k = max_force / radius, so max_force = k_n * r = 0.75 N) with
-restitution 0.5, collision cells one diameter wide, 8 grains per cell, and
-a walled box whose lowest lattice layer rests one radius above the floor.
+spring-dashpot normal contact of stiffness k_n = 150 N/m with restitution
+0.5, collision cells one diameter wide, 8 grains per cell, and a walled box
+whose lowest lattice layer rests one radius above the floor.
+
+The spring is set through physics.normal_stiffness with the force clamp
+(max_force) far above any load these columns carry.  The reference's own
+parameterization k = max_force / radius would put the clamp at k_n * r =
+0.75 N, below the 0.82 N weight of a 64-grain stack, so floor contacts under
+the large column would saturate and let grains sink through the floor.
 Lattice spacing is 1.02 d with a seeded uniform jitter of +-0.02 d.
 """
 from __future__ import annotations
@@ -20,6 +25,7 @@ RADIUS = DIAMETER / 2
 DENSITY = 2500.0
 GRAIN_MASS = DENSITY * 4.0 / 3.0 * math.pi * RADIUS ** 3
 K_N = 150.0
+MAX_FORCE = 1000.0  # clamp on the normal spring force, never reached here
 SPACING = 1.02
 
 
@@ -32,7 +38,7 @@ def column(nx: int, ny: int, nz: int, *, half_width: float = 0.9, ceiling: float
     c = default_config()
     c.dt = dt
     cells_xy = int(round(2 * half_width / DIAMETER))
-    cells_z = int(round(ceiling / DIAMETER))
+    cells_z = int(round(ceiling / DIAMETER)) + 2
     c.collision_grid_size_x = cells_xy
     c.collision_grid_size_y = cells_xy
     c.collision_grid_size_z = cells_z
@@ -41,10 +47,13 @@ def column(nx: int, ny: int, nz: int, *, half_width: float = 0.9, ceiling: float
     c.density_grid_size_x = c.density_grid_size_y = c.density_grid_size_z = density_nodes
     c.x_min, c.x_max = -half_width, half_width
     c.y_min, c.y_max = -half_width, half_width
-    c.z_min, c.z_max = 0.0, ceiling
+    # The cell domain reaches two diameters below the floor: soft contacts
+    # under a tall column push the lowest centres slightly through it.
+    c.z_min, c.z_max = -2 * DIAMETER, ceiling
     c.gravity = 9.81
     c.particle_radius = RADIUS
-    c.max_force = K_N * RADIUS
+    c.normal_stiffness = K_N
+    c.max_force = MAX_FORCE
     c.floor_z, c.ceiling_z = 0.0, ceiling
     c.wall_x_min, c.wall_x_max = -half_width, half_width
     c.wall_y_min, c.wall_y_max = -half_width, half_width
