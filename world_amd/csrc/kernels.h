@@ -13,6 +13,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
+constexpr int kContactList = 16;     // touching candidates buffered per grain in LDS
 
 // Small device-side control block: counters, the occupied-cell bounding box
 // and per-pass results.  Only k_count (block 0) and k_finalize write it

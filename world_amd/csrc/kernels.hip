@@ -349,6 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
         float* __restrict__ rhs_out, BlockPart* __restrict__ parts, int n) {
     const Params& P = *Pp;
+    __shared__ uint32_t s_list[kContactList][kBlock];
     const int lb = logical_block();
     const int p = lb * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
@@ -378,33 +379,57 @@ __global__ __launch_bounds__(kBlock) void k_force(
             const int base = g.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1);
             __builtin_memcpy(&col[c], start + base, sizeof(uint4));
         }
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const uint32_t s[4] = {col[c].x, col[c].y, col[c].z, col[c].w};
-#pragma unroll
-            for (int z = 0; z < 3; ++z) {
-                const uint32_t b = s[z];
-                const uint32_t e = min(s[z + 1], b + kcap);
-                for (uint32_t q = b; q < e; ++q) {
-                    const float4 pj = Sp[q];
-                    float d0, d1, d2;
-                    d0 = x[0] - pj.x;
-                    d1 = x[1] - pj.y;
-                    d2 = x[2] - pj.z;
-                    float s2 = 0.f;
-                    s2 += d0 * d0;
-                    s2 += d1 * d1;
-                    s2 += d2 * d2;
-                    // Beyond contact every term is an exact zero; NaN falls through.
-                    if (s2 > P.cut2 || q == static_cast<uint32_t>(p)) continue;
-                    const float norm = sqrtf(s2);
-                    pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
-                    if (ENERGY && __float_as_int(pj.w) > id) {
-                        e_grain += contact_energy(P, 2.f * P.radius - norm);
-                    }
+        // Phase 2: the full contact law for the listed candidates, in the
+        // order phase 1 found them (the canonical stencil order).
+        int listed = 0;
+        auto flush = [&]() {
+            for (int k = 0; k < listed; ++k) {
+                const uint32_t q = s_list[k][threadIdx.x];
+                const float4 pj = Sp[q];
+                float d0, d1, d2;
+                const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
+                pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
+                if (ENERGY && __float_as_int(pj.w) > id) {
+                    e_grain += contact_energy(P, 2.f * P.radius - norm);
                 }
             }
+            listed = 0;
+        };
+        // Phase 1: squared distances only.  A candidate beyond contact adds
+        // exact zeros in the reference's arithmetic, so only touching (or
+        // NaN) candidates are listed; a full list is flushed in place.
+        auto probe = [&](uint32_t q, float4 pj) {
+            const float d0 = x[0] - pj.x, d1 = x[1] - pj.y, d2 = x[2] - pj.z;
+            float s2 = 0.f;
+            s2 += d0 * d0;
+            s2 += d1 * d1;
+            s2 += d2 * d2;
+            if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
+                s_list[listed][threadIdx.x] = q;
+                if (++listed == kContactList) flush();
+            }
+        };
+        auto scan_range = [&](uint32_t b, uint32_t e) {
+            uint32_t q = b;
+            for (; q + 1 < e; q += 2) {
+                const float4 a0 = Sp[q], a1 = Sp[q + 1];
+                probe(q, a0);
+                probe(q + 1, a1);
+            }
+            if (q < e) probe(q, Sp[q]);
+        };
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const uint32_t s0 = col[c].x, s1 = col[c].y, s2 = col[c].z, s3 = col[c].w;
+            if (s1 - s0 <= kcap && s2 - s1 <= kcap && s3 - s2 <= kcap) {
+                scan_range(s0, s3);  // the column's three cells are contiguous
+            } else {
+                scan_range(s0, min(s1, s0 + kcap));
+                scan_range(s1, min(s2, s1 + kcap));
+                scan_range(s2, min(s3, s2 + kcap));
+            }
         }
+        flush();
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
         if (MODE == kModeRHS) {
             if (rhs_out) {
