@@ -235,6 +235,15 @@ int wd_stage_ms(wd_sim* sim, float out[4]);
 
 /* ---- parity / inspection entry points ---------------------------------- */
 
+/* Extension, tangential_model spring_history: the stored contact springs in
+ * original grain order, WD_HISTORY_SLOTS per grain: partner[i*S + k] is the
+ * partner's original index (walls: -1..-6 for x-lower, x-upper, y-lower,
+ * y-upper, floor, ceiling), xi[(i*S + k)*3 + a] the spring displacement,
+ * count[i] the number of valid slots.  The reference's friction is stateless
+ * (src/compute_rhs_kernel.cu:50-65); this has no counterpart there. */
+#define WD_HISTORY_SLOTS 24
+int wd_get_history(wd_sim* sim, int* partner, float* xi, int* count);
+
 /* Runs one neighbor pass on the current state and returns the cell order:
  * ids_out[p] = original index of the grain at cell-sorted position p, and
  * keys_out[p] = its flat collision cell (cx*Gy + cy)*Gz + cz

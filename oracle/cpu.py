@@ -51,6 +51,24 @@ def lib() -> C.CDLL:
         h.or_grain_energy.restype = None
         h.or_density_grid.argtypes = [P, C.c_int, vp, vp, C.c_int, vp]
         h.or_density_grid.restype = None
+        h.or_sim_create.argtypes = [P, C.c_int, vp, vp]
+        h.or_sim_create.restype = vp
+        h.or_sim_destroy.argtypes = [vp]
+        h.or_sim_destroy.restype = None
+        h.or_sim_set_dt.argtypes = [vp, C.c_float]
+        h.or_sim_set_dt.restype = None
+        h.or_sim_steps.argtypes = [vp, C.c_int, C.c_int]
+        h.or_sim_steps.restype = None
+        h.or_sim_state.argtypes = [vp, vp]
+        h.or_sim_state.restype = None
+        h.or_sim_overflow.argtypes = [vp]
+        h.or_sim_overflow.restype = C.c_longlong
+        h.or_sim_max_accel.argtypes = [vp]
+        h.or_sim_max_accel.restype = C.c_float
+        h.or_sim_history.argtypes = [vp, vp, vp, vp]
+        h.or_sim_history.restype = None
+        h.or_sim_energy.argtypes = [vp, C.c_int]
+        h.or_sim_energy.restype = C.c_double
         _lib = h
     return _lib
 
@@ -142,6 +160,58 @@ def density_grid(cfg, state, material, precision: int = 32) -> np.ndarray:
     out = np.zeros(4 * nodes, np.float64)
     lib().or_density_grid(C.byref(cfg), n, _p(st), _p(mat), precision, _p(out))
     return out
+
+
+HIST_SLOTS = 24
+
+
+class Sim:
+    """Stateful oracle simulation: FP32 state in original order plus the
+    contact history of the spring_history extension."""
+
+    def __init__(self, cfg, state, material):
+        st = _st(state)
+        self.n = st.size // 6
+        self.cfg = cfg.copy()
+        mat = np.ascontiguousarray(material, np.int32)
+        self._h = lib().or_sim_create(C.byref(self.cfg), self.n, _p(st), _p(mat))
+
+    def close(self):
+        if self._h:
+            lib().or_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_dt(self, dt: float) -> None:
+        lib().or_sim_set_dt(self._h, dt)
+
+    def steps(self, k: int, threads: int = 8) -> None:
+        lib().or_sim_steps(self._h, int(k), int(threads))
+
+    def state(self) -> np.ndarray:
+        out = np.zeros(6 * max(self.n, 1), np.float32)
+        lib().or_sim_state(self._h, _p(out))
+        return out[: 6 * self.n].reshape(self.n, 6)
+
+    def overflow(self) -> int:
+        return int(lib().or_sim_overflow(self._h))
+
+    def max_accel(self) -> float:
+        return float(lib().or_sim_max_accel(self._h))
+
+    def energy(self, precision: int = 32) -> float:
+        return float(lib().or_sim_energy(self._h, precision))
+
+    def history(self):
+        partner = np.zeros(max(self.n, 1) * HIST_SLOTS, np.int32)
+        xi = np.zeros(max(self.n, 1) * HIST_SLOTS * 3, np.float32)
+        cnt = np.zeros(max(self.n, 1), np.int32)
+        lib().or_sim_history(self._h, _p(partner), _p(xi), _p(cnt))
+        n = self.n
+        return (partner[: n * HIST_SLOTS].reshape(n, HIST_SLOTS),
+                xi[: n * HIST_SLOTS * 3].reshape(n, HIST_SLOTS, 3), cnt[:n])
 
 
 # -- the reference's own YAML loader (built from the reference sources) ------

@@ -24,6 +24,7 @@
 //   or_energy         closed-form free-fall energy, piecewise contact energy
 //   or_density_grid   node / cell-centre / momentum closed forms, mass
 //                     conservation
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -77,39 +78,45 @@ long long flat(const Grid& g, int x, int y, int z) {
     return (static_cast<long long>(x) * g.gy + y) * g.gz + z;
 }
 
-// Canonical cell table: members of each occupied cell in ascending index.
+// Canonical cell table: grains sorted by (flat cell, index), so a cell's
+// members are a contiguous run in ascending index.  Sparse: its cost depends
+// on the number of grains, not on the size of the domain grid.
 struct Bins {
     Grid g;
-    std::vector<int> first;   // dense per cell: offset into members, or -1
-    std::vector<int> count;   // dense per cell
-    std::vector<int> members;
-    long long over = 0;       // cells holding more than k grains
+    std::vector<long long> keys;  // sorted flat cells, one per grain
+    std::vector<int> members;     // grain indices in the same order
+    long long over = 0;           // cells holding more than k grains
+
+    // [first, first + count) of cell f in members.
+    void run(long long f, int& first, int& count) const {
+        const auto lo = std::lower_bound(keys.begin(), keys.end(), f);
+        const auto hi = std::upper_bound(lo, keys.end(), f);
+        first = static_cast<int>(lo - keys.begin());
+        count = static_cast<int>(hi - lo);
+    }
 };
 
 void build_bins(const wd_config& c, int n, const float* st, Bins& b) {
     b.g = grid_of(c);
-    const size_t cells = static_cast<size_t>(b.g.gx) * b.g.gy * b.g.gz;
-    b.first.assign(cells, -1);
-    b.count.assign(cells, 0);
-    std::vector<long long> key(n);
+    std::vector<std::pair<long long, int>> kv(n);
     for (int i = 0; i < n; ++i) {
         int cc[3];
         cell3(b.g, st + 6 * static_cast<size_t>(i), cc);
-        key[i] = flat(b.g, cc[0], cc[1], cc[2]);
-        b.count[key[i]]++;
+        kv[i] = {flat(b.g, cc[0], cc[1], cc[2]), i};
     }
-    int off = 0;
-    for (size_t k = 0; k < cells; ++k) {
-        if (b.count[k]) {
-            b.first[k] = off;
-            off += b.count[k];
-            if (b.count[k] > c.particles_per_cell) b.over++;
-        }
+    std::sort(kv.begin(), kv.end());
+    b.keys.resize(n);
+    b.members.resize(n);
+    b.over = 0;
+    for (int i = 0; i < n; ++i) {
+        b.keys[i] = kv[i].first;
+        b.members[i] = kv[i].second;
     }
-    b.members.assign(n, 0);
-    std::vector<int> fill(cells, 0);
-    for (int i = 0; i < n; ++i) {  // ascending i -> ascending index in a cell
-        b.members[b.first[key[i]] + fill[key[i]]++] = i;
+    for (int i = 0; i < n;) {
+        int j = i;
+        while (j < n && b.keys[j] == b.keys[i]) ++j;
+        if (j - i > c.particles_per_cell) b.over++;
+        i = j;
     }
 }
 
@@ -125,10 +132,11 @@ void for_neighbors(const Bins& b, int k, const float* p, int i, F&& fn) {
             for (int dz = -1; dz <= 1; ++dz) {
                 const int x = cc[0] + dx, y = cc[1] + dy, z = cc[2] + dz;
                 if (x < 0 || x >= b.g.gx || y < 0 || y >= b.g.gy || z < 0 || z >= b.g.gz) continue;
-                const long long f = flat(b.g, x, y, z);
-                const int cnt = b.count[f] < k ? b.count[f] : k;
+                int first = 0, cnt = 0;
+                b.run(flat(b.g, x, y, z), first, cnt);
+                if (cnt > k) cnt = k;
                 for (int s = 0; s < cnt; ++s) {
-                    const int j = b.members[b.first[f] + s];
+                    const int j = b.members[first + s];
                     if (j != i) fn(j);
                 }
             }
@@ -152,9 +160,43 @@ double damping64(double e, double k, double m) {
 
 struct Phys {
     float g, r, fmax, kneg, k, e, mu;
+    float kt, dt;
+    bool history;
     float lower[3], upper[3];
     std::vector<float> mass;
 };
+
+// Contact history (extension): per grain, springs keyed by partner original
+// index (walls -1..-6), in contact order, at most kHistSlots entries.
+constexpr int kHistSlots = 24;
+struct HEntry {
+    int partner;
+    float xi[3];
+};
+using HList = std::vector<HEntry>;
+
+int wall_code(int a, int side) { return -(1 + 2 * a + side); }
+
+template <typename T>
+void hist_find(const HList* h, int partner, T xi[3]) {
+    xi[0] = xi[1] = xi[2] = T(0);
+    if (!h) return;
+    for (const HEntry& e : *h) {
+        if (e.partner == partner) {
+            for (int a = 0; a < 3; ++a) xi[a] = static_cast<T>(e.xi[a]);
+            return;
+        }
+    }
+}
+
+template <typename T>
+void hist_push(HList* h, int partner, const T xi[3]) {
+    if (!h || static_cast<int>(h->size()) >= kHistSlots) return;
+    HEntry e;
+    e.partner = partner;
+    for (int a = 0; a < 3; ++a) e.xi[a] = static_cast<float>(xi[a]);
+    h->push_back(e);
+}
 
 Phys phys_of(const wd_config& c) {
     Phys p;
@@ -170,6 +212,9 @@ Phys phys_of(const wd_config& c) {
     }
     p.e = c.restitution;
     p.mu = c.friction;
+    p.kt = c.tangential_stiffness_ratio * p.k;
+    p.dt = c.dt;
+    p.history = c.tangential_model == WD_TANGENTIAL_SPRING_HISTORY;
     p.lower[0] = c.wall_x_min; p.lower[1] = c.wall_y_min; p.lower[2] = c.floor_z;
     p.upper[0] = c.wall_x_max; p.upper[1] = c.wall_y_max; p.upper[2] = c.ceiling_z;
     p.mass.assign(c.masses, c.masses + c.num_masses);
@@ -214,6 +259,36 @@ void friction(T f[3], const T vt[3], T limit, T c, Branches* br) {
     for (int a = 0; a < 3; ++a) f[a] -= mag * vt[a] / n;
 }
 
+// Tangential spring with history (extension): rotate the stored spring into
+// the tangent plane, add v_t dt, F_t = -k_t xi - c v_t, and on Coulomb slip
+// scale F_t to mu * fp and reset the spring to match.  Same FP32 order as
+// the GPU's spring_friction.
+template <typename T>
+void spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], T kt, T mu, T dt,
+                 Branches* br) {
+    T vt[3];
+    for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
+    T xn = 0;
+    for (int a = 0; a < 3; ++a) xn += xi[a] * n[a];
+    for (int a = 0; a < 3; ++a) xi[a] = xi[a] - xn * n[a];
+    for (int a = 0; a < 3; ++a) xi[a] = xi[a] + vt[a] * dt;
+    T ft[3];
+    for (int a = 0; a < 3; ++a) ft[a] = -(kt * xi[a]) - c * vt[a];
+    T s = 0;
+    for (int a = 0; a < 3; ++a) s += ft[a] * ft[a];
+    const T fm = std::sqrt(s);
+    const T lim = mu * fp;
+    const bool slip = br ? br->take(fm > lim) : (fm > lim);
+    if (slip) {
+        const T scale = lim / fm;
+        for (int a = 0; a < 3; ++a) {
+            ft[a] = ft[a] * scale;
+            xi[a] = -(ft[a] + c * vt[a]) / kt;
+        }
+    }
+    for (int a = 0; a < 3; ++a) f[a] += ft[a];
+}
+
 // max(0, min(fmax, kneg * d)) with fminf/fmaxf semantics (a NaN operand
 // yields the other one) and the branch choices recorded / replayed.
 template <typename T>
@@ -230,7 +305,8 @@ T spring(T fmax, T kneg, T d, Branches* br) {
 // `scale` receives sum |f_term| over every force term (tolerance scale).
 template <typename T>
 void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const float* st,
-                 const int* mat, int i, T out_a[3], Branches* br, double* scale) {
+                 const int* mat, int i, T out_a[3], Branches* br, double* scale,
+                 const HList* oldh = nullptr, HList* newh = nullptr) {
     (void)n;
     const float* si = st + 6 * static_cast<size_t>(i);
     T pos[3], vel[3];
@@ -248,6 +324,7 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
     T c_wall;
     if (sizeof(T) == 4) c_wall = static_cast<T>(damping32(ph.e, ph.k, ph.mass[mi]));
     else c_wall = static_cast<T>(damping64(ph.e, kk, m_i));
+    const T kt = static_cast<T>(ph.kt), dt = static_cast<T>(ph.dt);
     T f[3] = {0, 0, 0};
     double sc = 0.0;
     f[2] -= m_i * static_cast<T>(ph.g);
@@ -262,7 +339,18 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
         const bool cl = br ? br->take(dl < T(0)) : (dl < T(0));
         if (cl) {
             f[a] -= c_wall * vel[a];
-            friction(f, vt, mu * fl, c_wall, br);
+            if (ph.history) {
+                T nrm[3] = {0, 0, 0};
+                nrm[a] = T(1);
+                T vn = 0;
+                for (int q = 0; q < 3; ++q) vn += vel[q] * nrm[q];
+                T xi[3];
+                hist_find(oldh, wall_code(a, 0), xi);
+                spring_fric(f, nrm, vel, vn, fl, c_wall, xi, kt, mu, dt, br);
+                hist_push(newh, wall_code(a, 0), xi);
+            } else {
+                friction(f, vt, mu * fl, c_wall, br);
+            }
             sc += std::fabs(static_cast<double>(c_wall * vel[a])) + std::fabs(static_cast<double>(mu * fl));
         }
         const T du = static_cast<T>(ph.upper[a]) - pos[a] - r;
@@ -272,7 +360,18 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
         const bool cu = br ? br->take(du < T(0)) : (du < T(0));
         if (cu) {
             f[a] -= c_wall * vel[a];
-            friction(f, vt, mu * fu, c_wall, br);
+            if (ph.history) {
+                T nrm[3] = {0, 0, 0};
+                nrm[a] = T(-1);
+                T vn = 0;
+                for (int q = 0; q < 3; ++q) vn += vel[q] * nrm[q];
+                T xi[3];
+                hist_find(oldh, wall_code(a, 1), xi);
+                spring_fric(f, nrm, vel, vn, fu, c_wall, xi, kt, mu, dt, br);
+                hist_push(newh, wall_code(a, 1), xi);
+            } else {
+                friction(f, vt, mu * fu, c_wall, br);
+            }
             sc += std::fabs(static_cast<double>(c_wall * vel[a])) + std::fabs(static_cast<double>(mu * fu));
         }
     }
@@ -314,7 +413,14 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
                 f[a] -= cp * vn * nrm[a];
                 vt[a] = vr[a] - vn * nrm[a];
             }
-            friction(f, vt, mu * fp, cp, br);
+            if (ph.history) {
+                T xi[3];
+                hist_find(oldh, j, xi);
+                spring_fric(f, nrm, vr, vn, fp, cp, xi, kt, mu, dt, br);
+                hist_push(newh, j, xi);
+            } else {
+                friction(f, vt, mu * fp, cp, br);
+            }
             sc += std::fabs(static_cast<double>(cp * vn)) + std::fabs(static_cast<double>(mu * fp));
         }
     });
@@ -334,7 +440,7 @@ T contact_u(T delta, T r, T fmax, T k, T clamp) {
 // src/energy_kernel.cu:30-81 for one grain; pairs counted from the lower index.
 template <typename T>
 T grain_energy(const wd_config& c, const Phys& ph, const Bins& b, const float* st, const int* mat,
-               int i) {
+               int i, const HList* h = nullptr) {
     const float* si = st + 6 * static_cast<size_t>(i);
     const T r = static_cast<T>(ph.r), fmax = static_cast<T>(ph.fmax), k = static_cast<T>(ph.k);
     const T clamp = c.normal_stiffness > 0.f ? static_cast<T>(c.max_force / c.normal_stiffness) : r;
@@ -349,6 +455,21 @@ T grain_energy(const wd_config& c, const Phys& ph, const Bins& b, const float* s
         total += contact_u<T>(r - (p - static_cast<T>(ph.lower[a])), r, fmax, k, clamp);
         total += contact_u<T>(r - (static_cast<T>(ph.upper[a]) - p), r, fmax, k, clamp);
     }
+    const T kt = static_cast<T>(ph.kt);
+    auto spring_u = [&](int partner) {
+        T xi[3];
+        hist_find(h, partner, xi);
+        T s2 = 0;
+        for (int a = 0; a < 3; ++a) s2 += xi[a] * xi[a];
+        total += T(0.5f) * kt * s2;
+    };
+    if (ph.history) {  // stored tangential springs of the wall contacts
+        for (int a = 0; a < 3; ++a) {
+            const T p = static_cast<T>(si[a]);
+            if (p - static_cast<T>(ph.lower[a]) - r < T(0)) spring_u(wall_code(a, 0));
+            if (static_cast<T>(ph.upper[a]) - p - r < T(0)) spring_u(wall_code(a, 1));
+        }
+    }
     for_neighbors(b, c.particles_per_cell, si, i, [&](int j) {
         if (j <= i) return;
         const float* sj = st + 6 * static_cast<size_t>(j);
@@ -357,7 +478,9 @@ T grain_energy(const wd_config& c, const Phys& ph, const Bins& b, const float* s
             const T d = static_cast<T>(si[a]) - static_cast<T>(sj[a]);
             d2 += d * d;
         }
-        total += contact_u<T>(T(2.f) * r - std::sqrt(d2), r, fmax, k, clamp);
+        const T norm = std::sqrt(d2);
+        total += contact_u<T>(T(2.f) * r - norm, r, fmax, k, clamp);
+        if (ph.history && norm - r - r < T(0)) spring_u(j);
     });
     return total;
 }
@@ -382,6 +505,74 @@ void axis_w(T p, T lo, T hi, int size, int& i0, int& i1, T& w0, T& w1) {
     w1 = frac;
 }
 
+// A stateful oracle simulation: original-order FP32 state plus the contact
+// history, advanced by take_step (src/particle_dynamics.cu:264-281).
+struct OrSim {
+    wd_config c;
+    int n = 0;
+    std::vector<float> st;
+    std::vector<int> mat;
+    std::vector<HList> hist;
+    long long over = 0;
+    float amax = 0.f;
+};
+
+void force_pass(OrSim& s, const Phys& ph, const std::vector<HList>& oldh, std::vector<HList>& newh,
+                std::vector<float>& acc, int threads) {
+    Bins b;
+    build_bins(s.c, s.n, s.st.data(), b);
+    s.over += b.over;
+    newh.assign(s.n, HList());
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (int i = 0; i < s.n; ++i) {
+        grain_force<float>(s.c, ph, b, s.n, s.st.data(), s.mat.data(), i, &acc[3 * static_cast<size_t>(i)],
+                           nullptr, nullptr, ph.history ? &oldh[i] : nullptr,
+                           ph.history ? &newh[i] : nullptr);
+    }
+    s.amax = 0.f;
+    for (int i = 0; i < s.n; ++i) {
+        float q = 0.f;
+        for (int a = 0; a < 3; ++a) q += acc[3 * i + a] * acc[3 * i + a];
+        const float m = std::sqrt(q);
+        if (m > s.amax) s.amax = m;
+    }
+}
+
+void sim_steps(OrSim& s, int nsteps, int threads) {
+    const Phys ph = phys_of(s.c);
+    const float dt = s.c.dt;
+    std::vector<float> acc(3 * static_cast<size_t>(s.n));
+    std::vector<HList> newh;
+    const bool picard = s.c.time_integrator == WD_BACKWARD_EULER_PICARD;
+    for (int step = 0; step < nsteps; ++step) {
+        if (!picard) {
+            force_pass(s, ph, s.hist, newh, acc, threads);
+            for (int i = 0; i < s.n; ++i) {
+                float* q = s.st.data() + 6 * static_cast<size_t>(i);
+                for (int a = 0; a < 3; ++a) q[3 + a] += dt * acc[3 * i + a];
+                for (int a = 0; a < 3; ++a) q[a] += dt * q[3 + a];
+            }
+            if (ph.history) s.hist.swap(newh);
+        } else {
+            const std::vector<float> base(s.st);
+            if (s.c.picard_iterations <= 0) continue;
+            for (int k = 0; k < s.c.picard_iterations; ++k) {
+                force_pass(s, ph, s.hist, newh, acc, threads);  // springs restart from step n
+                for (int i = 0; i < s.n; ++i) {
+                    float* q = s.st.data() + 6 * static_cast<size_t>(i);
+                    const float* q0 = base.data() + 6 * static_cast<size_t>(i);
+                    for (int a = 0; a < 3; ++a) {
+                        const float vn = q0[3 + a] + dt * acc[3 * i + a];
+                        q[3 + a] = vn;
+                        q[a] = q0[a] + dt * vn;
+                    }
+                }
+            }
+            if (ph.history) s.hist.swap(newh);
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -391,6 +582,7 @@ extern "C" {
 int or_initial_state(const wd_config* c, float* st, int* mat, int cap, int* n_out) {
     const float r = c->particle_radius;
     if (c->init_type == WD_INIT_TWO_PARTICLES) {
+        *n_out = 2;
         if (cap < 2) return -1;
         const float s[12] = {c->init_x0, c->init_y0, c->init_z0, 0, 0, 0,
                              c->init_x0, c->init_y0, c->init_z0 + c->two_particle_separation, 0, 0, 0};
@@ -400,6 +592,7 @@ int or_initial_state(const wd_config* c, float* st, int* mat, int cap, int* n_ou
         return 0;
     }
     if (c->init_type == WD_INIT_SINGLE_PARTICLE) {
+        *n_out = 1;
         if (cap < 1) return -1;
         const float s[6] = {c->init_x0, c->init_y0, c->init_z0, c->init_vx0, c->init_vy0, c->init_vz0};
         std::memcpy(st, s, sizeof(s));
@@ -507,61 +700,78 @@ void or_rhs(const wd_config* c, int n, const float* st, const int* mat, int prec
     }
 }
 
-// nsteps of take_step (src/particle_dynamics.cu:264-281) in FP32 on the
-// original-order state, in place.  Returns the over-capacity cell count
-// summed over every neighbor pass; *max_accel gets max |a| of the last force
-// pass (src/particle_dynamics.cu:328-344).
-long long or_steps(const wd_config* c, int n, float* st, const int* mat, int nsteps, int threads,
-                   float* max_accel) {
-    const Phys ph = phys_of(*c);
-    const float dt = c->dt;
-    long long over = 0;
-    std::vector<float> acc(3 * static_cast<size_t>(n));
-    std::vector<float> base;
-    const bool picard = c->time_integrator == WD_BACKWARD_EULER_PICARD;
-    float amax = 0.f;
-    auto force_pass = [&]() {
-        Bins b;
-        build_bins(*c, n, st, b);
-        over += b.over;
-#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
-        for (int i = 0; i < n; ++i) {
-            grain_force<float>(*c, ph, b, n, st, mat, i, &acc[3 * static_cast<size_t>(i)], nullptr, nullptr);
-        }
-        amax = 0.f;
-        for (int i = 0; i < n; ++i) {
-            float s = 0.f;
-            for (int a = 0; a < 3; ++a) s += acc[3 * i + a] * acc[3 * i + a];
-            const float m = std::sqrt(s);
-            if (m > amax) amax = m;
-        }
-    };
-    for (int step = 0; step < nsteps; ++step) {
-        if (!picard) {
-            force_pass();
-            for (int i = 0; i < n; ++i) {
-                float* s = st + 6 * static_cast<size_t>(i);
-                for (int a = 0; a < 3; ++a) s[3 + a] += dt * acc[3 * i + a];
-                for (int a = 0; a < 3; ++a) s[a] += dt * s[3 + a];
-            }
-        } else {
-            base.assign(st, st + 6 * static_cast<size_t>(n));
-            for (int k = 0; k < c->picard_iterations; ++k) {
-                force_pass();
-                for (int i = 0; i < n; ++i) {
-                    float* s = st + 6 * static_cast<size_t>(i);
-                    const float* s0 = base.data() + 6 * static_cast<size_t>(i);
-                    for (int a = 0; a < 3; ++a) {
-                        const float vn = s0[3 + a] + dt * acc[3 * i + a];
-                        s[3 + a] = vn;
-                        s[a] = s0[a] + dt * vn;
-                    }
-                }
-            }
+void* or_sim_create(const wd_config* c, int n, const float* st, const int* mat) {
+    OrSim* s = new OrSim();
+    s->c = *c;
+    s->n = n;
+    s->st.assign(st, st + 6 * static_cast<size_t>(n));
+    s->mat.assign(mat, mat + n);
+    s->hist.assign(n, HList());
+    return s;
+}
+
+void or_sim_destroy(void* h) { delete static_cast<OrSim*>(h); }
+
+void or_sim_set_dt(void* h, float dt) { static_cast<OrSim*>(h)->c.dt = dt; }
+
+void or_sim_steps(void* h, int nsteps, int threads) { sim_steps(*static_cast<OrSim*>(h), nsteps, threads); }
+
+void or_sim_state(void* h, float* out) {
+    const OrSim* s = static_cast<OrSim*>(h);
+    std::memcpy(out, s->st.data(), sizeof(float) * s->st.size());
+}
+
+long long or_sim_overflow(void* h) { return static_cast<OrSim*>(h)->over; }
+
+float or_sim_max_accel(void* h) { return static_cast<OrSim*>(h)->amax; }
+
+// Stored springs in original order, kHistSlots per grain (zero-filled).
+void or_sim_history(void* h, int* partner, float* xi, int* count) {
+    const OrSim* s = static_cast<OrSim*>(h);
+    for (int i = 0; i < s->n; ++i) {
+        const HList& l = s->hist[i];
+        count[i] = static_cast<int>(l.size());
+        for (int k = 0; k < kHistSlots; ++k) {
+            const size_t o = static_cast<size_t>(i) * kHistSlots + k;
+            const bool v = k < static_cast<int>(l.size());
+            partner[o] = v ? l[k].partner : 0;
+            for (int a = 0; a < 3; ++a) xi[3 * o + a] = v ? l[k].xi[a] : 0.f;
         }
     }
-    if (max_accel) *max_accel = amax;
-    return over;
+}
+
+// Energy of the current state including the stored tangential springs.
+double or_sim_energy(void* h, int precision) {
+    const OrSim* s = static_cast<OrSim*>(h);
+    Bins b;
+    build_bins(s->c, s->n, s->st.data(), b);
+    const Phys ph = phys_of(s->c);
+    double total = 0.0;
+    for (int i = 0; i < s->n; ++i) {
+        const HList* hl = ph.history ? &s->hist[i] : nullptr;
+        total += precision == 32
+                ? static_cast<double>(grain_energy<float>(s->c, ph, b, s->st.data(), s->mat.data(), i, hl))
+                : grain_energy<double>(s->c, ph, b, s->st.data(), s->mat.data(), i, hl);
+    }
+    return total;
+}
+
+// nsteps of take_step in FP32 on the original-order state, in place, from an
+// empty contact history.  Returns the over-capacity cell count summed over
+// every neighbor pass; *max_accel gets max |a| of the last force pass
+// (src/particle_dynamics.cu:328-344).
+long long or_steps(const wd_config* c, int n, float* st, const int* mat, int nsteps, int threads,
+                   float* max_accel) {
+    OrSim s;
+    s.c = *c;
+    s.n = n;
+    s.st.assign(st, st + 6 * static_cast<size_t>(n));
+    s.mat.assign(mat, mat + n);
+    s.hist.assign(n, HList());
+    sim_steps(s, nsteps, threads);
+    std::memcpy(st, s.st.data(), sizeof(float) * s.st.size());
+    if (max_accel) *max_accel = s.amax;
+    return s.over;
 }
 
 // Total energy (src/energy_kernel.cu:30-81).  precision 32: per-grain FP32

@@ -179,3 +179,47 @@ def test_set_material_and_dt(oracle_lib):
     c2.dt = 5e-4
     want, _, _ = oracle_lib.steps(c2, st, mat, 20, threads=8)
     np.testing.assert_array_equal(got, want)
+
+
+def history_cfg(c):
+    c = c.copy()
+    c.tangential_model = _capi.WD_TANGENTIAL_SPRING_HISTORY
+    c.tangential_stiffness_ratio = 2.0 / 7.0
+    return c
+
+
+@pytest.mark.parametrize("picard", [False, True])
+def test_history_trajectory_and_remap_bit_exact(oracle_lib, picard):
+    c = history_cfg(workloads.column_collapse_4k(
+        integrator=_capi.WD_BACKWARD_EULER_PICARD if picard else _capi.WD_SEMI_IMPLICIT_EULER))
+    st, mat = jittered(c, sigma_x=0.002, sigma_v=0.3, seed=21)
+    steps = 15 if picard else 50
+    osim = oracle_lib.Sim(c, st, mat)
+    with ParticleDynamics(c) as sim:
+        sim.set_state(st)
+        for chunk in (1, steps - 1):
+            sim.take_step(chunk)
+            osim.steps(chunk)
+            np.testing.assert_array_equal(sim.state(), osim.state())
+            gp, gx, gc = sim.history()
+            op, ox, oc = osim.history()
+            np.testing.assert_array_equal(gc, oc)
+            assert gc.sum() > len(st)  # contacts exist
+            for i in range(len(st)):
+                k = gc[i]
+                np.testing.assert_array_equal(gp[i, :k], op[i, :k])
+                np.testing.assert_array_equal(gx[i, :k], ox[i, :k])
+        assert sim.compute_total_energy() == pytest.approx(osim.energy(32), rel=1e-6)
+
+
+def test_sliding_grain_on_gpu_decelerates_at_mu_g():
+    c = history_cfg(workloads.sliding_grain(friction=0.5))
+    with ParticleDynamics(c) as sim:
+        t, vx = [], []
+        for k in range(180):
+            sim.take_step(100)
+            t.append((k + 1) * 100 * c.dt)
+            vx.append(sim.state()[0, 3])
+    t, vx = np.array(t), np.array(vx)
+    sel = vx > 0.1
+    assert np.polyfit(t[sel], vx[sel], 1)[0] == pytest.approx(-0.5 * 9.81, rel=0.01)

@@ -311,3 +311,65 @@ def test_cic_momentum_and_mass_conservation(oracle_lib):
     c, st, mat = cic_cfg(0.13, -0.41, 0.77, v=(2.0, -3.0, 0.5))
     g = oracle_lib.density_grid(c, st, mat).reshape(4, -1).sum(axis=1)
     np.testing.assert_allclose(g, [0.04, 0.08, -0.12, 0.02], rtol=1e-6)
+
+
+# ---------------------------------------------------- tangential history --
+
+def history_cfg(c, ratio=2.0 / 7.0):
+    c = c.copy()
+    c.tangential_model = _capi.WD_TANGENTIAL_SPRING_HISTORY
+    c.tangential_stiffness_ratio = ratio
+    return c
+
+
+def test_history_without_friction_equals_viscous(oracle_lib):
+    # mu = 0: both tangential laws add exact zeros
+    c = workloads.column_collapse_4k(friction=0.0)
+    st, mat = oracle_lib.initial_state(c)
+    a, _, _ = oracle_lib.steps(c, st, mat, 30, threads=8)
+    b, _, _ = oracle_lib.steps(history_cfg(c), st, mat, 30, threads=8)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("history", [False, True])
+def test_sliding_grain_decelerates_at_mu_g(oracle_lib, history):
+    # model_problems/sliding_friction.yaml: a grain sliding on the floor
+    # decelerates at mu g; fit on vx > 0.1 as plot_friction_validation.py does
+    c = workloads.sliding_grain(friction=0.5)
+    if history:
+        c = history_cfg(c)
+    st, mat = oracle_lib.initial_state(c)
+    sim = oracle_lib.Sim(c, st, mat)
+    t, vx = [], []
+    for k in range(180):
+        sim.steps(100)
+        t.append((k + 1) * 100 * c.dt)
+        vx.append(sim.state()[0, 3])
+    t, vx = np.array(t), np.array(vx)
+    sel = vx > 0.1
+    slope = np.polyfit(t[sel], vx[sel], 1)[0]
+    assert slope == pytest.approx(-0.5 * 9.81, rel=0.01)
+
+
+def test_history_spring_sticks_then_slips(oracle_lib):
+    # A grain on the floor pushed sideways by a small constant velocity step:
+    # the stored spring grows by v dt per step while k_t |xi| < mu F_n (stick)
+    c = history_cfg(workloads.sliding_grain(friction=0.5, vx0=1e-4, dt=1e-4))
+    c.init_z0 = workloads.RADIUS - workloads.GRAIN_MASS * 9.81 / workloads.K_N  # static overlap
+    st, mat = oracle_lib.initial_state(c)
+    sim = oracle_lib.Sim(c, st, mat)
+    sim.steps(1)
+    partner, xi, cnt = sim.history()
+    assert cnt[0] == 1 and partner[0, 0] == -5  # the floor
+    # one step of stick: xi = v_t dt (the spring starts empty)
+    assert xi[0, 0, 0] == pytest.approx(1e-4 * 1e-4, rel=1e-5)
+    assert xi[0, 0, 2] == 0.0
+    # the stored spring's energy enters the total energy
+    e_with = sim.energy(64)
+    kt = 2.0 / 7.0 * workloads.K_N
+    s = sim.state()[0]
+    m = workloads.GRAIN_MASS
+    ov = workloads.RADIUS - s[2]
+    e_expected = (0.5 * m * float(np.dot(s[3:], s[3:])) + m * 9.81 * s[2]
+                  + 0.5 * workloads.K_N * ov * ov + 0.5 * kt * float(np.dot(xi[0, 0], xi[0, 0])))
+    assert e_with == pytest.approx(e_expected, rel=1e-6)

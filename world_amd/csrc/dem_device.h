@@ -22,6 +22,7 @@
 namespace wd {
 
 constexpr int kMaxMat = WD_MAX_MATERIALS;
+constexpr int kHist = 24;  // contact-history slots per grain (walls + pairs)
 
 // Device-resident solver constants (one copy per handle, read through a
 // pointer so that dt can change without re-capturing the step graph).
@@ -39,6 +40,8 @@ struct Params {
     float cut2;            // squared centre distance above which a pair cannot touch
     float mu;
     float dt;
+    float kt;              // tangential spring stiffness (history model)
+    int history;           // 1: tangential spring with history, 0: viscous (reference)
     int dgx, dgy, dgz;     // density grid
     float mass[kMaxMat];
     float c_wall[kMaxMat];
@@ -186,6 +189,48 @@ __device__ __forceinline__ void pair_contact(const Params& P, const float vi[3],
         viscous_friction(f, vt, P.mu * fp, c);
     }
 }
+
+// Tangential spring with history (extension; the reference's friction is
+// stateless, src/compute_rhs_kernel.cu:50-65).  The stored displacement xi is
+// rotated into the current tangent plane, advanced by the tangential relative
+// displacement v_t dt, and turned into F_t = -k_t xi - c v_t; beyond the
+// Coulomb limit mu * fp the force is scaled back to the limit and the spring
+// reset to match it (slip).  Returns 1 on slip.  n: unit normal into grain i;
+// vr: relative velocity of i; vn = vr . n.
+__device__ __forceinline__ int spring_friction(const Params& P, float f[3], const float n[3],
+                                               const float vr[3], float vn, float fp, float c,
+                                               float xi[3]) {
+    float vt[3];
+    for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
+    float xn = 0.f;
+    xn += xi[0] * n[0];
+    xn += xi[1] * n[1];
+    xn += xi[2] * n[2];
+    for (int a = 0; a < 3; ++a) xi[a] = xi[a] - xn * n[a];
+    for (int a = 0; a < 3; ++a) xi[a] = xi[a] + vt[a] * P.dt;
+    float ft[3];
+    for (int a = 0; a < 3; ++a) ft[a] = -(P.kt * xi[a]) - c * vt[a];
+    float s = 0.f;
+    s += ft[0] * ft[0];
+    s += ft[1] * ft[1];
+    s += ft[2] * ft[2];
+    const float fm = sqrtf(s);
+    const float lim = P.mu * fp;
+    int slip = 0;
+    if (fm > lim) {
+        const float scale = lim / fm;
+        for (int a = 0; a < 3; ++a) {
+            ft[a] = ft[a] * scale;
+            xi[a] = -(ft[a] + c * vt[a]) / P.kt;
+        }
+        slip = 1;
+    }
+    for (int a = 0; a < 3; ++a) f[a] += ft[a];
+    return slip;
+}
+
+// Partner code of wall (axis a, side 0 lower / 1 upper) in a history list.
+__device__ __forceinline__ int wall_partner(int a, int side) { return -(1 + 2 * a + side); }
 
 // Elastic energy of one contact of overlap delta: quadratic up to the clamp,
 // linear beyond it.

@@ -22,6 +22,8 @@
 
 using namespace wd;
 
+static_assert(WD_HISTORY_SLOTS == kHist, "history slot count differs between ABI and kernels");
+
 namespace {
 
 #define WD_HIP(expr)                                                                        \
@@ -60,6 +62,10 @@ struct wd_sim {
     float4 *Tp = nullptr, *Tv = nullptr, *Sp = nullptr, *Sv = nullptr;
     float4* NP[2] = {nullptr, nullptr};
     float4* NV[2] = {nullptr, nullptr};
+    float4 *HT = nullptr, *HS = nullptr;   // contact history [kHist][n], T and S slot order
+    int *HTc = nullptr, *HSc = nullptr;
+    float4* HN[2] = {nullptr, nullptr};     // Picard base history
+    int* HNc[2] = {nullptr, nullptr};
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
     int* tmp_id = nullptr;
     unsigned long long* status = nullptr;
@@ -102,7 +108,8 @@ struct wd_sim {
         pending.clear();
         for (auto e : spare) (void)hipEventDestroy(e);
         spare.clear();
-        void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], key, slot, count,
+        void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], HT, HS, HTc, HSc,
+                        HN[0], HN[1], HNc[0], HNc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log};
         for (void* b : bufs) {
@@ -110,6 +117,8 @@ struct wd_sim {
         }
         dp = nullptr; ctl = nullptr; Tp = Tv = Sp = Sv = nullptr;
         NP[0] = NP[1] = NV[0] = NV[1] = nullptr;
+        HT = HS = HN[0] = HN[1] = nullptr;
+        HTc = HSc = HNc[0] = HNc[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
         st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
@@ -118,6 +127,7 @@ struct wd_sim {
     }
 
     bool picard() const { return cfg.time_integrator == WD_BACKWARD_EULER_PICARD; }
+    bool history() const { return cfg.tangential_model == WD_TANGENTIAL_SPRING_HISTORY; }
 
     hipEvent_t take_event() {
         if (!spare.empty()) {
@@ -170,9 +180,11 @@ struct wd_sim {
         prof->push_back({name, a, b});
     }
 
-    // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
+    // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder;
+    // the history lists move from (hin, hcin) to (hout, hcout) when enabled.
     void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
-                     float4* nout_v) {
+                     float4* nout_v, const float4* hin = nullptr, const int* hcin = nullptr,
+                     float4* hout = nullptr, int* hcout = nullptr) {
         const int gb = grain_blocks(n);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
         timed("k_count", [&] {
@@ -186,26 +198,36 @@ struct wd_sim {
             hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, Tp,
                                start, tmp_id, n);
         });
+        const Carry carry{nin_p, nin_v, nout_p, nout_v, hin, hcin, hout, hcout};
+        const bool h = history() && hin;
         timed("k_reorder", [&] {
-            if (nmode == 0) {
-                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
-            } else if (nmode == 1) {
-                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
-            } else {
-                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, nin_p, nin_v, nout_p, nout_v, n);
-            }
+#define WD_REORDER(M, H)                                                                         \
+    hipLaunchKernelGGL((k_reorder<M, H>), dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,     \
+                       tmp_id, Tp, Tv, Sp, Sv, carry, n)
+            if (nmode == 0) { if (h) WD_REORDER(0, true); else WD_REORDER(0, false); }
+            else if (nmode == 1) { if (h) WD_REORDER(1, true); else WD_REORDER(1, false); }
+            else { if (h) WD_REORDER(2, true); else WD_REORDER(2, false); }
+#undef WD_REORDER
         });
     }
 
+    // Neighbor pass for an evaluation of the current state (no Picard base).
+    void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr, HT, HTc, HS, HSc); }
+
+    // `hin`/`hcin`: stored springs of the state in S (cell-sorted) order.
     template <int MODE, bool ENERGY>
-    void launch_force(const float4* np, const float4* nv, float* rhs) {
+    void launch_force(const float4* np, const float4* nv, float* rhs, const float4* hin = nullptr,
+                      const int* hcin = nullptr) {
         const int gb = grain_blocks(n);
+        const HistoryIO hio{hin ? hin : HS, hcin ? hcin : HSc, HT, HTc};
         timed("k_force", [&] {
-            hipLaunchKernelGGL((k_force<MODE, ENERGY>), dim3(gb), dim3(kBlock), 0, stream, dp, ctl,
-                               Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, n);
+            if (history()) {
+                hipLaunchKernelGGL((k_force<MODE, ENERGY, true>), dim3(gb), dim3(kBlock), 0, stream,
+                                   dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, hio, n);
+            } else {
+                hipLaunchKernelGGL((k_force<MODE, ENERGY, false>), dim3(gb), dim3(kBlock), 0, stream,
+                                   dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, hio, n);
+            }
         });
         int flags = 0;
         if (MODE != kModeRHS) flags = kFinBox | kFinAccel | (ENERGY ? kFinEnergyLog : 0);
@@ -247,7 +269,7 @@ struct wd_sim {
         if (n == 0) return;
         if (!picard()) {
             hipEvent_t a = stage_begin();
-            enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+            enqueue_bin_current();
             stage_end(WD_STAGE_FIND_NEIGHBORS, a);
             a = stage_begin();
             if (energy_on) launch_force<kModeSIE, true>(nullptr, nullptr, nullptr);
@@ -255,17 +277,17 @@ struct wd_sim {
             stage_end(WD_STAGE_COMPUTE_RHS, a);
             return;
         }
+        // Picard: x^n (and its springs) are carried along every re-sort; each
+        // iteration starts the springs from step n's values.
         for (int k = 0; k < cfg.picard_iterations; ++k) {
+            const int b = k & 1, pb = (k - 1) & 1;
             hipEvent_t a = stage_begin();
-            if (k == 0) {
-                enqueue_bin(1, nullptr, nullptr, NP[0], NV[0]);
-            } else {
-                enqueue_bin(2, NP[(k - 1) & 1], NV[(k - 1) & 1], NP[k & 1], NV[k & 1]);
-            }
+            if (k == 0) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], HT, HTc, HN[0], HNc[0]);
+            else enqueue_bin(2, NP[pb], NV[pb], NP[b], NV[b], HN[pb], HNc[pb], HN[b], HNc[b]);
             stage_end(WD_STAGE_FIND_NEIGHBORS, a);
             a = stage_begin();
-            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[k & 1], NV[k & 1], nullptr);
-            else launch_force<kModeBEP, false>(NP[k & 1], NV[k & 1], nullptr);
+            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[b], NV[b], nullptr, HN[b], HNc[b]);
+            else launch_force<kModeBEP, false>(NP[b], NV[b], nullptr, HN[b], HNc[b]);
             stage_end(WD_STAGE_COMPUTE_RHS, a);
         }
     }
@@ -356,8 +378,11 @@ int validate(const wd_config& c) {
     if (c.particle_material < 0 || c.particle_material >= c.num_masses) {
         return fail(WD_ERR_ARG, "initialization.particle_material has no entry in materials.masses");
     }
-    if (c.tangential_model != WD_TANGENTIAL_VISCOUS) {
-        return fail(WD_ERR_ARG, "tangential_model spring_history is not available in this build");
+    if (c.tangential_model != WD_TANGENTIAL_VISCOUS && c.tangential_model != WD_TANGENTIAL_SPRING_HISTORY) {
+        return fail(WD_ERR_ARG, "unknown tangential_model");
+    }
+    if (c.tangential_model == WD_TANGENTIAL_SPRING_HISTORY && !(c.tangential_stiffness_ratio > 0.f)) {
+        return fail(WD_ERR_ARG, "tangential_stiffness_ratio must be positive for spring_history");
     }
     return WD_OK;
 }
@@ -393,6 +418,8 @@ void fill_params(const wd_config& c, int n, Params& P) {
     P.cut2 = static_cast<float>(4.0 * c.particle_radius * static_cast<double>(c.particle_radius) * (1.0 + 1e-5));
     P.mu = c.friction;
     P.dt = c.dt;
+    P.history = c.tangential_model == WD_TANGENTIAL_SPRING_HISTORY ? 1 : 0;
+    P.kt = c.tangential_stiffness_ratio * k;
     P.dgx = c.density_grid_size_x;
     P.dgy = c.density_grid_size_y;
     P.dgz = c.density_grid_size_z;
@@ -425,6 +452,14 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->Sp, n)); WD_TRY(dalloc(&s->Sv, n));
     if (s->picard()) {
         for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->NP[b], n)); WD_TRY(dalloc(&s->NV[b], n)); }
+    }
+    if (s->history()) {
+        const size_t hs = static_cast<size_t>(kHist) * n;
+        WD_TRY(dalloc(&s->HT, hs)); WD_TRY(dalloc(&s->HS, hs));
+        WD_TRY(dalloc(&s->HTc, n)); WD_TRY(dalloc(&s->HSc, n));
+        if (s->picard()) {
+            for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->HN[b], hs)); WD_TRY(dalloc(&s->HNc[b], n)); }
+        }
     }
     WD_TRY(dalloc(&s->key, n)); WD_TRY(dalloc(&s->slot, n)); WD_TRY(dalloc(&s->tmp_id, n));
     const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
@@ -459,7 +494,7 @@ int ingest(wd_sim* s, const float* st, const int* mat) {
     if (mat) WD_HIP(hipMemcpy(s->mat_d, mat, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
     const int gb = grain_blocks(s->n);
     hipLaunchKernelGGL(k_ingest, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->parts, s->st6,
-                       s->mat_d, s->Tp, s->Tv, s->key, s->n);
+                       s->mat_d, s->Tp, s->Tv, s->key, s->HTc, s->n);
     s->finalize(gb, kFinBox);
     WD_HIP(hipGetLastError());
     return s->sync();
@@ -638,7 +673,7 @@ extern "C" int wd_total_energy(wd_sim* s, float* out) {
         return WD_OK;
     }
     hipEvent_t a = s->stage_begin();
-    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->enqueue_bin_current();
     s->stage_end(WD_STAGE_FIND_NEIGHBORS, a);
     s->launch_force<kModeRHS, true>(nullptr, nullptr, nullptr);
     WD_HIP(hipGetLastError());
@@ -743,7 +778,7 @@ extern "C" int wd_cell_order(wd_sim* s, int* ids_out, uint32_t* keys_out) {
     WD_CHECK_SIM(s);
     if (!ids_out || !keys_out) return fail(WD_ERR_ARG, "wd_cell_order: null output");
     if (s->n == 0) return WD_OK;
-    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->enqueue_bin_current();
     int* ids = nullptr;
     uint32_t* keys = nullptr;
     int rc = dalloc(&ids, s->n);
@@ -768,7 +803,7 @@ extern "C" int wd_neighbor_rows(wd_sim* s, int* out, size_t capacity) {
     const size_t need = static_cast<size_t>(s->n) * 27 * s->cfg.particles_per_cell;
     if (capacity < need) return fail(WD_ERR_ARG, "wd_neighbor_rows: output too small");
     if (s->n == 0) return WD_OK;
-    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->enqueue_bin_current();
     int* rows = nullptr;
     int rc = dalloc(&rows, need);
     if (rc == WD_OK) {
@@ -791,7 +826,7 @@ extern "C" int wd_rhs(wd_sim* s, float* out) {
         int rc = dalloc(&s->rhs_buf, static_cast<size_t>(6) * s->n);
         if (rc != WD_OK) return rc;
     }
-    s->enqueue_bin(0, nullptr, nullptr, nullptr, nullptr);
+    s->enqueue_bin_current();
     s->launch_force<kModeRHS, false>(nullptr, nullptr, s->rhs_buf);
     WD_HIP(hipGetLastError());
     int rc = s->sync();
@@ -875,4 +910,32 @@ extern "C" int wd_profile_step(wd_sim* s, int nsteps, float* kernel_ms, int capa
         names[names_capacity - 1] = '\0';
     }
     return WD_OK;
+}
+
+extern "C" int wd_get_history(wd_sim* s, int* partner, float* xi, int* count) {
+    WD_CHECK_SIM(s);
+    if (!partner || !xi || !count) return fail(WD_ERR_ARG, "wd_get_history: null output");
+    if (!s->history()) return fail(WD_ERR_STATE, "wd_get_history: tangential_model is not spring_history");
+    if (s->n == 0) return WD_OK;
+    int* dp_ = nullptr;
+    float* dx = nullptr;
+    int* dc = nullptr;
+    const size_t slots = static_cast<size_t>(s->n) * kHist;
+    int rc = dalloc(&dp_, slots);
+    if (rc == WD_OK) rc = dalloc(&dx, 3 * slots);
+    if (rc == WD_OK) rc = dalloc(&dc, s->n);
+    if (rc == WD_OK) {
+        hipLaunchKernelGGL(k_export_history, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           s->stream, s->Tp, s->HT, s->HTc, dp_, dx, dc, s->n);
+        rc = s->sync();
+    }
+    if (rc == WD_OK && (hipMemcpy(partner, dp_, sizeof(int) * slots, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(xi, dx, sizeof(float) * 3 * slots, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(count, dc, sizeof(int) * s->n, hipMemcpyDeviceToHost) != hipSuccess)) {
+        rc = fail(WD_ERR_HIP, "wd_get_history: copy failed");
+    }
+    if (dp_) (void)hipFree(dp_);
+    if (dx) (void)hipFree(dx);
+    if (dc) (void)hipFree(dc);
+    return rc;
 }

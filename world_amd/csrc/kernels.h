@@ -64,4 +64,24 @@ __device__ __forceinline__ int logical_block() {
 
 enum ForceMode { kModeSIE = 0, kModeBEP = 1, kModeRHS = 2 };
 
+// Contact-history buffers of one force pass (slot-major [kHist][n]).
+struct HistoryIO {
+    const float4* in;   // springs of the evaluated state, cell-sorted slot order
+    const int* cnt_in;
+    float4* out;        // springs after the step, same slot order (unused for kModeRHS)
+    int* cnt_out;
+};
+
+// What k_reorder carries along the permutation besides the grain.
+struct Carry {
+    const float4* nin_p;  // Picard base state (NMODE 2)
+    const float4* nin_v;
+    float4* nout_p;       // Picard base state out (NMODE 1, 2)
+    float4* nout_v;
+    const float4* hin;    // contact history in (HIST)
+    const int* hcnt_in;
+    float4* hout;         // contact history out (HIST)
+    int* hcnt_out;
+};
+
 }  // namespace wd

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
                                                    const float* __restrict__ st6,
                                                    const int* __restrict__ mat,
                                                    float4* __restrict__ Tp, float4* __restrict__ Tv,
-                                                   uint32_t* __restrict__ key, int n) {
+                                                   uint32_t* __restrict__ key,
+                                                   int* __restrict__ hist_cnt, int n) {
     const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
         const float x = s[0], y = s[1], z = s[2];
         Tp[i] = make_float4(x, y, z, __int_as_float(i));
         Tv[i] = make_float4(s[3], s[4], s[5], __int_as_float(mat[i]));
+        if (hist_cnt) hist_cnt[i] = 0;  // an ingested state starts with no contact history
         int cx, cy, cz;
         grain_cell(P, x, y, z, cx, cy, cz);
         key[i] = pack_cell(cx, cy, cz);
@@ -299,14 +301,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Ctl* __restrict__ ctl,
 
 // Pass 4: final position = cell start + rank of the original id among the
 // cell's ids.  NMODE 1 also copies the grain into the Picard base buffers,
-// NMODE 2 carries the Picard base along the same permutation.
-template <int NMODE>
+// NMODE 2 carries the Picard base along the same permutation; HIST carries
+// the contact-history lists (count and entries) along with the grain.
+template <int NMODE, bool HIST>
 __global__ __launch_bounds__(kBlock) void k_reorder(
         const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
         const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
-        float4* __restrict__ Sv, const float4* __restrict__ NinP, const float4* __restrict__ NinV,
-        float4* __restrict__ NoutP, float4* __restrict__ NoutV, int n) {
+        float4* __restrict__ Sv, Carry carry, int n) {
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const SubGrid g = make_subgrid(ctl->bbox_cur);
@@ -325,29 +327,38 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     Sp[dst] = p;
     Sv[dst] = v;
     if (NMODE == 1) {
-        NoutP[dst] = p;
-        NoutV[dst] = v;
+        carry.nout_p[dst] = p;
+        carry.nout_v[dst] = v;
     } else if (NMODE == 2) {
-        NoutP[dst] = NinP[i];
-        NoutV[dst] = NinV[i];
+        carry.nout_p[dst] = carry.nin_p[i];
+        carry.nout_v[dst] = carry.nin_v[i];
+    }
+    if (HIST) {
+        const int h = carry.hcnt_in[i];
+        carry.hcnt_out[dst] = h;
+        for (int k = 0; k < h; ++k) {
+            carry.hout[static_cast<size_t>(k) * n + dst] = carry.hin[static_cast<size_t>(k) * n + i];
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Pass 5: forces at the cell-sorted state S, fused with the integrator.
 // MODE kModeSIE / kModeBEP write the next state to T (same slot), its cell
-// key and the bounding-box / max |a| reductions; kModeRHS only writes
+// key and the per-block box / max |a|; kModeRHS only writes
 // rhs = [v, F/m] in original order.  ENERGY adds the per-grain energy of S
-// (pair terms counted once, from the lower original index) to a
-// deterministic grid-wide sum: appended to the energy log for the
-// integrating modes, stored in ctl->energy_last for kModeRHS.
-template <int MODE, bool ENERGY>
+// (pair terms counted once, from the lower original index) to the block
+// partials.  HIST selects the tangential spring with history: the stored
+// springs of S are read from hist.in (slot-major [kHist][n]) and the
+// updated springs of this step are written to hist.out in contact order
+// (walls, then pairs in stencil order).
+template <int MODE, bool ENERGY, bool HIST>
 __global__ __launch_bounds__(kBlock) void k_force(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
-        float* __restrict__ rhs_out, BlockPart* __restrict__ parts, int n) {
+        float* __restrict__ rhs_out, BlockPart* __restrict__ parts, HistoryIO hist, int n) {
     const Params& P = *Pp;
     __shared__ uint32_t s_list[kContactList][kBlock];
     const int lb = logical_block();
@@ -364,8 +375,72 @@ __global__ __launch_bounds__(kBlock) void k_force(
         const int mat = __float_as_int(vi4.w);
         const float m = P.mass[mat];
         float f[3] = {0.f, 0.f, 0.f};
-        wall_contacts(P, x, v, m, P.c_wall[mat], f);
         if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
+
+        // Contact history: old springs matched by partner with a cursor (the
+        // old list was written in the same contact order), new springs
+        // appended in contact order.
+        const int h_old = HIST ? hist.cnt_in[p] : 0;
+        int h_cur = 0, h_new = 0;
+        auto h_take = [&](int partner, float xi[3]) {
+            xi[0] = xi[1] = xi[2] = 0.f;
+            for (int t = 0; t < h_old; ++t) {
+                int k = h_cur + t;
+                if (k >= h_old) k -= h_old;
+                const float4 h = hist.in[static_cast<size_t>(k) * n + p];
+                if (__float_as_int(h.w) == partner) {
+                    xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
+                    h_cur = k + 1;
+                    return;
+                }
+            }
+        };
+        auto h_put = [&](int partner, const float xi[3]) {
+            if (MODE != kModeRHS && h_new < kHist) {
+                hist.out[static_cast<size_t>(h_new) * n + p] =
+                        make_float4(xi[0], xi[1], xi[2], __int_as_float(partner));
+            }
+            ++h_new;
+        };
+
+        // Gravity and the six walls, lower then upper per axis
+        // (src/compute_rhs_kernel.cu:105-141).
+        const float c_wall = P.c_wall[mat];
+        f[2] -= m * P.gravity;
+        for (int a = 0; a < 3; ++a) {
+            for (int side = 0; side < 2; ++side) {
+                const float dist = side == 0 ? x[a] - P.wlo[a] - P.radius : P.whi[a] - x[a] - P.radius;
+                const float fw = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
+                if (side == 0) f[a] += fw;
+                else f[a] -= fw;
+                if (dist < 0.f) {
+                    f[a] -= c_wall * v[a];
+                    if (HIST) {
+                        float nrm[3] = {0.f, 0.f, 0.f};
+                        nrm[a] = side == 0 ? 1.f : -1.f;
+                        float vn = 0.f;
+                        vn += v[0] * nrm[0];
+                        vn += v[1] * nrm[1];
+                        vn += v[2] * nrm[2];
+                        float xi[3];
+                        h_take(wall_partner(a, side), xi);
+                        if (ENERGY) {
+                            float s2 = 0.f;
+                            s2 += xi[0] * xi[0];
+                            s2 += xi[1] * xi[1];
+                            s2 += xi[2] * xi[2];
+                            e_grain += 0.5f * P.kt * s2;
+                        }
+                        spring_friction(P, f, nrm, v, vn, fw, c_wall, xi);
+                        h_put(wall_partner(a, side), xi);
+                    } else {
+                        float vt[3];
+                        for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
+                        viscous_friction(f, vt, P.mu * fw, c_wall);
+                    }
+                }
+            }
+        }
 
         const SubGrid g = make_subgrid(ctl->bbox_cur);
         int cx, cy, cz;
@@ -388,9 +463,38 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 const float4 pj = Sp[q];
                 float d0, d1, d2;
                 const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
-                pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
-                if (ENERGY && __float_as_int(pj.w) > id) {
-                    e_grain += contact_energy(P, 2.f * P.radius - norm);
+                const int jd = __float_as_int(pj.w);
+                if (ENERGY && jd > id) e_grain += contact_energy(P, 2.f * P.radius - norm);
+                if (!HIST) {
+                    pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
+                    continue;
+                }
+                const float dist = norm - P.radius - P.radius;
+                const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
+                const float fp = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
+                f[0] += fp * nrm[0];
+                f[1] += fp * nrm[1];
+                f[2] += fp * nrm[2];
+                if (dist < 0.f) {
+                    const float4 vj = Sv[q];
+                    const float c = P.c_pair[mat * kMaxMat + __float_as_int(vj.w)];
+                    const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
+                    float vn = 0.f;
+                    vn += vr[0] * nrm[0];
+                    vn += vr[1] * nrm[1];
+                    vn += vr[2] * nrm[2];
+                    for (int a = 0; a < 3; ++a) f[a] -= c * vn * nrm[a];
+                    float xi[3];
+                    h_take(jd, xi);
+                    if (ENERGY && jd > id) {
+                        float s2 = 0.f;
+                        s2 += xi[0] * xi[0];
+                        s2 += xi[1] * xi[1];
+                        s2 += xi[2] * xi[2];
+                        e_grain += 0.5f * P.kt * s2;
+                    }
+                    spring_friction(P, f, nrm, vr, vn, fp, c, xi);
+                    h_put(jd, xi);
                 }
             }
             listed = 0;
@@ -430,6 +534,8 @@ __global__ __launch_bounds__(kBlock) void k_force(
             }
         }
         flush();
+        if (HIST && MODE != kModeRHS) hist.cnt_out[p] = min(h_new, kHist);
+
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
         if (MODE == kModeRHS) {
             if (rhs_out) {
@@ -516,27 +622,31 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* 
     }
 }
 
-template __global__ void k_reorder<0>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
-                                      const float4*, const float4*, float4*, float4*,
-                                      const float4*, const float4*, float4*, float4*, int);
-template __global__ void k_reorder<1>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
-                                      const float4*, const float4*, float4*, float4*,
-                                      const float4*, const float4*, float4*, float4*, int);
-template __global__ void k_reorder<2>(const Ctl*, const uint32_t*, const uint32_t*, const int*,
-                                      const float4*, const float4*, float4*, float4*,
-                                      const float4*, const float4*, float4*, float4*, int);
+#define WD_INST_REORDER(M, H)                                                                \
+    template __global__ void k_reorder<M, H>(const Ctl*, const uint32_t*, const uint32_t*,       \
+                                             const int*, const float4*, const float4*, float4*,  \
+                                             float4*, Carry, int);
+WD_INST_REORDER(0, false)
+WD_INST_REORDER(1, false)
+WD_INST_REORDER(2, false)
+WD_INST_REORDER(0, true)
+WD_INST_REORDER(1, true)
+WD_INST_REORDER(2, true)
+#undef WD_INST_REORDER
 
-#define WD_INST_FORCE(M, E)                                                                    \
-    template __global__ void k_force<M, E>(const Params*, Ctl*, const float4*, const float4*,  \
-                                           const float4*, const float4*, const uint32_t*,      \
-                                           float4*, float4*, uint32_t*, float*, BlockPart*,    \
-                                           int);
-WD_INST_FORCE(kModeSIE, false)
-WD_INST_FORCE(kModeSIE, true)
-WD_INST_FORCE(kModeBEP, false)
-WD_INST_FORCE(kModeBEP, true)
-WD_INST_FORCE(kModeRHS, false)
-WD_INST_FORCE(kModeRHS, true)
+#define WD_INST_FORCE(M, E, H)                                                                  \
+    template __global__ void k_force<M, E, H>(const Params*, Ctl*, const float4*, const float4*,    \
+                                              const float4*, const float4*, const uint32_t*,        \
+                                              float4*, float4*, uint32_t*, float*, BlockPart*,      \
+                                              HistoryIO, int);
+#define WD_INST_FORCE_H(M, E) WD_INST_FORCE(M, E, false) WD_INST_FORCE(M, E, true)
+WD_INST_FORCE_H(kModeSIE, false)
+WD_INST_FORCE_H(kModeSIE, true)
+WD_INST_FORCE_H(kModeBEP, false)
+WD_INST_FORCE_H(kModeBEP, true)
+WD_INST_FORCE_H(kModeRHS, false)
+WD_INST_FORCE_H(kModeRHS, true)
+#undef WD_INST_FORCE_H
 #undef WD_INST_FORCE
 
 // ---------------------------------------------------------------------------
@@ -589,6 +699,32 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
         }
     }
     row[w] = -1;
+}
+
+// Contact-history lists of the working layout, in original grain order:
+// partner[id * kHist + k], xi[(id * kHist + k) * 3 + a], count[id].
+__global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restrict__ Tp,
+                                                           const float4* __restrict__ hist,
+                                                           const int* __restrict__ cnt,
+                                                           int* __restrict__ partner,
+                                                           float* __restrict__ xi,
+                                                           int* __restrict__ count, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int id = __float_as_int(Tp[i].w);
+    const int h = cnt[i];
+    count[id] = h;
+    for (int k = 0; k < kHist; ++k) {
+        const size_t o = static_cast<size_t>(id) * kHist + k;
+        if (k < h) {
+            const float4 e = hist[static_cast<size_t>(k) * n + i];
+            partner[o] = __float_as_int(e.w);
+            xi[3 * o] = e.x; xi[3 * o + 1] = e.y; xi[3 * o + 2] = e.z;
+        } else {
+            partner[o] = 0;
+            xi[3 * o] = xi[3 * o + 1] = xi[3 * o + 2] = 0.f;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -218,6 +218,17 @@ class ParticleDynamics:
         check(lib().wd_neighbor_rows(self._h, ptr(out), out.size))
         return out[: self.n * row].reshape(self.n, row)
 
+    def history(self):
+        """Stored contact springs (spring_history model), original order:
+        partner ids [n, S], displacements [n, S, 3], counts [n]."""
+        S = _capi.WD_HISTORY_SLOTS
+        partner = np.zeros(max(self.n, 1) * S, np.int32)
+        xi = np.zeros(max(self.n, 1) * S * 3, np.float32)
+        cnt = np.zeros(max(self.n, 1), np.int32)
+        check(lib().wd_get_history(self._h, ptr(partner), ptr(xi), ptr(cnt)))
+        n = self.n
+        return partner[: n * S].reshape(n, S), xi[: n * S * 3].reshape(n, S, 3), cnt[:n]
+
     def rhs(self) -> np.ndarray:
         out = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
         check(lib().wd_rhs(self._h, ptr(out)))
