@@ -23,9 +23,16 @@ def jittered(c, sigma_x=0.0015, sigma_v=0.2, seed=3):
     return st, mat
 
 
-@pytest.fixture(scope="module")
-def col4k():
-    return workloads.column_collapse_4k()
+def viscous_cfg(c):
+    c = c.copy()
+    c.tangential_model = _capi.WD_TANGENTIAL_VISCOUS
+    return c
+
+
+@pytest.fixture(scope="module", params=["viscous", "history"])
+def col4k(request):
+    c = workloads.column_collapse_4k()
+    return viscous_cfg(c) if request.param == "viscous" else c
 
 
 def test_initial_state_matches_fixture(col4k):
@@ -73,8 +80,8 @@ def test_rhs_bit_exact_and_within_fp64(oracle_lib, col4k):
 
 @pytest.mark.parametrize("picard", [False, True])
 def test_trajectory_bit_exact(oracle_lib, picard):
-    c = workloads.column_collapse_4k(
-        integrator=_capi.WD_BACKWARD_EULER_PICARD if picard else _capi.WD_SEMI_IMPLICIT_EULER)
+    c = viscous_cfg(workloads.column_collapse_4k(
+        integrator=_capi.WD_BACKWARD_EULER_PICARD if picard else _capi.WD_SEMI_IMPLICIT_EULER))
     steps = 20 if picard else 50
     st0, mat = _capi.initial_state(c)
     with ParticleDynamics(c) as sim:
@@ -110,19 +117,23 @@ def test_graph_replay_equals_eager_launches(col4k):
 
 def test_energy_standalone_and_fused(oracle_lib, col4k):
     st, mat = jittered(col4k, sigma_x=0.002, sigma_v=0.3, seed=5)
+    osim = oracle_lib.Sim(col4k, st, mat)
     with ParticleDynamics(col4k) as sim:
         sim.set_state(st)
         e0 = sim.compute_total_energy()
-        assert e0 == pytest.approx(oracle_lib.energy(col4k, st, mat, 32), rel=1e-6)
-        assert e0 == pytest.approx(oracle_lib.energy(col4k, st, mat, 64), rel=1e-5)
+        assert e0 == pytest.approx(osim.energy(32), rel=1e-6)
+        assert e0 == pytest.approx(osim.energy(64), rel=1e-5)
         sim.enable_energy_log(8)
         sim.take_step(3)
         log = sim.energy_log()
         assert len(log) == 3
-        assert log[0] == pytest.approx(oracle_lib.energy(col4k, st, mat, 32), rel=1e-12)
-        s3 = sim.state()
+        assert log[0] == pytest.approx(osim.energy(32), rel=1e-12)
+        osim.steps(3)
+        np.testing.assert_array_equal(sim.state(), osim.state())
         sim.take_step(1)
-        assert sim.energy_log()[3] == pytest.approx(oracle_lib.energy(col4k, s3, mat, 32), rel=1e-12)
+        # the fused value is the energy of the state the step starts from,
+        # stored tangential springs included
+        assert sim.energy_log()[3] == pytest.approx(osim.energy(32), rel=1e-12)
 
 
 def test_density_grid_matches_oracle(oracle_lib, col4k):

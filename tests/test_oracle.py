@@ -322,9 +322,15 @@ def history_cfg(c, ratio=2.0 / 7.0):
     return c
 
 
+def viscous_cfg(c):
+    c = c.copy()
+    c.tangential_model = _capi.WD_TANGENTIAL_VISCOUS
+    return c
+
+
 def test_history_without_friction_equals_viscous(oracle_lib):
     # mu = 0: both tangential laws add exact zeros
-    c = workloads.column_collapse_4k(friction=0.0)
+    c = viscous_cfg(workloads.column_collapse_4k(friction=0.0))
     st, mat = oracle_lib.initial_state(c)
     a, _, _ = oracle_lib.steps(c, st, mat, 30, threads=8)
     b, _, _ = oracle_lib.steps(history_cfg(c), st, mat, 30, threads=8)
@@ -336,8 +342,7 @@ def test_sliding_grain_decelerates_at_mu_g(oracle_lib, history):
     # model_problems/sliding_friction.yaml: a grain sliding on the floor
     # decelerates at mu g; fit on vx > 0.1 as plot_friction_validation.py does
     c = workloads.sliding_grain(friction=0.5)
-    if history:
-        c = history_cfg(c)
+    c = history_cfg(c) if history else viscous_cfg(c)
     st, mat = oracle_lib.initial_state(c)
     sim = oracle_lib.Sim(c, st, mat)
     t, vx = [], []

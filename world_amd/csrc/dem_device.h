@@ -124,8 +124,10 @@ __device__ __forceinline__ void viscous_friction(float f[3], const float vt[3], 
 __device__ __forceinline__ void wall_contacts(const Params& P, const float x[3], const float v[3],
                                               float m_i, float c_wall, float f[3]) {
     f[2] -= m_i * P.gravity;
+    #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float vt[3];
+        #pragma unroll
         for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
         const float dl = x[a] - P.wlo[a] - P.radius;
         const float fl = fmaxf(0.f, fminf(P.max_force, P.kneg * dl));
@@ -201,14 +203,18 @@ __device__ __forceinline__ int spring_friction(const Params& P, float f[3], cons
                                                const float vr[3], float vn, float fp, float c,
                                                float xi[3]) {
     float vt[3];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
     float xn = 0.f;
     xn += xi[0] * n[0];
     xn += xi[1] * n[1];
     xn += xi[2] * n[2];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) xi[a] = xi[a] - xn * n[a];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) xi[a] = xi[a] + vt[a] * P.dt;
     float ft[3];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) ft[a] = -(P.kt * xi[a]) - c * vt[a];
     float s = 0.f;
     s += ft[0] * ft[0];
@@ -219,12 +225,14 @@ __device__ __forceinline__ int spring_friction(const Params& P, float f[3], cons
     int slip = 0;
     if (fm > lim) {
         const float scale = lim / fm;
+        #pragma unroll
         for (int a = 0; a < 3; ++a) {
             ft[a] = ft[a] * scale;
             xi[a] = -(ft[a] + c * vt[a]) / P.kt;
         }
         slip = 1;
     }
+    #pragma unroll
     for (int a = 0; a < 3; ++a) f[a] += ft[a];
     return slip;
 }
@@ -251,6 +259,7 @@ __device__ __forceinline__ float grain_energy_self(const Params& P, const float 
     s += v[2] * v[2];
     total += 0.5f * m * s;
     total += m * P.gravity * (x[2] - P.wlo[2]);
+    #pragma unroll
     for (int a = 0; a < 3; ++a) {
         total += contact_energy(P, P.radius - (x[a] - P.wlo[a]));
         total += contact_energy(P, P.radius - (P.whi[a] - x[a]));

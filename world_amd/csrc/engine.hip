@@ -62,10 +62,8 @@ struct wd_sim {
     float4 *Tp = nullptr, *Tv = nullptr, *Sp = nullptr, *Sv = nullptr;
     float4* NP[2] = {nullptr, nullptr};
     float4* NV[2] = {nullptr, nullptr};
-    float4 *HT = nullptr, *HS = nullptr;   // contact history [kHist][n], T and S slot order
-    int *HTc = nullptr, *HSc = nullptr;
-    float4* HN[2] = {nullptr, nullptr};     // Picard base history
-    int* HNc[2] = {nullptr, nullptr};
+    float4* H[2] = {nullptr, nullptr};  // contact-history generations [kHist][n], by original id
+    int* Hc[2] = {nullptr, nullptr};
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
     int* tmp_id = nullptr;
     unsigned long long* status = nullptr;
@@ -108,8 +106,8 @@ struct wd_sim {
         pending.clear();
         for (auto e : spare) (void)hipEventDestroy(e);
         spare.clear();
-        void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], HT, HS, HTc, HSc,
-                        HN[0], HN[1], HNc[0], HNc[1], key, slot, count,
+        void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], H[0], H[1], Hc[0],
+                        Hc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log};
         for (void* b : bufs) {
@@ -117,8 +115,8 @@ struct wd_sim {
         }
         dp = nullptr; ctl = nullptr; Tp = Tv = Sp = Sv = nullptr;
         NP[0] = NP[1] = NV[0] = NV[1] = nullptr;
-        HT = HS = HN[0] = HN[1] = nullptr;
-        HTc = HSc = HNc[0] = HNc[1] = nullptr;
+        H[0] = H[1] = nullptr;
+        Hc[0] = Hc[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
         st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
@@ -180,11 +178,9 @@ struct wd_sim {
         prof->push_back({name, a, b});
     }
 
-    // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder;
-    // the history lists move from (hin, hcin) to (hout, hcout) when enabled.
+    // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
     void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
-                     float4* nout_v, const float4* hin = nullptr, const int* hcin = nullptr,
-                     float4* hout = nullptr, int* hcout = nullptr) {
+                     float4* nout_v) {
         const int gb = grain_blocks(n);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
         timed("k_count", [&] {
@@ -198,28 +194,28 @@ struct wd_sim {
             hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, Tp,
                                start, tmp_id, n);
         });
-        const Carry carry{nin_p, nin_v, nout_p, nout_v, hin, hcin, hout, hcout};
-        const bool h = history() && hin;
+        const Carry carry{nin_p, nin_v, nout_p, nout_v};
         timed("k_reorder", [&] {
-#define WD_REORDER(M, H)                                                                         \
-    hipLaunchKernelGGL((k_reorder<M, H>), dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,     \
-                       tmp_id, Tp, Tv, Sp, Sv, carry, n)
-            if (nmode == 0) { if (h) WD_REORDER(0, true); else WD_REORDER(0, false); }
-            else if (nmode == 1) { if (h) WD_REORDER(1, true); else WD_REORDER(1, false); }
-            else { if (h) WD_REORDER(2, true); else WD_REORDER(2, false); }
-#undef WD_REORDER
+            if (nmode == 0) {
+                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
+            } else if (nmode == 1) {
+                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
+            } else {
+                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
+            }
         });
     }
 
-    // Neighbor pass for an evaluation of the current state (no Picard base).
-    void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr, HT, HTc, HS, HSc); }
+    void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr); }
 
-    // `hin`/`hcin`: stored springs of the state in S (cell-sorted) order.
+    // last_pass: the pass completes a step (its springs become current).
     template <int MODE, bool ENERGY>
-    void launch_force(const float4* np, const float4* nv, float* rhs, const float4* hin = nullptr,
-                      const int* hcin = nullptr) {
+    void launch_force(const float4* np, const float4* nv, float* rhs, bool last_pass = true) {
         const int gb = grain_blocks(n);
-        const HistoryIO hio{hin ? hin : HS, hcin ? hcin : HSc, HT, HTc};
+        const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}};
         timed("k_force", [&] {
             if (history()) {
                 hipLaunchKernelGGL((k_force<MODE, ENERGY, true>), dim3(gb), dim3(kBlock), 0, stream,
@@ -230,8 +226,12 @@ struct wd_sim {
             }
         });
         int flags = 0;
-        if (MODE != kModeRHS) flags = kFinBox | kFinAccel | (ENERGY ? kFinEnergyLog : 0);
-        else if (ENERGY) flags = kFinEnergyLast;
+        if (MODE != kModeRHS) {
+            flags = kFinBox | kFinAccel | (ENERGY ? kFinEnergyLog : 0);
+            if (history() && last_pass) flags |= kFinFlipHistory;
+        } else if (ENERGY) {
+            flags = kFinEnergyLast;
+        }
         if (flags) finalize(gb, flags);
     }
 
@@ -277,17 +277,19 @@ struct wd_sim {
             stage_end(WD_STAGE_COMPUTE_RHS, a);
             return;
         }
-        // Picard: x^n (and its springs) are carried along every re-sort; each
-        // iteration starts the springs from step n's values.
+        // Picard: x^n is carried along every re-sort; every iteration starts
+        // the springs from step n's generation, and only the last iteration's
+        // springs become current.
         for (int k = 0; k < cfg.picard_iterations; ++k) {
             const int b = k & 1, pb = (k - 1) & 1;
+            const bool last = k + 1 == cfg.picard_iterations;
             hipEvent_t a = stage_begin();
-            if (k == 0) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], HT, HTc, HN[0], HNc[0]);
-            else enqueue_bin(2, NP[pb], NV[pb], NP[b], NV[b], HN[pb], HNc[pb], HN[b], HNc[b]);
+            if (k == 0) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0]);
+            else enqueue_bin(2, NP[pb], NV[pb], NP[b], NV[b]);
             stage_end(WD_STAGE_FIND_NEIGHBORS, a);
             a = stage_begin();
-            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[b], NV[b], nullptr, HN[b], HNc[b]);
-            else launch_force<kModeBEP, false>(NP[b], NV[b], nullptr, HN[b], HNc[b]);
+            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[b], NV[b], nullptr, last);
+            else launch_force<kModeBEP, false>(NP[b], NV[b], nullptr, last);
             stage_end(WD_STAGE_COMPUTE_RHS, a);
         }
     }
@@ -455,11 +457,7 @@ int allocate(wd_sim* s) {
     }
     if (s->history()) {
         const size_t hs = static_cast<size_t>(kHist) * n;
-        WD_TRY(dalloc(&s->HT, hs)); WD_TRY(dalloc(&s->HS, hs));
-        WD_TRY(dalloc(&s->HTc, n)); WD_TRY(dalloc(&s->HSc, n));
-        if (s->picard()) {
-            for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->HN[b], hs)); WD_TRY(dalloc(&s->HNc[b], n)); }
-        }
+        for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->H[b], hs)); WD_TRY(dalloc(&s->Hc[b], n)); }
     }
     WD_TRY(dalloc(&s->key, n)); WD_TRY(dalloc(&s->slot, n)); WD_TRY(dalloc(&s->tmp_id, n));
     const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
@@ -494,7 +492,7 @@ int ingest(wd_sim* s, const float* st, const int* mat) {
     if (mat) WD_HIP(hipMemcpy(s->mat_d, mat, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
     const int gb = grain_blocks(s->n);
     hipLaunchKernelGGL(k_ingest, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->parts, s->st6,
-                       s->mat_d, s->Tp, s->Tv, s->key, s->HTc, s->n);
+                       s->mat_d, s->Tp, s->Tv, s->key, s->Hc[0], s->Hc[1], s->n);
     s->finalize(gb, kFinBox);
     WD_HIP(hipGetLastError());
     return s->sync();
@@ -925,9 +923,16 @@ extern "C" int wd_get_history(wd_sim* s, int* partner, float* xi, int* count) {
     if (rc == WD_OK) rc = dalloc(&dx, 3 * slots);
     if (rc == WD_OK) rc = dalloc(&dc, s->n);
     if (rc == WD_OK) {
-        hipLaunchKernelGGL(k_export_history, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           s->stream, s->Tp, s->HT, s->HTc, dp_, dx, dc, s->n);
         rc = s->sync();
+        int par = 0;
+        if (rc == WD_OK && hipMemcpy(&par, &s->ctl->hist_parity, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = fail(WD_ERR_HIP, "wd_get_history: parity read failed");
+        }
+        if (rc == WD_OK) {
+            hipLaunchKernelGGL(k_export_history, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               s->stream, s->H[par], s->Hc[par], dp_, dx, dc, s->n);
+            rc = s->sync();
+        }
     }
     if (rc == WD_OK && (hipMemcpy(partner, dp_, sizeof(int) * slots, hipMemcpyDeviceToHost) != hipSuccess ||
                         hipMemcpy(xi, dx, sizeof(float) * 3 * slots, hipMemcpyDeviceToHost) != hipSuccess ||

@@ -28,7 +28,7 @@ struct Ctl {
     int error;                 // nonzero: a bounded wait gave up
     int energy_count;          // entries written to the energy log
     int energy_capacity;
-    int pad;
+    int hist_parity;           // current contact-history generation; flipped by k_finalize
     double energy_last;        // last standalone energy
 };
 
@@ -46,7 +46,8 @@ enum FinalizeFlags {
     kFinAccel = 2,      // accel_last <- reduced max
     kFinEnergyLog = 4,  // append the energy sum to the log
     kFinEnergyLast = 8, // energy_last <- energy sum
-    kFinCicScale = 16   // cic_scale_bits <- reduced max
+    kFinCicScale = 16,  // cic_scale_bits <- reduced max
+    kFinFlipHistory = 32  // the step is complete: its springs become current
 };
 
 // Launch geometry helpers: grid rounded up to a multiple of the XCD count,
@@ -64,12 +65,13 @@ __device__ __forceinline__ int logical_block() {
 
 enum ForceMode { kModeSIE = 0, kModeBEP = 1, kModeRHS = 2 };
 
-// Contact-history buffers of one force pass (slot-major [kHist][n]).
+// Contact-history buffers (slot-major [kHist][n], indexed by ORIGINAL grain
+// id, so they never move with the cell order).  Two generations: the force
+// pass reads generation ctl->hist_parity and writes the other one;
+// k_finalize flips the parity once a step is complete.
 struct HistoryIO {
-    const float4* in;   // springs of the evaluated state, cell-sorted slot order
-    const int* cnt_in;
-    float4* out;        // springs after the step, same slot order (unused for kModeRHS)
-    int* cnt_out;
+    float4* buf[2];
+    int* cnt[2];
 };
 
 // What k_reorder carries along the permutation besides the grain.
@@ -78,10 +80,6 @@ struct Carry {
     const float4* nin_v;
     float4* nout_p;       // Picard base state out (NMODE 1, 2)
     float4* nout_v;
-    const float4* hin;    // contact history in (HIST)
-    const int* hcnt_in;
-    float4* hout;         // contact history out (HIST)
-    int* hcnt_out;
 };
 
 }  // namespace wd

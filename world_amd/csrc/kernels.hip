@@ -83,6 +83,7 @@ __device__ void store_block_part(BlockPart* parts, int slot, const int lo[3], co
                 wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]),
                 static_cast<int>(wave_max_u(accel))};
     if (lane == 0) {
+        #pragma unroll
         for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
     }
     const double e = with_energy ? block_sum(energy) : 0.0;  // contains __syncthreads
@@ -90,11 +91,13 @@ __device__ void store_block_part(BlockPart* parts, int slot, const int lo[3], co
     if (threadIdx.x == 0) {
         unsigned am = static_cast<unsigned>(s_box[0][6]);
         for (int w = 1; w < nw; ++w) {
+            #pragma unroll
             for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
             for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
             am = max(am, static_cast<unsigned>(s_box[w][6]));
         }
         BlockPart bp;
+        #pragma unroll
         for (int k = 0; k < 6; ++k) bp.box[k] = v[k];
         bp.accel = am;
         bp.pad = 0;
@@ -113,7 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
                                                    const int* __restrict__ mat,
                                                    float4* __restrict__ Tp, float4* __restrict__ Tv,
                                                    uint32_t* __restrict__ key,
-                                                   int* __restrict__ hist_cnt, int n) {
+                                                   int* __restrict__ hist_cnt0,
+                                                   int* __restrict__ hist_cnt1, int n) {
     const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
@@ -122,7 +126,10 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
         const float x = s[0], y = s[1], z = s[2];
         Tp[i] = make_float4(x, y, z, __int_as_float(i));
         Tv[i] = make_float4(s[3], s[4], s[5], __int_as_float(mat[i]));
-        if (hist_cnt) hist_cnt[i] = 0;  // an ingested state starts with no contact history
+        if (hist_cnt0) {  // an ingested state starts with no contact history
+            hist_cnt0[i] = 0;
+            hist_cnt1[i] = 0;
+        }
         int cx, cy, cz;
         grain_cell(P, x, y, z, cx, cy, cz);
         key[i] = pack_cell(cx, cy, cz);
@@ -301,9 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Ctl* __restrict__ ctl,
 
 // Pass 4: final position = cell start + rank of the original id among the
 // cell's ids.  NMODE 1 also copies the grain into the Picard base buffers,
-// NMODE 2 carries the Picard base along the same permutation; HIST carries
-// the contact-history lists (count and entries) along with the grain.
-template <int NMODE, bool HIST>
+// NMODE 2 carries the Picard base along the same permutation.
+template <int NMODE>
 __global__ __launch_bounds__(kBlock) void k_reorder(
         const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
@@ -333,13 +339,6 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
         carry.nout_p[dst] = carry.nin_p[i];
         carry.nout_v[dst] = carry.nin_v[i];
     }
-    if (HIST) {
-        const int h = carry.hcnt_in[i];
-        carry.hcnt_out[dst] = h;
-        for (int k = 0; k < h; ++k) {
-            carry.hout[static_cast<size_t>(k) * n + dst] = carry.hin[static_cast<size_t>(k) * n + i];
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -349,9 +348,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // rhs = [v, F/m] in original order.  ENERGY adds the per-grain energy of S
 // (pair terms counted once, from the lower original index) to the block
 // partials.  HIST selects the tangential spring with history: the stored
-// springs of S are read from hist.in (slot-major [kHist][n]) and the
-// updated springs of this step are written to hist.out in contact order
-// (walls, then pairs in stencil order).
+// springs of S are read from generation ctl->hist_parity (indexed by original
+// id) and the updated springs of this step are written to the other
+// generation in contact order (walls, then pairs in stencil order).
 template <int MODE, bool ENERGY, bool HIST>
 __global__ __launch_bounds__(kBlock) void k_force(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
@@ -377,27 +376,39 @@ __global__ __launch_bounds__(kBlock) void k_force(
         float f[3] = {0.f, 0.f, 0.f};
         if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
 
-        // Contact history: old springs matched by partner with a cursor (the
-        // old list was written in the same contact order), new springs
-        // appended in contact order.
-        const int h_old = HIST ? hist.cnt_in[p] : 0;
+        // Contact history: the old list was written in the same contact order,
+        // so its next entry is prefetched one contact ahead and normally
+        // matches; a miss falls back to a search of the whole list.
+        const int par = HIST ? ctl->hist_parity : 0;
+        const float4* __restrict__ h_in = hist.buf[par];
+        float4* __restrict__ h_out = hist.buf[par ^ 1];
+        const int h_old = HIST ? hist.cnt[par][id] : 0;
         int h_cur = 0, h_new = 0;
-        auto h_take = [&](int partner, float xi[3]) {
-            xi[0] = xi[1] = xi[2] = 0.f;
-            for (int t = 0; t < h_old; ++t) {
-                int k = h_cur + t;
-                if (k >= h_old) k -= h_old;
-                const float4 h = hist.in[static_cast<size_t>(k) * n + p];
-                if (__float_as_int(h.w) == partner) {
-                    xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
-                    h_cur = k + 1;
-                    return;
+        float4 h_next = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (HIST && h_old > 0) h_next = h_in[id];
+        auto h_take = [&](int partner, float xi[3]) __attribute__((always_inline)) {
+            if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
+                xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
+            } else {
+                xi[0] = xi[1] = xi[2] = 0.f;
+                int found = -1;
+                for (int t = 0; t < h_old; ++t) {
+                    const float4 h = h_in[static_cast<size_t>(t) * n + id];
+                    if (__float_as_int(h.w) == partner) {
+                        xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
+                        found = t;
+                        break;
+                    }
                 }
+                if (found < 0) return;  // a new contact: the spring starts at zero
+                h_cur = found;
             }
+            ++h_cur;
+            if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + id];
         };
-        auto h_put = [&](int partner, const float xi[3]) {
+        auto h_put = [&](int partner, const float xi[3]) __attribute__((always_inline)) {
             if (MODE != kModeRHS && h_new < kHist) {
-                hist.out[static_cast<size_t>(h_new) * n + p] =
+                h_out[static_cast<size_t>(h_new) * n + id] =
                         make_float4(xi[0], xi[1], xi[2], __int_as_float(partner));
             }
             ++h_new;
@@ -407,7 +418,9 @@ __global__ __launch_bounds__(kBlock) void k_force(
         // (src/compute_rhs_kernel.cu:105-141).
         const float c_wall = P.c_wall[mat];
         f[2] -= m * P.gravity;
+        #pragma unroll
         for (int a = 0; a < 3; ++a) {
+            #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const float dist = side == 0 ? x[a] - P.wlo[a] - P.radius : P.whi[a] - x[a] - P.radius;
                 const float fw = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
@@ -435,6 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                         h_put(wall_partner(a, side), xi);
                     } else {
                         float vt[3];
+                        #pragma unroll
                         for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
                         viscous_friction(f, vt, P.mu * fw, c_wall);
                     }
@@ -446,18 +460,26 @@ __global__ __launch_bounds__(kBlock) void k_force(
         int cx, cy, cz;
         grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
         const uint32_t kcap = static_cast<uint32_t>(P.kcap);
-        // Cell starts of the 9 stencil columns (cz-1 .. cz+2 each), issued
-        // together so their latencies overlap.
-        uint4 col[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int base = g.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1);
-            __builtin_memcpy(&col[c], start + base, sizeof(uint4));
-        }
+        // Cell starts of stencil column c (cells cz-1 .. cz+1 and the next
+        // cell's start), one 16-byte load.
+        auto column = [&](int c) __attribute__((always_inline)) {
+            uint4 q4;
+            __builtin_memcpy(&q4, start + g.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+            return q4;
+        };
+        // Candidates of segment z (0..2) of a column: cell z's first kcap grains.
+        auto seg_begin = [](const uint4& q4, int z) __attribute__((always_inline)) {
+            return z == 0 ? q4.x : (z == 1 ? q4.y : q4.z);
+        };
+        auto seg_end = [kcap](const uint4& q4, int z) __attribute__((always_inline)) {
+            const uint32_t b = z == 0 ? q4.x : (z == 1 ? q4.y : q4.z);
+            const uint32_t e = z == 0 ? q4.y : (z == 1 ? q4.z : q4.w);
+            return min(e, b + kcap);
+        };
         // Phase 2: the full contact law for the listed candidates, in the
         // order phase 1 found them (the canonical stencil order).
         int listed = 0;
-        auto flush = [&]() {
+        auto flush = [&]() __attribute__((always_inline)) {
             for (int k = 0; k < listed; ++k) {
                 const uint32_t q = s_list[k][threadIdx.x];
                 const float4 pj = Sp[q];
@@ -483,6 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                     vn += vr[0] * nrm[0];
                     vn += vr[1] * nrm[1];
                     vn += vr[2] * nrm[2];
+#pragma unroll
                     for (int a = 0; a < 3; ++a) f[a] -= c * vn * nrm[a];
                     float xi[3];
                     h_take(jd, xi);
@@ -501,8 +524,9 @@ __global__ __launch_bounds__(kBlock) void k_force(
         };
         // Phase 1: squared distances only.  A candidate beyond contact adds
         // exact zeros in the reference's arithmetic, so only touching (or
-        // NaN) candidates are listed; a full list is flushed in place.
-        auto probe = [&](uint32_t q, float4 pj) {
+        // NaN) candidates are listed.  The scan stops when the list is full,
+        // phase 2 drains it, and the scan resumes where it stopped.
+        auto probe = [&](uint32_t q, float4 pj) __attribute__((always_inline)) {
             const float d0 = x[0] - pj.x, d1 = x[1] - pj.y, d2 = x[2] - pj.z;
             float s2 = 0.f;
             s2 += d0 * d0;
@@ -510,31 +534,42 @@ __global__ __launch_bounds__(kBlock) void k_force(
             s2 += d2 * d2;
             if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
                 s_list[listed][threadIdx.x] = q;
-                if (++listed == kContactList) flush();
+                ++listed;
             }
         };
-        auto scan_range = [&](uint32_t b, uint32_t e) {
-            uint32_t q = b;
-            for (; q + 1 < e; q += 2) {
-                const float4 a0 = Sp[q], a1 = Sp[q + 1];
-                probe(q, a0);
-                probe(q + 1, a1);
+        int c = 0, z = 0;
+        uint4 quad = column(0);
+        uint4 next = column(1);
+        uint32_t q = seg_begin(quad, 0), e = seg_end(quad, 0);
+        for (;;) {
+            while (listed <= kContactList - 2) {
+                if (q + 1 < e) {  // two candidates in flight
+                    const float4 a0 = Sp[q], a1 = Sp[q + 1];
+                    probe(q, a0);
+                    probe(q + 1, a1);
+                    q += 2;
+                } else if (q < e) {
+                    probe(q, Sp[q]);
+                    ++q;
+                } else if (z < 2) {
+                    ++z;
+                    q = seg_begin(quad, z);
+                    e = seg_end(quad, z);
+                } else if (c < 8) {
+                    ++c;
+                    quad = next;
+                    if (c < 8) next = column(c + 1);
+                    z = 0;
+                    q = seg_begin(quad, 0);
+                    e = seg_end(quad, 0);
+                } else {
+                    break;
+                }
             }
-            if (q < e) probe(q, Sp[q]);
-        };
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const uint32_t s0 = col[c].x, s1 = col[c].y, s2 = col[c].z, s3 = col[c].w;
-            if (s1 - s0 <= kcap && s2 - s1 <= kcap && s3 - s2 <= kcap) {
-                scan_range(s0, s3);  // the column's three cells are contiguous
-            } else {
-                scan_range(s0, min(s1, s0 + kcap));
-                scan_range(s1, min(s2, s1 + kcap));
-                scan_range(s2, min(s3, s2 + kcap));
-            }
+            flush();
+            if (c == 8 && z == 2 && q >= e) break;
         }
-        flush();
-        if (HIST && MODE != kModeRHS) hist.cnt_out[p] = min(h_new, kHist);
+        if (HIST && MODE != kModeRHS) hist.cnt[par ^ 1][id] = min(h_new, kHist);
 
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
         if (MODE == kModeRHS) {
@@ -546,11 +581,14 @@ __global__ __launch_bounds__(kBlock) void k_force(
         } else {
             float vn[3], xn[3];
             if (MODE == kModeSIE) {
+                #pragma unroll
                 for (int k = 0; k < 3; ++k) vn[k] = v[k] + P.dt * a[k];
+                #pragma unroll
                 for (int k = 0; k < 3; ++k) xn[k] = x[k] + P.dt * vn[k];
             } else {
                 const float4 bp = Np[p], bv = Nv[p];
                 const float x0[3] = {bp.x, bp.y, bp.z}, v0[3] = {bv.x, bv.y, bv.z};
+                #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     vn[k] = v0[k] + P.dt * a[k];
                     xn[k] = x0[k] + P.dt * vn[k];
@@ -588,6 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* 
     double e = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         const BlockPart& bp = parts[b];
+        #pragma unroll
         for (int k = 0; k < 3; ++k) {
             lo[k] = min(lo[k], bp.box[k]);
             hi[k] = max(hi[k], bp.box[k + 3]);
@@ -600,38 +639,39 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* 
     int v[7] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
                 wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]), static_cast<int>(wave_max_u(am))};
     if (lane == 0) {
+        #pragma unroll
         for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
     unsigned amax = static_cast<unsigned>(s_box[0][6]);
     for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        #pragma unroll
         for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
         for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
         amax = max(amax, static_cast<unsigned>(s_box[w][6]));
     }
     if (flags & kFinBox) {
+        #pragma unroll
         for (int k = 0; k < 6; ++k) ctl->bbox_cur[k] = v[k];
     }
     if (flags & kFinAccel) ctl->accel_last = amax;
     if (flags & kFinCicScale) ctl->cic_scale_bits = amax;
     if (flags & kFinEnergyLast) ctl->energy_last = total;
+    if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
     if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
         energy_log[ctl->energy_count] = total;
         ctl->energy_count = ctl->energy_count + 1;
     }
 }
 
-#define WD_INST_REORDER(M, H)                                                                \
-    template __global__ void k_reorder<M, H>(const Ctl*, const uint32_t*, const uint32_t*,       \
-                                             const int*, const float4*, const float4*, float4*,  \
-                                             float4*, Carry, int);
-WD_INST_REORDER(0, false)
-WD_INST_REORDER(1, false)
-WD_INST_REORDER(2, false)
-WD_INST_REORDER(0, true)
-WD_INST_REORDER(1, true)
-WD_INST_REORDER(2, true)
+#define WD_INST_REORDER(M)                                                                   \
+    template __global__ void k_reorder<M>(const Ctl*, const uint32_t*, const uint32_t*,          \
+                                          const int*, const float4*, const float4*, float4*,     \
+                                          float4*, Carry, int);
+WD_INST_REORDER(0)
+WD_INST_REORDER(1)
+WD_INST_REORDER(2)
 #undef WD_INST_REORDER
 
 #define WD_INST_FORCE(M, E, H)                                                                  \
@@ -701,23 +741,21 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
     row[w] = -1;
 }
 
-// Contact-history lists of the working layout, in original grain order:
+// Contact-history lists of one generation, transposed to per-grain rows:
 // partner[id * kHist + k], xi[(id * kHist + k) * 3 + a], count[id].
-__global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restrict__ Tp,
-                                                           const float4* __restrict__ hist,
+__global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restrict__ hist,
                                                            const int* __restrict__ cnt,
                                                            int* __restrict__ partner,
                                                            float* __restrict__ xi,
                                                            int* __restrict__ count, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int id = __float_as_int(Tp[i].w);
-    const int h = cnt[i];
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int h = cnt[id];
     count[id] = h;
     for (int k = 0; k < kHist; ++k) {
         const size_t o = static_cast<size_t>(id) * kHist + k;
         if (k < h) {
-            const float4 e = hist[static_cast<size_t>(k) * n + i];
+            const float4 e = hist[static_cast<size_t>(k) * n + id];
             partner[o] = __float_as_int(e.w);
             xi[3 * o] = e.x; xi[3 * o + 1] = e.y; xi[3 * o + 2] = e.z;
         } else {
@@ -810,14 +848,18 @@ __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict
     axis_weights(p.z, P.lo[2], P.hi[2], P.dgz, iz[0], iz[1], wz[0], wz[1]);
     const float mom[3] = {m * v.x, m * v.y, m * v.z};
     const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
+    #pragma unroll
     for (int a = 0; a < 2; ++a) {
+        #pragma unroll
         for (int b = 0; b < 2; ++b) {
+            #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const float w = wx[a] * wy[b] * wz[c];
                 if (w == 0.f) continue;
                 const size_t node = (static_cast<size_t>(ix[a]) * P.dgy + iy[b]) * P.dgz + iz[c];
                 atomicAdd(&acc[node], static_cast<unsigned long long>(
                                               llrint(static_cast<double>(m * w) * scale)));
+                #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     atomicAdd(&acc[(k + 1) * nodes + node],
                               static_cast<unsigned long long>(

@@ -3,8 +3,10 @@
 Material and box shared by all of them: monodisperse spheres of diameter
 d = 0.01 m and density 2500 kg/m^3 (m = 1.309e-3 kg), g = 9.81, a linear
 spring-dashpot normal contact of stiffness k_n = 150 N/m with restitution
-0.5, collision cells one diameter wide, 8 grains per cell, and a walled box
-whose lowest lattice layer rests one radius above the floor.
+0.5, a tangential spring k_t = (2/7) k_n with contact history capped at the
+Coulomb limit (physics.tangential_model: spring_history), collision cells one
+diameter wide, 8 grains per cell, and a walled box whose lowest lattice layer
+rests one radius above the floor.
 
 The spring is set through physics.normal_stiffness with the force clamp
 (max_force) far above any load these columns carry.  The reference's own
@@ -18,7 +20,8 @@ from __future__ import annotations
 import math
 
 from ._capi import (WD_BACKWARD_EULER_PICARD, WD_INIT_CUBE, WD_PERTURB_GAUSSIAN,
-                    WD_SEMI_IMPLICIT_EULER, WdConfig, default_config)
+                    WD_SEMI_IMPLICIT_EULER, WD_TANGENTIAL_SPRING_HISTORY, WdConfig,
+                    default_config)
 
 DIAMETER = 0.01
 RADIUS = DIAMETER / 2
@@ -59,6 +62,8 @@ def column(nx: int, ny: int, nz: int, *, half_width: float = 0.9, ceiling: float
     c.wall_y_min, c.wall_y_max = -half_width, half_width
     c.restitution = 0.5
     c.friction = friction
+    c.tangential_model = WD_TANGENTIAL_SPRING_HISTORY
+    c.tangential_stiffness_ratio = 2.0 / 7.0
     c.set_masses([0.0, GRAIN_MASS])
     c.init_type = WD_INIT_CUBE
     pitch = 2 * SPACING * RADIUS
