@@ -233,6 +233,32 @@ int wd_overflow_count(wd_sim* sim, int* count_out);
  * its time is reported under COMPUTE_RHS and TAKE_STEP stays 0. */
 int wd_stage_ms(wd_sim* sim, float out[4]);
 
+/* ---- batched ensemble (extension) ---------------------------------------
+ * The reference runs ensemble members one after another, each in its own
+ * ParticleDynamics, and rejects members of unequal grain counts
+ * (src/chaos.cpp:163-171; the dataset driver's row loop, src/dataset.cpp:
+ * 189-232).  A batch handle advances E members with one launch sequence per
+ * step.  Members may differ in their initialization (grain counts, seeds,
+ * perturbation) and in physics.friction; every other field must match.
+ * Each member evolves exactly as it would alone: the same cell order, the
+ * same force sums and therefore the same bits.
+ *
+ * Every single-simulation entry point also accepts a batch handle and then
+ * works on the concatenation of the members: state, material and history
+ * arrays hold member 0's grains, then member 1's, ... (grain ids are global,
+ * offset by wd_batch_member_range's `first`); wd_total_energy and the energy
+ * log sum over all members; wd_max_accel is the maximum over all members;
+ * wd_overflow_count counts over-capacity cells of every member;
+ * wd_density_grid returns member 0's grid; each density-log record holds the
+ * E grids back to back. */
+int wd_batch_create(const wd_config* configs, int members, wd_sim** sim_out);
+int wd_batch_members(const wd_sim* sim, int* members_out);
+/* Grains of member m are [first, first + count) of the concatenated arrays. */
+int wd_batch_member_range(const wd_sim* sim, int member, int* first_out, int* count_out);
+/* CIC grid (4 * nodes floats, src/particle_dynamics.cu:289-293) of one
+ * member, or of all members back to back when member is -1. */
+int wd_batch_density_grid(wd_sim* sim, int member, float* grid_out);
+
 /* ---- parity / inspection entry points ---------------------------------- */
 
 /* Extension, tangential_model spring_history: the stored contact springs in

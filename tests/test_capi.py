@@ -73,3 +73,29 @@ def test_file_initializer_round_trip(tmp_path):
         assert "short" in str(e)
     else:
         raise AssertionError("short state file accepted")
+
+
+def test_batch_rejects_incompatible_members_before_device_work():
+    import ctypes as C
+
+    a = workloads.column(3, 3, 3, friction=0.1)
+    b = workloads.column(4, 4, 4, friction=0.5)
+    b.restitution = 0.7  # members may differ only in initialization and friction
+    arr = (_capi.WdConfig * 2)(a, b)
+    h = C.c_void_p()
+    rc = _capi.lib().wd_batch_create(arr, 2, C.byref(h))
+    assert rc == _capi.WD_ERR_ARG and not h.value
+    assert b"ensemble members differ" in _capi.lib().wd_last_error()
+    rc = _capi.lib().wd_batch_create(arr, 0, C.byref(h))
+    assert rc == _capi.WD_ERR_ARG
+
+
+def test_uneven_ensemble_workload():
+    cs = workloads.uneven_ensemble()
+    assert len(cs) == 64
+    n = []
+    for m, c in enumerate(cs):
+        n.append(_capi.grain_count(c))
+        assert c.init_perturbation_seed == 1000 + m
+        assert 0.0 <= c.friction <= 0.6
+    assert 2048 * 0.8 < min(n) and max(n) <= 32768 * 1.1 and len(set(n)) > 32

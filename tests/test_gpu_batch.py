@@ -1,0 +1,104 @@
+"""Batched uneven ensemble: every member of one batch handle evolves bit for
+bit as the same configuration run alone.
+
+The reference runs members sequentially and requires equal grain counts
+(src/chaos.cpp:163-171).  Here members differ in grain count (8 to 4,096,
+so that 256-grain blocks span up to four members), initialization and
+friction; their states, contact springs and CIC grids must equal the single
+runs' exactly, and the single runs are pinned to the oracle by
+test_gpu_parity.py."""
+import numpy as np
+import pytest
+
+from world_amd import _capi, workloads
+from world_amd.sim import Ensemble, ParticleDynamics
+
+pytestmark = pytest.mark.gpu
+
+
+def members(integrator=_capi.WD_SEMI_IMPLICIT_EULER):
+    cs = [
+        workloads.column(3, 3, 3, friction=0.2),
+        workloads.column(2, 2, 2, friction=0.0),
+        workloads.column(4, 4, 5, friction=0.6),
+        workloads.perturbed(workloads.column_collapse_4k(), 50e-6, 1001),
+        workloads.column(7, 7, 20, friction=0.4, seed=77),
+    ]
+    for c in cs:
+        c.time_integrator = integrator
+    return cs
+
+
+def jittered(c, seed):
+    st, mat = _capi.initial_state(c)
+    st = st.reshape(-1, 6).copy()
+    rng = np.random.default_rng(seed)
+    st[:, :3] += rng.normal(0, 0.0015, (len(st), 3)).astype(np.float32)
+    st[:, 3:] = rng.normal(0, 0.2, (len(st), 3)).astype(np.float32)
+    return st
+
+
+def singles(cs, nsteps, states=None):
+    out = []
+    for m, c in enumerate(cs):
+        with ParticleDynamics(c) as sim:
+            if states is not None:
+                sim.set_state(states[m])
+            sim.take_step(nsteps)
+            sim.compute_density_grid()
+            out.append(dict(state=sim.state(), history=sim.history(),
+                            grid=sim.host_density_grid.copy(), amax=sim.compute_max_acceleration(),
+                            energy=sim.compute_total_energy()))
+    return out
+
+
+def check_members(ens, want):
+    got = ens.member_states()
+    partner, xi, cnt = ens.history()
+    for m, (first, count) in enumerate(ens.ranges):
+        np.testing.assert_array_equal(got[m], want[m]["state"], err_msg=f"member {m}")
+        wp, wx, wc = want[m]["history"]
+        np.testing.assert_array_equal(cnt[first:first + count], wc)
+        for i in range(count):
+            k = wc[i]
+            p = partner[first + i, :k]
+            p = np.where(p >= 0, p - first, p)
+            np.testing.assert_array_equal(p, wp[i, :k])
+            np.testing.assert_array_equal(xi[first + i, :k], wx[i, :k])
+        np.testing.assert_array_equal(ens.member_density_grid(m), want[m]["grid"])
+    assert ens.compute_max_acceleration() == max(w["amax"] for w in want)
+
+
+@pytest.mark.parametrize("integrator", [_capi.WD_SEMI_IMPLICIT_EULER, _capi.WD_BACKWARD_EULER_PICARD])
+def test_batch_from_initialization_equals_single_runs(integrator):
+    cs = members(integrator)
+    want = singles(cs, 40)
+    with Ensemble(cs) as ens:
+        assert ens.members == len(cs)
+        assert [c for _, c in ens.ranges] == [len(w["state"]) for w in want]
+        ens.take_step(40)
+        check_members(ens, want)
+        e = ens.compute_total_energy()
+        assert e == pytest.approx(sum(w["energy"] for w in want), rel=1e-5)
+
+
+def test_batch_contact_rich_equals_single_runs():
+    cs = members()
+    states = [jittered(c, 10 + m) for m, c in enumerate(cs)]
+    want = singles(cs, 37, states)
+    with Ensemble(cs) as ens:
+        ens.set_state(np.concatenate(states))
+        for k in (1, 16, 20):  # eager, one graph chunk, chunk + remainder
+            ens.take_step(k)
+        check_members(ens, want)
+
+
+def test_batch_density_grids_block():
+    cs = members()
+    with Ensemble(cs) as ens:
+        ens.take_step(5)
+        allg = ens.density_grids()
+        for m in range(ens.members):
+            np.testing.assert_array_equal(allg[m], ens.member_density_grid(m))
+        ens.compute_density_grid()
+        np.testing.assert_array_equal(ens.host_density_grid, allg[0])

@@ -21,6 +21,7 @@ WD_INIT_CUBE, WD_INIT_TWO_PARTICLES, WD_INIT_SINGLE_PARTICLE, WD_INIT_FILE = ran
 WD_SEMI_IMPLICIT_EULER, WD_BACKWARD_EULER_PICARD = 0, 1
 WD_TANGENTIAL_VISCOUS, WD_TANGENTIAL_SPRING_HISTORY = 0, 1
 WD_PERTURB_UNIFORM, WD_PERTURB_GAUSSIAN = 0, 1
+WD_OK, WD_ERR_ARG, WD_ERR_HIP, WD_ERR_IO, WD_ERR_STATE, WD_ERR_DEVICE = 0, -1, -2, -3, -4, -5
 
 
 class WdConfig(C.Structure):
@@ -128,6 +129,11 @@ _SIGNATURES = [
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_get_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("wd_batch_create", C.c_int, [C.POINTER(WdConfig), C.c_int, C.POINTER(C.c_void_p)]),
+    ("wd_batch_members", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("wd_batch_member_range", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
+    ("wd_batch_density_grid", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]
@@ -176,6 +182,12 @@ def load_config(path: str | os.PathLike) -> WdConfig:
     cfg = WdConfig()
     check(lib().wd_config_load(str(path).encode(), C.byref(cfg)))
     return cfg
+
+
+def grain_count(cfg: WdConfig) -> int:
+    n = C.c_int(0)
+    check(lib().wd_config_grain_count(C.byref(cfg), C.byref(n)))
+    return n.value
 
 
 def initial_state(cfg: WdConfig) -> tuple[np.ndarray, np.ndarray]:

@@ -5,7 +5,6 @@
 // -ffp-contract=off and correctly rounded division and square root, so the
 // results are reproducible bit for bit by a CPU evaluation of the same order):
 //   cell_axis          src/find_neighbors_kernel.cu:11-22
-//   wall_contacts      src/compute_rhs_kernel.cu:110-141
 //   pair_contact       src/compute_rhs_kernel.cu:145-197
 //   viscous_friction   src/compute_rhs_kernel.cu:66-78
 //   contact_energy     src/energy_kernel.cu:22-28
@@ -43,6 +42,16 @@ struct Params {
     float kt;              // tangential spring stiffness (history model)
     int history;           // 1: tangential spring with history, 0: viscous (reference)
     int dgx, dgy, dgz;     // density grid
+    // Ensemble members (1 for a single simulation).  Member m owns grains
+    // with ids [mgoff[m], mgoff[m+1]) and the cell range
+    // [mcell[m], mcell[m+1]) of the concatenated member sub-grids; mbox and
+    // mcell are rewritten by k_finalize for every new state.
+    int members;
+    const int* mgoff;      // [members + 1]
+    int* mbox;             // [members][6] occupied cells of the current state
+    uint32_t* mcell;       // [members + 1]
+    const float* mmu;      // [members] friction coefficient
+    unsigned* mscale;      // [members] float bits of the largest CIC deposit magnitude
     float mass[kMaxMat];
     float c_wall[kMaxMat];
     float c_pair[kMaxMat * kMaxMat];
@@ -81,6 +90,10 @@ __device__ __forceinline__ void grain_cell(const Params& P, float x, float y, fl
     cz = cell_axis(z, P.lo[2], P.hi[2], P.gz);
 }
 
+// Member index and material id of a grain, packed in its velocity's .w.
+__device__ __forceinline__ int grain_member(float w) { return __float_as_int(w) >> 4; }
+__device__ __forceinline__ int grain_mat(float w) { return __float_as_int(w) & (kMaxMat - 1); }
+
 // Sub-grid covering the occupied cells' bounding box with one empty cell of
 // padding on every side, so every 27-cell stencil of an occupied cell is
 // addressable without bounds checks.
@@ -104,6 +117,28 @@ __device__ __forceinline__ SubGrid make_subgrid(const int* bbox) {
     return s;
 }
 
+// The cells of member m: its padded sub-grid, placed at `base` in the
+// concatenated cell arrays.
+struct MemberCells {
+    SubGrid g;
+    uint32_t base;
+    __device__ __forceinline__ uint32_t index(int cx, int cy, int cz) const {
+        return base + static_cast<uint32_t>(g.index(cx, cy, cz));
+    }
+};
+
+__device__ __forceinline__ MemberCells member_cells(const Params& P, int m) {
+    MemberCells c;
+    c.g = make_subgrid(P.mbox + 6 * m);
+    c.base = P.mcell[m];
+    return c;
+}
+
+// Member of a grain given its velocity .w (uniformly 0 for one member).
+__device__ __forceinline__ int member_of(const Params& P, float w) {
+    return P.members == 1 ? 0 : grain_member(w);
+}
+
 // Reference viscous Coulomb friction: oppose the tangential velocity,
 // magnitude min(limit, c * |v_t|); no force below |v_t| = 1e-8.
 __device__ __forceinline__ void viscous_friction(float f[3], const float vt[3], float limit,
@@ -118,32 +153,6 @@ __device__ __forceinline__ void viscous_friction(float f[3], const float vt[3], 
     f[0] -= mag * vt[0] / n;
     f[1] -= mag * vt[1] / n;
     f[2] -= mag * vt[2] / n;
-}
-
-// Gravity and the six axis-aligned walls, lower then upper per axis.
-__device__ __forceinline__ void wall_contacts(const Params& P, const float x[3], const float v[3],
-                                              float m_i, float c_wall, float f[3]) {
-    f[2] -= m_i * P.gravity;
-    #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float vt[3];
-        #pragma unroll
-        for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
-        const float dl = x[a] - P.wlo[a] - P.radius;
-        const float fl = fmaxf(0.f, fminf(P.max_force, P.kneg * dl));
-        f[a] += fl;
-        if (dl < 0.f) {
-            f[a] -= c_wall * v[a];
-            viscous_friction(f, vt, P.mu * fl, c_wall);
-        }
-        const float du = P.whi[a] - x[a] - P.radius;
-        const float fu = fmaxf(0.f, fminf(P.max_force, P.kneg * du));
-        f[a] -= fu;
-        if (du < 0.f) {
-            f[a] -= c_wall * v[a];
-            viscous_friction(f, vt, P.mu * fu, c_wall);
-        }
-    }
 }
 
 // Centre distance |x_i - x_j| with the reference's summation order; also
@@ -163,7 +172,7 @@ __device__ __forceinline__ float pair_norm(const float xi[3], float xj0, float x
 // One neighbor j (cell-sorted slot q) acting on grain i, given the
 // separation d and its norm.  A pair beyond contact adds a zero spring force,
 // as in the reference.
-__device__ __forceinline__ void pair_contact(const Params& P, const float vi[3], int mat_i,
+__device__ __forceinline__ void pair_contact(const Params& P, float mu, const float vi[3], int mat_i,
                                              float d0, float d1, float d2, float norm,
                                              const float4* __restrict__ vel, int q, float f[3]) {
     const float dist = norm - P.radius - P.radius;
@@ -174,7 +183,7 @@ __device__ __forceinline__ void pair_contact(const Params& P, const float vi[3],
     f[2] += fp * n2;
     if (dist < 0.f) {
         const float4 vj = vel[q];
-        const int mat_j = __float_as_int(vj.w);
+        const int mat_j = grain_mat(vj.w);
         const float c = P.c_pair[mat_i * kMaxMat + mat_j];
         const float r0 = vi[0] - vj.x, r1 = vi[1] - vj.y, r2 = vi[2] - vj.z;
         float vn = 0.f;
@@ -188,7 +197,7 @@ __device__ __forceinline__ void pair_contact(const Params& P, const float vi[3],
         vt[1] = r1 - vn * n1;
         f[2] -= c * vn * n2;
         vt[2] = r2 - vn * n2;
-        viscous_friction(f, vt, P.mu * fp, c);
+        viscous_friction(f, vt, mu * fp, c);
     }
 }
 
@@ -199,9 +208,9 @@ __device__ __forceinline__ void pair_contact(const Params& P, const float vi[3],
 // Coulomb limit mu * fp the force is scaled back to the limit and the spring
 // reset to match it (slip).  Returns 1 on slip.  n: unit normal into grain i;
 // vr: relative velocity of i; vn = vr . n.
-__device__ __forceinline__ int spring_friction(const Params& P, float f[3], const float n[3],
-                                               const float vr[3], float vn, float fp, float c,
-                                               float xi[3]) {
+__device__ __forceinline__ int spring_friction(const Params& P, float mu, float f[3],
+                                               const float n[3], const float vr[3], float vn,
+                                               float fp, float c, float xi[3]) {
     float vt[3];
     #pragma unroll
     for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
@@ -221,7 +230,7 @@ __device__ __forceinline__ int spring_friction(const Params& P, float f[3], cons
     s += ft[1] * ft[1];
     s += ft[2] * ft[2];
     const float fm = sqrtf(s);
-    const float lim = P.mu * fp;
+    const float lim = mu * fp;
     int slip = 0;
     if (fm > lim) {
         const float scale = lim / fm;

@@ -69,7 +69,17 @@ struct wd_sim {
     unsigned long long* status = nullptr;
     size_t scan_cells = 0, ntiles_max = 0;
     float* st6 = nullptr;
-    int* mat_d = nullptr;
+    int* mat_d = nullptr;  // original-order (member << 4 | material) codes
+    // ensemble members (1 for a single simulation)
+    int members = 1;
+    std::vector<int> goff;             // grain id offsets, members + 1
+    std::vector<wd_config> mcfg;       // per-member configuration
+    int* d_mgoff = nullptr;
+    int* d_mbox = nullptr;
+    uint32_t* d_mcell = nullptr;
+    float* d_mmu = nullptr;
+    unsigned* d_mscale = nullptr;
+    size_t nodes = 0;                  // CIC nodes per member channel
     int nblocks = 0;
     BlockPart* parts = nullptr;
     double* energy_log = nullptr;
@@ -109,7 +119,7 @@ struct wd_sim {
         void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], H[0], H[1], Hc[0],
                         Hc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
-                        cic_out, rhs_buf, cic_log};
+                        cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -120,6 +130,8 @@ struct wd_sim {
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
         st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
+        d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
+        d_mscale = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -184,26 +196,27 @@ struct wd_sim {
         const int gb = grain_blocks(n);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
         timed("k_count", [&] {
-            hipLaunchKernelGGL(k_count, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, count, n);
+            hipLaunchKernelGGL(k_count, dim3(gb), dim3(kBlock), 0, stream, dp, ctl, Tv, key, slot, count,
+                               n);
         });
         timed("k_scan", [&] {
             hipLaunchKernelGGL(k_scan, dim3(scan_grid), dim3(kScanThreads), 0, stream, dp, ctl,
                                count, start, status);
         });
         timed("k_scatter", [&] {
-            hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, ctl, key, slot, Tp,
+            hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, dp, key, slot, Tp, Tv,
                                start, tmp_id, n);
         });
         const Carry carry{nin_p, nin_v, nout_p, nout_v};
         timed("k_reorder", [&] {
             if (nmode == 0) {
-                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
                                    tmp_id, Tp, Tv, Sp, Sv, carry, n);
             } else if (nmode == 1) {
-                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
                                    tmp_id, Tp, Tv, Sp, Sv, carry, n);
             } else {
-                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, ctl, key, start,
+                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
                                    tmp_id, Tp, Tv, Sp, Sv, carry, n);
             }
         });
@@ -237,8 +250,8 @@ struct wd_sim {
 
     void finalize(int nparts, int flags) {
         timed("k_finalize", [&] {
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, stream, ctl, parts, nparts,
-                               energy_log, flags);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeBlock), 0, stream, dp, ctl, parts, nparts,
+                               key, Tp, Tv, energy_log, flags);
         });
     }
 
@@ -256,12 +269,12 @@ struct wd_sim {
             finalize(sb, kFinCicScale);
             timed("k_cic_deposit", [&] {
                 hipLaunchKernelGGL(k_cic_deposit, dim3(grain_blocks(n)), dim3(kBlock), 0, stream, dp,
-                                   ctl, Tp, Tv, cic_acc, n);
+                                   Tp, Tv, cic_acc, n);
             });
         }
         timed("k_cic_convert", [&] {
-            hipLaunchKernelGGL(k_cic_convert, dim3(zb), dim3(kBlock), 0, stream, ctl, cic_acc, out,
-                               cic_count, std::max(n, 1));
+            hipLaunchKernelGGL(k_cic_convert, dim3(zb), dim3(kBlock), 0, stream, dp, cic_acc, out,
+                               cic_count);
         });
     }
 
@@ -461,7 +474,8 @@ int allocate(wd_sim* s) {
     }
     WD_TRY(dalloc(&s->key, n)); WD_TRY(dalloc(&s->slot, n)); WD_TRY(dalloc(&s->tmp_id, n));
     const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
-                         (c.collision_grid_size_y + 2) * (c.collision_grid_size_z + 2);
+                         (c.collision_grid_size_y + 2) * (c.collision_grid_size_z + 2) *
+                         static_cast<size_t>(s->members);
     s->ntiles_max = (cells + 1 + kScanTile - 1) / kScanTile;
     s->scan_cells = s->ntiles_max * kScanTile;
     WD_TRY(dalloc(&s->count, s->scan_cells));
@@ -471,9 +485,27 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->mat_d, n));
     s->nblocks = grain_blocks(n);
     WD_TRY(dalloc(&s->parts, s->nblocks));
-    s->cic_count = static_cast<size_t>(4) * c.density_grid_size_x * c.density_grid_size_y *
-                   c.density_grid_size_z;
-    WD_TRY(dalloc(&s->cic_acc, s->cic_count));
+    s->nodes = static_cast<size_t>(c.density_grid_size_x) * c.density_grid_size_y * c.density_grid_size_z;
+    s->cic_count = static_cast<size_t>(4) * s->nodes * static_cast<size_t>(s->members);
+    const int E = s->members;
+    WD_TRY(dalloc(&s->d_mgoff, E + 1));
+    WD_TRY(dalloc(&s->d_mbox, 6 * E));
+    WD_TRY(dalloc(&s->d_mcell, E + 1));
+    WD_TRY(dalloc(&s->d_mmu, E));
+    WD_TRY(dalloc(&s->d_mscale, E));
+    std::vector<float> mus(E);
+    for (int m = 0; m < E; ++m) mus[m] = s->mcfg[m].friction;
+    WD_HIP(hipMemcpy(s->d_mgoff, s->goff.data(), sizeof(int) * (E + 1), hipMemcpyHostToDevice));
+    WD_HIP(hipMemcpy(s->d_mmu, mus.data(), sizeof(float) * E, hipMemcpyHostToDevice));
+    WD_HIP(hipMemset(s->d_mcell, 0, sizeof(uint32_t) * (E + 1)));
+    WD_HIP(hipMemset(s->d_mscale, 0, sizeof(unsigned) * E));
+    s->hp.members = E;
+    s->hp.mgoff = s->d_mgoff;
+    s->hp.mbox = s->d_mbox;
+    s->hp.mcell = s->d_mcell;
+    s->hp.mmu = s->d_mmu;
+    s->hp.mscale = s->d_mscale;
+    WD_TRY(dalloc(&s->cic_acc, s->cic_count));  // int64 fixed point, all members
     WD_TRY(dalloc(&s->cic_out, s->cic_count));
 #undef WD_TRY
     WD_HIP(hipMemset(s->ctl, 0, sizeof(Ctl)));
@@ -489,7 +521,13 @@ int ingest(wd_sim* s, const float* st, const int* mat) {
     if (s->n == 0) return WD_OK;
     WD_HIP(hipStreamSynchronize(s->stream));
     WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
-    if (mat) WD_HIP(hipMemcpy(s->mat_d, mat, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    if (mat) {
+        std::vector<int> code(static_cast<size_t>(s->n));
+        for (int m = 0; m < s->members; ++m) {
+            for (int i = s->goff[m]; i < s->goff[m + 1]; ++i) code[i] = (m << 4) | mat[i];
+        }
+        WD_HIP(hipMemcpy(s->mat_d, code.data(), sizeof(int) * code.size(), hipMemcpyHostToDevice));
+    }
     const int gb = grain_blocks(s->n);
     hipLaunchKernelGGL(k_ingest, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->parts, s->st6,
                        s->mat_d, s->Tp, s->Tv, s->key, s->Hc[0], s->Hc[1], s->n);
@@ -514,23 +552,60 @@ int export_state(wd_sim* s) {
 
 namespace {
 
-int create_with(const wd_config* cfg, int n, const float* st, const int* mat, wd_sim** out) {
+// Fields every ensemble member must share (the grid, the box, the material
+// and the integrator); members may differ in their initial grains and in
+// friction.
+int check_compatible(const wd_config& a, const wd_config& b) {
+    const bool same =
+            a.dt == b.dt && a.collision_grid_size_x == b.collision_grid_size_x &&
+            a.collision_grid_size_y == b.collision_grid_size_y &&
+            a.collision_grid_size_z == b.collision_grid_size_z &&
+            a.particles_per_cell == b.particles_per_cell &&
+            a.density_grid_size_x == b.density_grid_size_x &&
+            a.density_grid_size_y == b.density_grid_size_y &&
+            a.density_grid_size_z == b.density_grid_size_z && a.x_min == b.x_min &&
+            a.x_max == b.x_max && a.y_min == b.y_min && a.y_max == b.y_max && a.z_min == b.z_min &&
+            a.z_max == b.z_max && a.gravity == b.gravity && a.particle_radius == b.particle_radius &&
+            a.max_force == b.max_force && a.floor_z == b.floor_z && a.ceiling_z == b.ceiling_z &&
+            a.wall_x_min == b.wall_x_min && a.wall_x_max == b.wall_x_max &&
+            a.wall_y_min == b.wall_y_min && a.wall_y_max == b.wall_y_max &&
+            a.restitution == b.restitution && a.tangential_model == b.tangential_model &&
+            a.tangential_stiffness_ratio == b.tangential_stiffness_ratio &&
+            a.normal_stiffness == b.normal_stiffness && a.num_masses == b.num_masses &&
+            std::memcmp(a.masses, b.masses, sizeof(float) * a.num_masses) == 0 &&
+            a.time_integrator == b.time_integrator && a.picard_iterations == b.picard_iterations;
+    return same ? WD_OK : fail(WD_ERR_ARG, "ensemble members differ in more than initial state and friction");
+}
+
+int create_with(const wd_config* cfgs, int members, const std::vector<int>& goff, const float* st,
+                const int* mat, wd_sim** out) {
     *out = nullptr;
-    int rc = validate(*cfg);
-    if (rc != WD_OK) return rc;
-    for (int i = 0; i < n; ++i) {
-        if (mat[i] < 0 || mat[i] >= cfg->num_masses) return fail(WD_ERR_ARG, "material id out of range");
+    for (int m = 0; m < members; ++m) {
+        int rc = validate(cfgs[m]);
+        if (rc == WD_OK && m > 0) rc = check_compatible(cfgs[0], cfgs[m]);
+        if (rc != WD_OK) return rc;
+        if (members > 1 && goff[m + 1] <= goff[m]) return fail(WD_ERR_ARG, "ensemble member without grains");
+    }
+    const int n = goff[members];
+    if (members > (1 << 26)) return fail(WD_ERR_ARG, "too many ensemble members");
+    for (int m = 0; m < members; ++m) {
+        for (int i = goff[m]; i < goff[m + 1]; ++i) {
+            if (mat[i] < 0 || mat[i] >= cfgs[m].num_masses) return fail(WD_ERR_ARG, "material id out of range");
+        }
     }
     wd_sim* s = new wd_sim();
-    s->cfg = *cfg;
+    s->cfg = cfgs[0];
     s->n = n;
-    s->timing = cfg->kernel_timing != 0;
-    fill_params(*cfg, n, s->hp);
+    s->members = members;
+    s->goff = goff;
+    s->mcfg.assign(cfgs, cfgs + members);
+    s->timing = cfgs[0].kernel_timing != 0;
+    fill_params(cfgs[0], n, s->hp);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(WD_ERR_HIP, "wd_create: hipStreamCreate failed");
     }
-    rc = allocate(s);
+    int rc = allocate(s);
     if (rc == WD_OK) rc = ingest(s, st, mat);
     if (rc != WD_OK) {
         delete s;
@@ -552,7 +627,7 @@ extern "C" int wd_create(const wd_config* cfg, wd_sim** out) {
     std::vector<int> mat(static_cast<size_t>(std::max(n, 1)));
     rc = wd_initial_state(cfg, st.data(), mat.data(), n);
     if (rc != WD_OK) return rc;
-    return create_with(cfg, n, st.data(), mat.data(), out);
+    return create_with(cfg, 1, {0, n}, st.data(), mat.data(), out);
 }
 
 extern "C" int wd_create_from_state(const wd_config* cfg, int n, const float* state,
@@ -560,7 +635,7 @@ extern "C" int wd_create_from_state(const wd_config* cfg, int n, const float* st
     if (!cfg || !out || n < 0 || (n > 0 && (!state || !material))) {
         return fail(WD_ERR_ARG, "wd_create_from_state: bad argument");
     }
-    return create_with(cfg, n, state, material, out);
+    return create_with(cfg, 1, {0, n}, state, material, out);
 }
 
 extern "C" int wd_destroy(wd_sim* s) {
@@ -599,6 +674,7 @@ extern "C" int wd_get_material(wd_sim* s, int* out) {
     int rc = export_state(s);
     if (rc != WD_OK) return rc;
     if (s->n) WD_HIP(hipMemcpy(out, s->mat_d, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->n; ++i) out[i] &= (kMaxMat - 1);
     return WD_OK;
 }
 
@@ -610,7 +686,11 @@ extern "C" int wd_set_material(wd_sim* s, const int* in) {
     }
     if (s->n == 0) return WD_OK;
     WD_HIP(hipStreamSynchronize(s->stream));
-    WD_HIP(hipMemcpy(s->mat_d, in, sizeof(int) * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    std::vector<int> code(static_cast<size_t>(s->n));
+    for (int m = 0; m < s->members; ++m) {
+        for (int i = s->goff[m]; i < s->goff[m + 1]; ++i) code[i] = (m << 4) | in[i];
+    }
+    WD_HIP(hipMemcpy(s->mat_d, code.data(), sizeof(int) * code.size(), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_set_material, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
                        s->Tp, s->Tv, s->mat_d, s->n);
     WD_HIP(hipGetLastError());
@@ -659,7 +739,7 @@ extern "C" int wd_density_grid(wd_sim* s, float* out) {
     WD_HIP(hipGetLastError());
     int rc = s->sync();
     if (rc != WD_OK) return rc;
-    WD_HIP(hipMemcpy(out, s->cic_out, sizeof(float) * s->cic_count, hipMemcpyDeviceToHost));
+    WD_HIP(hipMemcpy(out, s->cic_out, sizeof(float) * 4 * s->nodes, hipMemcpyDeviceToHost));
     return WD_OK;
 }
 
@@ -806,7 +886,7 @@ extern "C" int wd_neighbor_rows(wd_sim* s, int* out, size_t capacity) {
     int rc = dalloc(&rows, need);
     if (rc == WD_OK) {
         hipLaunchKernelGGL(k_neighbor_rows, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
-                           s->dp, s->ctl, s->Sp, s->start, rows, s->n);
+                           s->dp, s->Sv, s->Sp, s->start, rows, s->n);
         rc = s->sync();
     }
     if (rc == WD_OK && hipMemcpy(out, rows, sizeof(int) * need, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -943,4 +1023,61 @@ extern "C" int wd_get_history(wd_sim* s, int* partner, float* xi, int* count) {
     if (dx) (void)hipFree(dx);
     if (dc) (void)hipFree(dc);
     return rc;
+}
+
+extern "C" int wd_batch_create(const wd_config* cfgs, int members, wd_sim** out) {
+    if (!cfgs || !out || members < 1) return fail(WD_ERR_ARG, "wd_batch_create: bad argument");
+    *out = nullptr;
+    std::vector<int> goff(static_cast<size_t>(members) + 1, 0);
+    for (int m = 0; m < members; ++m) {
+        int k = 0;
+        const int rc = wd_config_grain_count(&cfgs[m], &k);
+        if (rc != WD_OK) return rc;
+        goff[m + 1] = goff[m] + k;
+    }
+    const int n = goff[members];
+    std::vector<float> st(static_cast<size_t>(6) * std::max(n, 1));
+    std::vector<int> mat(static_cast<size_t>(std::max(n, 1)));
+    for (int m = 0; m < members; ++m) {
+        const int rc = wd_initial_state(&cfgs[m], st.data() + 6 * static_cast<size_t>(goff[m]),
+                                        mat.data() + goff[m], goff[m + 1] - goff[m]);
+        if (rc != WD_OK) return rc;
+    }
+    return create_with(cfgs, members, goff, st.data(), mat.data(), out);
+}
+
+extern "C" int wd_batch_members(const wd_sim* s, int* members) {
+    WD_CHECK_SIM(s);
+    if (!members) return fail(WD_ERR_ARG, "wd_batch_members: null output");
+    *members = s->members;
+    return WD_OK;
+}
+
+extern "C" int wd_batch_member_range(const wd_sim* s, int member, int* first, int* count) {
+    WD_CHECK_SIM(s);
+    if (!first || !count || member < 0 || member >= s->members) {
+        return fail(WD_ERR_ARG, "wd_batch_member_range: bad argument");
+    }
+    *first = s->goff[member];
+    *count = s->goff[member + 1] - s->goff[member];
+    return WD_OK;
+}
+
+extern "C" int wd_batch_density_grid(wd_sim* s, int member, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out || member < -1 || member >= s->members) return fail(WD_ERR_ARG, "wd_batch_density_grid: bad argument");
+    hipEvent_t a = s->stage_begin();
+    s->enqueue_cic(s->cic_out);
+    s->stage_end(WD_STAGE_DENSITY_GRID, a);
+    WD_HIP(hipGetLastError());
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    const size_t per = 4 * s->nodes;
+    if (member < 0) {
+        WD_HIP(hipMemcpy(out, s->cic_out, sizeof(float) * s->cic_count, hipMemcpyDeviceToHost));
+    } else {
+        WD_HIP(hipMemcpy(out, s->cic_out + per * static_cast<size_t>(member), sizeof(float) * per,
+                         hipMemcpyDeviceToHost));
+    }
+    return WD_OK;
 }

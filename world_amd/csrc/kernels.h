@@ -13,6 +13,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
+constexpr int kFinalizeBlock = 1024;  // one block; one wave per ensemble member at a time
 constexpr int kContactList = 16;     // touching candidates buffered per grain in LDS
 
 // Small device-side control block: counters, the occupied-cell bounding box
@@ -21,9 +22,7 @@ constexpr int kContactList = 16;     // touching candidates buffered per grain i
 struct Ctl {
     unsigned tile_counter;     // scan tile dispenser; zeroed by k_count
     unsigned epoch;            // scan look-back tag; bumped by k_count
-    int bbox_cur[6];           // occupied cells (min xyz, max xyz) of the current state
     unsigned accel_last;       // float bits, max |a| of the last completed force pass
-    unsigned cic_scale_bits;   // float bits, max per-grain deposit magnitude
     unsigned long long overflow_lifetime;  // over-capacity cells, summed over passes
     int error;                 // nonzero: a bounded wait gave up
     int energy_count;          // entries written to the energy log
@@ -32,9 +31,15 @@ struct Ctl {
     double energy_last;        // last standalone energy
 };
 
-// Per-block partial results of a grain pass, reduced by k_finalize.
+// Per-block partial results of a grain pass, reduced by k_finalize.  A block
+// covers consecutive grains, so it holds a run of members m0..m1; the boxes
+// of its first and last member are reduced here (members strictly between
+// them lie wholly inside the block and are reduced by k_finalize from the
+// cell keys).
 struct BlockPart {
-    int box[6];       // min cx, cy, cz / max cx, cy, cz of the block's grains
+    int m0, m1;       // first and last member of the block's grains (-1: none)
+    int box0[6];      // min cx, cy, cz / max cx, cy, cz of member m0's grains
+    int box1[6];      // the same for member m1 (when m1 != m0)
     unsigned accel;   // float bits of the block's max |a| (or deposit magnitude)
     unsigned pad;
     double energy;    // deterministic block sum of per-grain energies
@@ -42,11 +47,11 @@ struct BlockPart {
 
 // What k_finalize publishes.
 enum FinalizeFlags {
-    kFinBox = 1,        // bbox_cur <- reduced box
+    kFinBox = 1,        // member boxes and cell offsets <- reduced boxes
     kFinAccel = 2,      // accel_last <- reduced max
     kFinEnergyLog = 4,  // append the energy sum to the log
     kFinEnergyLast = 8, // energy_last <- energy sum
-    kFinCicScale = 16,  // cic_scale_bits <- reduced max
+    kFinCicScale = 16,  // per-member CIC magnitudes <- reduced maxima
     kFinFlipHistory = 32  // the step is complete: its springs become current
 };
 

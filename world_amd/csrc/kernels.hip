@@ -72,33 +72,66 @@ __device__ double block_sum(double v) {
     return total;
 }
 
-// Reduces the block's box, max and energy and stores them, with plain
-// stores, in parts[slot] (thread 0).
-__device__ void store_block_part(BlockPart* parts, int slot, const int lo[3], const int hi[3],
-                                 unsigned accel, double energy, bool with_energy) {
-    __shared__ int s_box[16][7];
+// Reduces, over the block, the boxes of its first and last member (lanes
+// with member < 0 contribute nothing), the max of `accel` and the sum of
+// `energy`, and stores them with plain stores in parts[slot] (thread 0).
+__device__ void store_block_part(BlockPart* parts, int slot, int member, const int lo[3],
+                                 const int hi[3], unsigned accel, double energy, bool with_energy) {
+    __shared__ int s_m[16][2];
+    __shared__ int s_box[16][13];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
-    int v[7] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
-                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]),
-                static_cast<int>(wave_max_u(accel))};
+    // block's member range
+    int mlo = wave_min(member < 0 ? kEmptyMin : member);
+    int mhi = wave_max(member);
     if (lane == 0) {
-        #pragma unroll
-        for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
+        s_m[wave][0] = mlo;
+        s_m[wave][1] = mhi;
+    }
+    __syncthreads();
+    mlo = kEmptyMin;
+    mhi = -1;
+    for (int w = 0; w < nw; ++w) {
+        mlo = min(mlo, s_m[w][0]);
+        mhi = max(mhi, s_m[w][1]);
+    }
+    const bool in0 = member >= 0 && member == mlo;
+    const bool in1 = member >= 0 && member == mhi && mhi != mlo;
+    int v[13];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = wave_min(in0 ? lo[k] : kEmptyMin);
+        v[3 + k] = wave_max(in0 ? hi[k] : kEmptyMax);
+        v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
+        v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+    }
+    v[12] = static_cast<int>(wave_max_u(accel));
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 13; ++k) s_box[wave][k] = v[k];
     }
     const double e = with_energy ? block_sum(energy) : 0.0;  // contains __syncthreads
     if (!with_energy) __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned am = static_cast<unsigned>(s_box[0][6]);
+        unsigned am = static_cast<unsigned>(s_box[0][12]);
         for (int w = 1; w < nw; ++w) {
-            #pragma unroll
-            for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
-            for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
-            am = max(am, static_cast<unsigned>(s_box[w][6]));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k] = min(v[k], s_box[w][k]);
+                v[3 + k] = max(v[3 + k], s_box[w][3 + k]);
+                v[6 + k] = min(v[6 + k], s_box[w][6 + k]);
+                v[9 + k] = max(v[9 + k], s_box[w][9 + k]);
+            }
+            am = max(am, static_cast<unsigned>(s_box[w][12]));
         }
         BlockPart bp;
-        #pragma unroll
-        for (int k = 0; k < 6; ++k) bp.box[k] = v[k];
+        bp.m0 = mhi < 0 ? -1 : mlo;
+        bp.m1 = mhi;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            bp.box0[k] = v[k];
+            bp.box1[k] = v[6 + k];
+        }
         bp.accel = am;
         bp.pad = 0;
         bp.energy = e;
@@ -121,11 +154,13 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
     const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    int member = -1;
     if (i < n) {
+        member = mat[i] >> 4;
         const float* s = st6 + 6 * static_cast<size_t>(i);
         const float x = s[0], y = s[1], z = s[2];
         Tp[i] = make_float4(x, y, z, __int_as_float(i));
-        Tv[i] = make_float4(s[3], s[4], s[5], __int_as_float(mat[i]));
+        Tv[i] = make_float4(s[3], s[4], s[5], __int_as_float(mat[i]));  // member << 4 | material
         if (hist_cnt0) {  // an ingested state starts with no contact history
             hist_cnt0[i] = 0;
             hist_cnt1[i] = 0;
@@ -137,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
         lo[1] = hi[1] = cy;
         lo[2] = hi[2] = cz;
     }
-    store_block_part(parts, logical_block(), lo, hi, 0u, 0.0, false);
+    store_block_part(parts, logical_block(), member, lo, hi, 0u, 0.0, false);
 }
 
 // Original-order material ids pushed into the working layout.
@@ -166,24 +201,27 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4* __restrict__ Tp
 
 // ---------------------------------------------------------------------------
 // Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
-__global__ __launch_bounds__(kBlock) void k_count(Ctl* ctl, const uint32_t* __restrict__ key,
+__global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp, Ctl* ctl,
+                                                  const float4* __restrict__ Tv,
+                                                  const uint32_t* __restrict__ key,
                                                   uint32_t* __restrict__ slot,
                                                   uint32_t* __restrict__ count, int n) {
+    const Params& P = *Pp;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->tile_counter = 0;
         ctl->epoch = ctl->epoch + 1;
     }
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i < n) {
-        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
         int cx, cy, cz;
         unpack_cell(key[i], cx, cy, cz);
-        slot[i] = atomicAdd(&count[g.index(cx, cy, cz)], 1u);
+        slot[i] = atomicAdd(&count[mc.index(cx, cy, cz)], 1u);
     }
 }
 
-// Pass 2: start[c] = sum of count[0..c) over the padded sub-grid, written up
-// to and including index `cells`; counts are zeroed for the next pass and
+// Pass 2: start[c] = sum of count[0..c) over the concatenated member
+// sub-grids, written up to and including index `cells`; counts are zeroed for the next pass and
 // over-capacity cells are added to the lifetime overflow total.  Tiles are
 // handed out by an atomic counter, so a tile only waits on tiles already
 // owned by running blocks (forward progress without co-residency limits).
@@ -191,8 +229,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
                                                        uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ start,
                                                        unsigned long long* __restrict__ status) {
-    const SubGrid g = make_subgrid(ctl->bbox_cur);
-    const int cells = g.cells();
+    const int cells = static_cast<int>(Pp->mcell[Pp->members]);
     const int ntiles = (cells + 1 + kScanTile - 1) / kScanTile;
     const unsigned long long tag = static_cast<unsigned long long>(ctl->epoch & 0x3FFFFFFFu) << 34;
     const uint32_t kcap = static_cast<uint32_t>(Pp->kcap);
@@ -291,18 +328,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
 
 // Pass 3: original ids into cell-sorted slots (order inside a cell is still
 // arbitrary here).
-__global__ __launch_bounds__(kBlock) void k_scatter(const Ctl* __restrict__ ctl,
+__global__ __launch_bounds__(kBlock) void k_scatter(const Params* __restrict__ Pp,
                                                     const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ slot,
                                                     const float4* __restrict__ Tp,
+                                                    const float4* __restrict__ Tv,
                                                     const uint32_t* __restrict__ start,
                                                     int* __restrict__ tmp_id, int n) {
+    const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i < n) {
-        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
         int cx, cy, cz;
         unpack_cell(key[i], cx, cy, cz);
-        tmp_id[start[g.index(cx, cy, cz)] + slot[i]] = __float_as_int(Tp[i].w);
+        tmp_id[start[mc.index(cx, cy, cz)] + slot[i]] = __float_as_int(Tp[i].w);
     }
 }
 
@@ -311,19 +350,20 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Ctl* __restrict__ ctl,
 // NMODE 2 carries the Picard base along the same permutation.
 template <int NMODE>
 __global__ __launch_bounds__(kBlock) void k_reorder(
-        const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
+        const Params* __restrict__ Pp, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
         const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
         float4* __restrict__ Sv, Carry carry, int n) {
+    const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const SubGrid g = make_subgrid(ctl->bbox_cur);
-    int cx, cy, cz;
-    unpack_cell(key[i], cx, cy, cz);
-    const int c = g.index(cx, cy, cz);
-    const uint32_t b = start[c], e = start[c + 1];
     const float4 p = Tp[i];
     const float4 v = Tv[i];
+    const MemberCells mc = member_cells(P, member_of(P, v.w));
+    int cx, cy, cz;
+    unpack_cell(key[i], cx, cy, cz);
+    const uint32_t c = mc.index(cx, cy, cz);
+    const uint32_t b = start[c], e = start[c + 1];
     const int id = __float_as_int(p.w);
     uint32_t rank = 0;
     if (e - b > 1) {
@@ -365,13 +405,16 @@ __global__ __launch_bounds__(kBlock) void k_force(
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
     unsigned accel_bits = 0;
     float e_grain = 0.f;
+    int lane_member = -1;
     if (p < n) {
         const float4 pi = Sp[p];
         const float4 vi4 = Sv[p];
         const float x[3] = {pi.x, pi.y, pi.z};
         const float v[3] = {vi4.x, vi4.y, vi4.z};
         const int id = __float_as_int(pi.w);
-        const int mat = __float_as_int(vi4.w);
+        const int mat = grain_mat(vi4.w);
+        const int member = member_of(P, vi4.w);
+        const float mu = P.members == 1 ? P.mu : P.mmu[member];
         const float m = P.mass[mat];
         float f[3] = {0.f, 0.f, 0.f};
         if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
@@ -444,19 +487,19 @@ __global__ __launch_bounds__(kBlock) void k_force(
                             s2 += xi[2] * xi[2];
                             e_grain += 0.5f * P.kt * s2;
                         }
-                        spring_friction(P, f, nrm, v, vn, fw, c_wall, xi);
+                        spring_friction(P, mu, f, nrm, v, vn, fw, c_wall, xi);
                         h_put(wall_partner(a, side), xi);
                     } else {
                         float vt[3];
                         #pragma unroll
                         for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
-                        viscous_friction(f, vt, P.mu * fw, c_wall);
+                        viscous_friction(f, vt, mu * fw, c_wall);
                     }
                 }
             }
         }
 
-        const SubGrid g = make_subgrid(ctl->bbox_cur);
+        const MemberCells mc = member_cells(P, member);
         int cx, cy, cz;
         grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
         const uint32_t kcap = static_cast<uint32_t>(P.kcap);
@@ -464,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
         // cell's start), one 16-byte load.
         auto column = [&](int c) __attribute__((always_inline)) {
             uint4 q4;
-            __builtin_memcpy(&q4, start + g.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+            __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
             return q4;
         };
         // Candidates of segment z (0..2) of a column: cell z's first kcap grains.
@@ -488,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 const int jd = __float_as_int(pj.w);
                 if (ENERGY && jd > id) e_grain += contact_energy(P, 2.f * P.radius - norm);
                 if (!HIST) {
-                    pair_contact(P, v, mat, d0, d1, d2, norm, Sv, q, f);
+                    pair_contact(P, mu, v, mat, d0, d1, d2, norm, Sv, q, f);
                     continue;
                 }
                 const float dist = norm - P.radius - P.radius;
@@ -499,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 f[2] += fp * nrm[2];
                 if (dist < 0.f) {
                     const float4 vj = Sv[q];
-                    const float c = P.c_pair[mat * kMaxMat + __float_as_int(vj.w)];
+                    const float c = P.c_pair[mat * kMaxMat + grain_mat(vj.w)];
                     const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
                     float vn = 0.f;
                     vn += vr[0] * nrm[0];
@@ -516,7 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                         s2 += xi[2] * xi[2];
                         e_grain += 0.5f * P.kt * s2;
                     }
-                    spring_friction(P, f, nrm, vr, vn, fp, c, xi);
+                    spring_friction(P, mu, f, nrm, vr, vn, fp, c, xi);
                     h_put(jd, xi);
                 }
             }
@@ -596,6 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
             }
             Tp[p] = make_float4(xn[0], xn[1], xn[2], pi.w);
             Tv[p] = make_float4(vn[0], vn[1], vn[2], vi4.w);
+            lane_member = member;
             int nx, ny, nz;
             grain_cell(P, xn[0], xn[1], xn[2], nx, ny, nz);
             key_out[p] = pack_cell(nx, ny, nz);
@@ -611,52 +655,119 @@ __global__ __launch_bounds__(kBlock) void k_force(
         }
     }
     if (MODE != kModeRHS || ENERGY) {
-        store_block_part(parts, lb, lo, hi, accel_bits, static_cast<double>(e_grain), ENERGY);
+        store_block_part(parts, lb, lane_member, lo, hi, accel_bits, static_cast<double>(e_grain),
+                         ENERGY);
     }
 }
 
-// One block: reduces the per-block partials of the pass that just ran, in a
-// fixed order, and publishes what `flags` asks for.
-__global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* __restrict__ parts,
-                                                     int nparts, double* __restrict__ energy_log,
-                                                     int flags) {
-    __shared__ int s_box[16][7];
+// Largest CIC deposit magnitude of one grain as float bits (0 for a
+// massless grain, +inf bits for a non-finite one).
+__device__ __forceinline__ unsigned cic_grain_bits(const Params& P, float4 p, float4 v) {
+    const float m = P.mass[grain_mat(v.w)];
+    float mag = fabsf(m);
+    mag = fmaxf(mag, fabsf(m * v.x));
+    mag = fmaxf(mag, fabsf(m * v.y));
+    mag = fmaxf(mag, fabsf(m * v.z));
+    const bool finite = isfinite(m * v.x) && isfinite(m * v.y) && isfinite(m * v.z) &&
+                        isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(m);
+    return (m == 0.f) ? 0u : (finite ? __float_as_uint(mag) : 0x7F800000u);
+}
+
+// Reduces, with one wave, the boxes that the block partials hold for member
+// m over its grain range [mgoff[m], mgoff[m+1]) into v (valid in every lane:
+// min lo[3], max hi[3]).  A partial block whose first and last members are
+// both other than m (it spans three or more members) is folded grain by
+// grain with `grain(i, lo, hi)`.
+template <class Grain>
+__device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict__ parts, int m,
+                                   int v[6], Grain grain) {
+    const int lane = threadIdx.x & 63;
+    const int bs = kBlock;  // grains per partial
+    const int g0 = P.mgoff[m], g1 = P.mgoff[m + 1];
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    if (g1 > g0) {
+        for (int b = g0 / bs + lane; b <= (g1 - 1) / bs; b += 64) {
+            const BlockPart& bp = parts[b];
+            if (bp.m0 == m || bp.m1 == m) {
+                const int* bx = bp.m0 == m ? bp.box0 : bp.box1;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = min(lo[k], bx[k]);
+                    hi[k] = max(hi[k], bx[3 + k]);
+                }
+            } else {
+                for (int i = max(g0, b * bs); i < min(g1, (b + 1) * bs); ++i) grain(i, lo, hi);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = wave_min(lo[k]);
+        v[3 + k] = wave_max(hi[k]);
+    }
+}
+
+__global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __restrict__ Pp, Ctl* ctl,
+                                                     const BlockPart* __restrict__ parts,
+                                                     int nparts, const uint32_t* __restrict__ key,
+                                                     const float4* __restrict__ Tp,
+                                                     const float4* __restrict__ Tv,
+                                                     double* __restrict__ energy_log, int flags) {
+    const Params& P = *Pp;
+    __shared__ int s_box[16][1];
     unsigned am = 0u;
     double e = 0.0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        const BlockPart& bp = parts[b];
-        #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = min(lo[k], bp.box[k]);
-            hi[k] = max(hi[k], bp.box[k + 3]);
-        }
-        am = max(am, bp.accel);
-        e += bp.energy;
+        am = max(am, parts[b].accel);
+        e += parts[b].energy;
     }
     const double total = block_sum(e);
+    const unsigned amax_w = wave_max_u(am);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int v[7] = {wave_min(lo[0]), wave_min(lo[1]), wave_min(lo[2]),
-                wave_max(hi[0]), wave_max(hi[1]), wave_max(hi[2]), static_cast<int>(wave_max_u(am))};
-    if (lane == 0) {
-        #pragma unroll
-        for (int k = 0; k < 7; ++k) s_box[wave][k] = v[k];
-    }
+    const int nw = blockDim.x >> 6;
+    if (lane == 0) s_box[wave][0] = static_cast<int>(amax_w);
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    unsigned amax = static_cast<unsigned>(s_box[0][6]);
-    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
-        #pragma unroll
-        for (int k = 0; k < 3; ++k) v[k] = min(v[k], s_box[w][k]);
-        for (int k = 3; k < 6; ++k) v[k] = max(v[k], s_box[w][k]);
-        amax = max(amax, static_cast<unsigned>(s_box[w][6]));
+    unsigned amax = 0u;
+    for (int w = 0; w < nw; ++w) amax = max(amax, static_cast<unsigned>(s_box[w][0]));
+    __syncthreads();
+    // One wave per member.
+    if (flags & (kFinBox | kFinCicScale)) {
+        for (int m = wave; m < P.members; m += nw) {
+            int v[6];
+            if (flags & kFinBox) {
+                wave_member_reduce(P, parts, m, v, [&](int i, int lo[3], int hi[3]) {
+                    int c3[3];
+                    unpack_cell(key[i], c3[0], c3[1], c3[2]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = min(lo[k], c3[k]);
+                        hi[k] = max(hi[k], c3[k]);
+                    }
+                });
+                if (lane < 6) P.mbox[6 * m + lane] = v[lane];
+            }
+            if (flags & kFinCicScale) {
+                wave_member_reduce(P, parts, m, v, [&](int i, int lo[3], int hi[3]) {
+                    hi[0] = max(hi[0], static_cast<int>(cic_grain_bits(P, Tp[i], Tv[i])));
+                });
+                if (lane == 0) P.mscale[m] = v[3] < 0 ? 0u : static_cast<unsigned>(v[3]);
+            }
+        }
     }
     if (flags & kFinBox) {
-        #pragma unroll
-        for (int k = 0; k < 6; ++k) ctl->bbox_cur[k] = v[k];
+        __syncthreads();  // every member's box is stored
+        if (threadIdx.x == 0) {
+            uint32_t off = 0;
+            for (int m = 0; m < P.members; ++m) {
+                P.mcell[m] = off;
+                const int* bx = P.mbox + 6 * m;
+                off += static_cast<uint32_t>((bx[3] - bx[0] + 3) * (bx[4] - bx[1] + 3) * (bx[5] - bx[2] + 3));
+            }
+            P.mcell[P.members] = off;
+        }
     }
+    if (threadIdx.x != 0) return;
     if (flags & kFinAccel) ctl->accel_last = amax;
-    if (flags & kFinCicScale) ctl->cic_scale_bits = amax;
     if (flags & kFinEnergyLast) ctl->energy_last = total;
     if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
     if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
@@ -666,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Ctl* ctl, const BlockPart* 
 }
 
 #define WD_INST_REORDER(M)                                                                   \
-    template __global__ void k_reorder<M>(const Ctl*, const uint32_t*, const uint32_t*,          \
+    template __global__ void k_reorder<M>(const Params*, const uint32_t*, const uint32_t*,       \
                                           const int*, const float4*, const float4*, float4*,     \
                                           float4*, Carry, int);
 WD_INST_REORDER(0)
@@ -708,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_export_order(const Params* __restric
 }
 
 __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restrict__ Pp,
-                                                          const Ctl* __restrict__ ctl,
+                                                          const float4* __restrict__ Sv,
                                                           const float4* __restrict__ Sp,
                                                           const uint32_t* __restrict__ start,
                                                           int* __restrict__ out, int n) {
@@ -718,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
     const float4 pi = Sp[p];
     const size_t row_len = static_cast<size_t>(27) * P.kcap;
     int* row = out + static_cast<size_t>(__float_as_int(pi.w)) * row_len;
-    const SubGrid g = make_subgrid(ctl->bbox_cur);
+    const MemberCells g = member_cells(P, P.members == 1 ? 0 : grain_member(Sv[p].w));
     int cx, cy, cz;
     grain_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
@@ -726,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
     for (int dx = -1; dx <= 1; ++dx) {
         for (int dy = -1; dy <= 1; ++dy) {
             for (int dz = -1; dz <= 1; ++dz) {
-                const int c = g.index(cx + dx, cy + dy, cz + dz);
+                const uint32_t c = g.index(cx + dx, cy + dy, cz + dz);
                 const uint32_t b = start[c];
                 const uint32_t e = min(start[c + 1], b + kcap);
                 for (uint32_t q = b; q < e; ++q) {
@@ -786,20 +897,14 @@ __global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__
                                                       const float4* __restrict__ Tv, int n) {
     const Params& P = *Pp;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned bits = 0u;
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    int member = -1;
     if (i < n) {
-        const float4 p = Tp[i], v = Tv[i];
-        const float m = P.mass[__float_as_int(v.w)];
-        float mag = fabsf(m);
-        mag = fmaxf(mag, fabsf(m * v.x));
-        mag = fmaxf(mag, fabsf(m * v.y));
-        mag = fmaxf(mag, fabsf(m * v.z));
-        const bool finite = isfinite(m * v.x) && isfinite(m * v.y) && isfinite(m * v.z) &&
-                            isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(m);
-        bits = (m == 0.f) ? 0u : (finite ? __float_as_uint(mag) : 0x7F800000u);
+        const float4 v = Tv[i];
+        member = member_of(P, v.w);
+        hi[0] = static_cast<int>(cic_grain_bits(P, Tp[i], v));
     }
-    const int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    store_block_part(parts, blockIdx.x, lo, hi, bits, 0.0, false);
+    store_block_part(parts, blockIdx.x, member, lo, hi, 0u, 0.0, false);
 }
 
 // Fixed-point exponent S for n grains of largest magnitude M: n*M*2^S < 2^62.
@@ -827,27 +932,29 @@ __device__ __forceinline__ void axis_weights(float p, float lo, float hi, int si
 }
 
 __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict__ Pp,
-                                                        const Ctl* __restrict__ ctl,
                                                         const float4* __restrict__ Tp,
                                                         const float4* __restrict__ Tv,
                                                         unsigned long long* __restrict__ acc,
                                                         int n) {
     const Params& P = *Pp;
-    const unsigned sb = ctl->cic_scale_bits;
-    if (sb >= 0x7F800000u) return;  // non-finite input: the grid is reported as NaN
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double scale = ldexp(1.0, cic_shift(__uint_as_float(sb), n));
     const float4 p = Tp[i], v = Tv[i];
-    const float m = P.mass[__float_as_int(v.w)];
+    const int member = member_of(P, v.w);
+    const unsigned sb = P.mscale[member];
+    if (sb >= 0x7F800000u) return;  // non-finite input: the member's grid is reported as NaN
+    const float m = P.mass[grain_mat(v.w)];
     if (m == 0.f) return;
+    const double scale =
+            ldexp(1.0, cic_shift(__uint_as_float(sb), P.mgoff[member + 1] - P.mgoff[member]));
+    const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
+    acc += static_cast<size_t>(member) * 4 * nodes;
     int ix[2], iy[2], iz[2];
     float wx[2], wy[2], wz[2];
     axis_weights(p.x, P.lo[0], P.hi[0], P.dgx, ix[0], ix[1], wx[0], wx[1]);
     axis_weights(p.y, P.lo[1], P.hi[1], P.dgy, iy[0], iy[1], wy[0], wy[1]);
     axis_weights(p.z, P.lo[2], P.hi[2], P.dgz, iz[0], iz[1], wz[0], wz[1]);
     const float mom[3] = {m * v.x, m * v.y, m * v.z};
-    const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
     #pragma unroll
     for (int a = 0; a < 2; ++a) {
         #pragma unroll
@@ -870,15 +977,18 @@ __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cic_convert(const Ctl* __restrict__ ctl,
+__global__ __launch_bounds__(kBlock) void k_cic_convert(const Params* __restrict__ Pp,
                                                         const unsigned long long* __restrict__ acc,
-                                                        float* __restrict__ out, size_t count,
-                                                        int n) {
-    const unsigned sb = ctl->cic_scale_bits;
-    const bool bad = sb >= 0x7F800000u;
-    const double inv = bad ? 0.0 : ldexp(1.0, -cic_shift(__uint_as_float(sb), n));
+                                                        float* __restrict__ out, size_t count) {
+    const Params& P = *Pp;
+    const size_t per = static_cast<size_t>(4) * P.dgx * P.dgy * P.dgz;
     for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
          k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int member = static_cast<int>(k / per);
+        const unsigned sb = P.mscale[member];
+        const bool bad = sb >= 0x7F800000u;
+        const int nm = P.mgoff[member + 1] - P.mgoff[member];
+        const double inv = bad ? 0.0 : ldexp(1.0, -cic_shift(__uint_as_float(sb), max(nm, 1)));
         out[k] = bad ? __int_as_float(0x7FC00000)
                      : static_cast<float>(static_cast<double>(static_cast<long long>(acc[k])) * inv);
     }

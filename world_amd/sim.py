@@ -48,6 +48,9 @@ class ParticleDynamics:
         self.config = (config or _capi.default_config()).copy()
         self._h = C.c_void_p()
         check(lib().wd_create(C.byref(self.config), C.byref(self._h)))
+        self._setup()
+
+    def _setup(self) -> None:
         n = C.c_int(0)
         check(lib().wd_num_grains(self._h, C.byref(n)))
         self.n = n.value
@@ -233,3 +236,51 @@ class ParticleDynamics:
         out = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
         check(lib().wd_rhs(self._h, ptr(out)))
         return out[: K_STATE_STRIDE * self.n].reshape(self.n, K_STATE_STRIDE)
+
+
+class Ensemble(ParticleDynamics):
+    """E members advanced together by one handle (wd_batch_create).
+
+    The reference runs ensemble members one after another in separate
+    ParticleDynamics objects and requires equal grain counts
+    (src/chaos.cpp:163-171); here members may differ in grain count,
+    initialization and friction, and each evolves exactly as it would alone.
+    The inherited arrays (host_state, host_material, ...) hold the members'
+    grains back to back; member_* accessors slice one member out."""
+
+    def __init__(self, configs):
+        configs = [c.copy() for c in configs]
+        if not configs:
+            raise ValueError("an ensemble needs at least one member")
+        arr = (WdConfig * len(configs))(*configs)
+        self.configs = configs
+        self.config = configs[0].copy()
+        self._h = C.c_void_p()
+        check(lib().wd_batch_create(arr, len(configs), C.byref(self._h)))
+        self._setup()
+        self.members = len(configs)
+        self.ranges = []
+        for m in range(self.members):
+            first, count = C.c_int(0), C.c_int(0)
+            check(lib().wd_batch_member_range(self._h, m, C.byref(first), C.byref(count)))
+            self.ranges.append((first.value, count.value))
+
+    def member_state(self, m: int) -> np.ndarray:
+        """State of member m (its grains in their own original order)."""
+        first, count = self.ranges[m]
+        return self.state()[first:first + count]
+
+    def member_states(self) -> list[np.ndarray]:
+        st = self.state()
+        return [st[f:f + c] for f, c in self.ranges]
+
+    def member_density_grid(self, m: int) -> np.ndarray:
+        out = np.zeros(K_CHANNELS * self.density_grid_nodes(), np.float32)
+        check(lib().wd_batch_density_grid(self._h, int(m), ptr(out)))
+        return out
+
+    def density_grids(self) -> np.ndarray:
+        """All members' CIC grids, [members, 4 * nodes]."""
+        out = np.zeros(self.members * K_CHANNELS * self.density_grid_nodes(), np.float32)
+        check(lib().wd_batch_density_grid(self._h, -1, ptr(out)))
+        return out.reshape(self.members, -1)

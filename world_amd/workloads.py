@@ -127,3 +127,29 @@ def perturbed(c: WdConfig, sigma_m: float, seed: int) -> WdConfig:
     out.perturbation_distribution = WD_PERTURB_GAUSSIAN
     out.init_perturbation_seed = seed
     return out
+
+
+def uneven_ensemble(members: int = 64, *, seed: int = 2024, n_min: int = 2048,
+                    n_max: int = 32768, aspect: tuple[float, float] = (1.0, 4.0),
+                    mu: tuple[float, float] = (0.0, 0.6), sigma_m: float = 50e-6,
+                    dt: float = 1e-3) -> list[WdConfig]:
+    """Seed-drawn uneven column-collapse ensemble in one 1.8 m x 1.8 m x 1 m box.
+
+    Member e draws a grain budget N_e ~ U{n_min..n_max}, an aspect ratio
+    h/w ~ U[aspect] and a friction mu_e ~ U[mu]; its column is w x w x h
+    lattice sites with w = round((N_e / a)^(1/3)) and h = round(N_e / w^2),
+    every grain offset by an isotropic Gaussian of sigma_m, perturbation seed
+    1000 + e (the chaos driver's member seeding, src/chaos.cpp:165)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in range(members):
+        n = int(rng.integers(n_min, n_max + 1))
+        a = float(rng.uniform(*aspect))
+        f = float(rng.uniform(*mu))
+        w = max(1, int(round((n / a) ** (1.0 / 3.0))))
+        h = max(1, int(round(n / (w * w))))
+        c = column(w, w, h, friction=f, ceiling=1.0, dt=dt)
+        out.append(perturbed(c, sigma_m, 1000 + e))
+    return out
