@@ -259,6 +259,38 @@ int wd_batch_member_range(const wd_sim* sim, int member, int* first_out, int* co
  * member, or of all members back to back when member is -1. */
 int wd_batch_density_grid(wd_sim* sim, int member, float* grid_out);
 
+/* ---- parameters and the discrete adjoint (extension) ---------------------
+ * The reference has no differentiation.  These entry points give gradients
+ * of a function of the state at the end of a window of semi-implicit Euler
+ * steps with the spring_history tangential model, with respect to the
+ * window's initial positions and velocities, the friction coefficient and
+ * the normal stiffness.  The adjoint is the exact derivative of the forward
+ * step with every contact set and clamp / stick-slip branch frozen to the
+ * forward's (src/compute_rhs_kernel.cu:81-212 and
+ * src/semi_implicit_euler_kernel.cu:4-16 are the functions differentiated,
+ * with the tangential spring of this engine in place of
+ * src/compute_rhs_kernel.cu:66-78). */
+
+/* physics.friction of one member (0 for a single simulation) and
+ * physics.normal_stiffness (all members); the damping constants follow k_n
+ * (src/compute_rhs_kernel.cu:41-47).  Take effect at the next step. */
+int wd_set_friction(wd_sim* sim, int member, float mu);
+int wd_set_normal_stiffness(wd_sim* sim, float k_n);
+
+/* Advances nsteps steps from the current state (as wd_step) and keeps a
+ * checkpoint of the state and its springs every checkpoint_every steps. */
+int wd_adjoint_record(wd_sim* sim, int nsteps, int checkpoint_every);
+
+/* Given the cotangent of the window's final CIC grids (members x 4 x nodes
+ * floats, the layout of wd_batch_density_grid(sim, -1, ...); may be null) and
+ * of its final state (6 floats per grain, original order; may be null),
+ * writes dL/d(initial state) (6 floats per grain) and, per member,
+ * {dL/dfriction, dL/dnormal_stiffness} (2 doubles per member).  Recomputes
+ * each segment from its checkpoint; leaves the handle at the window's end.
+ * Fails if a cell held more than particles_per_cell grains in the window. */
+int wd_adjoint_backward(wd_sim* sim, const float* grid_cotangent, const float* state_cotangent,
+                        float* grad_state0, double* grad_params);
+
 /* ---- parity / inspection entry points ---------------------------------- */
 
 /* Extension, tangential_model spring_history: the stored contact springs in

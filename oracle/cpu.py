@@ -69,6 +69,17 @@ def lib() -> C.CDLL:
         h.or_sim_history.restype = None
         h.or_sim_energy.argtypes = [vp, C.c_int]
         h.or_sim_energy.restype = C.c_double
+        h.or_rec_create.argtypes = [P, C.c_int, vp, vp, C.c_int, C.c_int]
+        h.or_rec_create.restype = vp
+        h.or_rec_destroy.argtypes = [vp]
+        h.or_rec_destroy.restype = None
+        h.or_rec_state.argtypes = [vp, vp]
+        h.or_rec_state.restype = None
+        h.or_rec_contacts.argtypes = [vp, C.c_int, vp, vp, vp]
+        h.or_rec_contacts.restype = C.c_longlong
+        h.or_rec_replay.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp,
+                                    C.c_double, C.c_double, vp, vp, vp, vp, C.c_int]
+        h.or_rec_replay.restype = None
         _lib = h
     return _lib
 
@@ -212,6 +223,70 @@ class Sim:
         n = self.n
         return (partner[: n * HIST_SLOTS].reshape(n, HIST_SLOTS),
                 xi[: n * HIST_SLOTS * 3].reshape(n, HIST_SLOTS, 3), cnt[:n])
+
+
+class Recording:
+    """An FP32 SIE trajectory with contact history whose per-step contact sets
+    and clamp / slip decisions are recorded, and replayed with those branches
+    frozen: in FP32 (bit-identical to the forward), FP64 (the function that
+    central finite differences sample), FP64 with one forward-mode tangent
+    (the JVP), or the FP32 primal with an FP64 tangent (the JVP at the exact
+    points of the FP32 trajectory)."""
+
+    def __init__(self, cfg, state, material, nsteps: int, threads: int = 8):
+        st = _st(state)
+        self.n = st.size // 6
+        self.cfg = cfg.copy()
+        self.nsteps = int(nsteps)
+        self.mat = np.ascontiguousarray(material, np.int32)
+        self.nodes = (cfg.density_grid_size_x * cfg.density_grid_size_y *
+                      cfg.density_grid_size_z)
+        self._h = lib().or_rec_create(C.byref(self.cfg), self.n, _p(st), _p(self.mat),
+                                      self.nsteps, int(threads))
+
+    def close(self):
+        if self._h:
+            lib().or_rec_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def final_state(self) -> np.ndarray:
+        out = np.zeros(6 * max(self.n, 1), np.float32)
+        lib().or_rec_state(self._h, _p(out))
+        return out[: 6 * self.n].reshape(self.n, 6)
+
+    def contacts(self, step: int):
+        cnt = np.zeros(max(self.n, 1), np.int32)
+        total = lib().or_rec_contacts(self._h, int(step), _p(cnt), None, None)
+        partners = np.zeros(max(total, 1), np.int32)
+        flags = np.zeros(max(total, 1), np.int32)
+        lib().or_rec_contacts(self._h, int(step), _p(cnt), _p(partners), _p(flags))
+        return cnt[: self.n], partners[:total], flags[:total]
+
+    def replay(self, precision: int, x0, v0, mu: float = 0.0, k: float = 0.0, tx0=None,
+               tv0=None, tmu: float = 0.0, tk: float = 0.0, threads: int = 8):
+        """Final state [n, 6], CIC grid [4 * nodes] and, for precision 65
+        (FP64 primal) or 66 (the FP32 primal of the forward, bit for bit, with
+        an FP64 tangent), their tangents."""
+        x0 = np.ascontiguousarray(x0, np.float64).reshape(-1)
+        v0 = np.ascontiguousarray(v0, np.float64).reshape(-1)
+        assert x0.size == 3 * self.n and v0.size == 3 * self.n
+        st = np.zeros(6 * max(self.n, 1), np.float64)
+        rho = np.zeros(4 * self.nodes, np.float64)
+        dst = np.zeros_like(st) if precision in (65, 66) else None
+        drho = np.zeros_like(rho) if precision in (65, 66) else None
+        tx = None if tx0 is None else np.ascontiguousarray(tx0, np.float64).reshape(-1)
+        tv = None if tv0 is None else np.ascontiguousarray(tv0, np.float64).reshape(-1)
+        lib().or_rec_replay(self._h, int(precision), _p(x0), _p(v0), float(mu), float(k),
+                            None if tx is None else _p(tx), None if tv is None else _p(tv),
+                            float(tmu), float(tk), _p(st), None if dst is None else _p(dst),
+                            _p(rho), None if drho is None else _p(drho), int(threads))
+        st = st[: 6 * self.n].reshape(self.n, 6)
+        if precision in (65, 66):
+            return st, rho, dst[: 6 * self.n].reshape(self.n, 6), drho
+        return st, rho
 
 
 # -- the reference's own YAML loader (built from the reference sources) ------

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "../include/world_amd.h"
@@ -264,8 +265,9 @@ void friction(T f[3], const T vt[3], T limit, T c, Branches* br) {
 // scale F_t to mu * fp and reset the spring to match.  Same FP32 order as
 // the GPU's spring_friction.
 template <typename T>
-void spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], T kt, T mu, T dt,
-                 Branches* br) {
+bool spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], T kt, T mu, T dt,
+                 Branches* br, int forced_slip = -1) {
+    using std::sqrt;
     T vt[3];
     for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
     T xn = 0;
@@ -276,9 +278,9 @@ void spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], 
     for (int a = 0; a < 3; ++a) ft[a] = -(kt * xi[a]) - c * vt[a];
     T s = 0;
     for (int a = 0; a < 3; ++a) s += ft[a] * ft[a];
-    const T fm = std::sqrt(s);
+    const T fm = sqrt(s);
     const T lim = mu * fp;
-    const bool slip = br ? br->take(fm > lim) : (fm > lim);
+    const bool slip = forced_slip >= 0 ? forced_slip != 0 : (br ? br->take(fm > lim) : (fm > lim));
     if (slip) {
         const T scale = lim / fm;
         for (int a = 0; a < 3; ++a) {
@@ -287,6 +289,7 @@ void spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], 
         }
     }
     for (int a = 0; a < 3; ++a) f[a] += ft[a];
+    return slip;
 }
 
 // max(0, min(fmax, kneg * d)) with fminf/fmaxf semantics (a NaN operand
@@ -300,13 +303,22 @@ T spring(T fmax, T kneg, T d, Branches* br) {
     return neg ? T(0) : inner;
 }
 
+// One contact of one grain in one step, as the FP32 forward took it: the
+// partner (original index, or a wall code), whether the normal spring was
+// clamped at max_force, and whether the tangential spring slipped.
+struct RContact {
+    int partner;
+    uint8_t capped, slip;
+};
+
 // Force on grain i (src/compute_rhs_kernel.cu:81-212).  T = float follows the
 // reference's FP32 order; T = double replays the FP32 branch decisions.
 // `scale` receives sum |f_term| over every force term (tolerance scale).
 template <typename T>
 void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const float* st,
                  const int* mat, int i, T out_a[3], Branches* br, double* scale,
-                 const HList* oldh = nullptr, HList* newh = nullptr) {
+                 const HList* oldh = nullptr, HList* newh = nullptr,
+                 std::vector<RContact>* rec = nullptr) {
     (void)n;
     const float* si = st + 6 * static_cast<size_t>(i);
     T pos[3], vel[3];
@@ -346,8 +358,9 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
                 for (int q = 0; q < 3; ++q) vn += vel[q] * nrm[q];
                 T xi[3];
                 hist_find(oldh, wall_code(a, 0), xi);
-                spring_fric(f, nrm, vel, vn, fl, c_wall, xi, kt, mu, dt, br);
+                const bool slip = spring_fric(f, nrm, vel, vn, fl, c_wall, xi, kt, mu, dt, br);
                 hist_push(newh, wall_code(a, 0), xi);
+                if (rec) rec->push_back({wall_code(a, 0), static_cast<uint8_t>(fl == fmax), static_cast<uint8_t>(slip)});
             } else {
                 friction(f, vt, mu * fl, c_wall, br);
             }
@@ -367,8 +380,9 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
                 for (int q = 0; q < 3; ++q) vn += vel[q] * nrm[q];
                 T xi[3];
                 hist_find(oldh, wall_code(a, 1), xi);
-                spring_fric(f, nrm, vel, vn, fu, c_wall, xi, kt, mu, dt, br);
+                const bool slip = spring_fric(f, nrm, vel, vn, fu, c_wall, xi, kt, mu, dt, br);
                 hist_push(newh, wall_code(a, 1), xi);
+                if (rec) rec->push_back({wall_code(a, 1), static_cast<uint8_t>(fu == fmax), static_cast<uint8_t>(slip)});
             } else {
                 friction(f, vt, mu * fu, c_wall, br);
             }
@@ -416,8 +430,9 @@ void grain_force(const wd_config& c, const Phys& ph, const Bins& b, int n, const
             if (ph.history) {
                 T xi[3];
                 hist_find(oldh, j, xi);
-                spring_fric(f, nrm, vr, vn, fp, cp, xi, kt, mu, dt, br);
+                const bool slip = spring_fric(f, nrm, vr, vn, fp, cp, xi, kt, mu, dt, br);
                 hist_push(newh, j, xi);
+                if (rec) rec->push_back({j, static_cast<uint8_t>(fp == fmax), static_cast<uint8_t>(slip)});
             } else {
                 friction(f, vt, mu * fp, cp, br);
             }
@@ -573,6 +588,379 @@ void sim_steps(OrSim& s, int nsteps, int threads) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Frozen-branch replay: the checker of the adjoint.
+//
+// rec_forward runs the FP32 SIE forward with contact history (the same
+// arithmetic as sim_steps) and records, per step and grain, the contacts in
+// the order the force pass took them with their clamp and slip decisions.
+// replay<T> re-runs that trajectory with every contact set and branch frozen
+// to the record: T = float reproduces the forward bit for bit (a check of the
+// replay itself), T = double is the FP64 function that central finite
+// differences sample, T = Dual carries one forward-mode tangent (the JVP).
+
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double a) : v(a), d(0.0) {}  // NOLINT: constants promote
+    Dual(double a, double b) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline Dual sqrt(Dual a) {
+    const double r = std::sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+// Float primal with an FP64 tangent: the primal operations are the FP32
+// forward's own (bit for bit), the tangent is their exact derivative
+// evaluated at those FP32 values.  This is the Jacobian at the very points
+// the GPU visits.
+struct DualF {
+    float v = 0.f;
+    double d = 0.0;
+    DualF() = default;
+    DualF(double a) : v(static_cast<float>(a)), d(0.0) {}  // NOLINT: constants promote
+    DualF(float a, double b) : v(a), d(b) {}
+};
+inline DualF operator+(DualF a, DualF b) { return {a.v + b.v, a.d + b.d}; }
+inline DualF operator-(DualF a, DualF b) { return {a.v - b.v, a.d - b.d}; }
+inline DualF operator-(DualF a) { return {-a.v, -a.d}; }
+inline DualF operator*(DualF a, DualF b) {
+    return {a.v * b.v, a.d * static_cast<double>(b.v) + static_cast<double>(a.v) * b.d};
+}
+inline DualF operator/(DualF a, DualF b) {
+    const double bv = b.v;
+    return {a.v / b.v, (a.d * bv - static_cast<double>(a.v) * b.d) / (bv * bv)};
+}
+inline DualF& operator+=(DualF& a, DualF b) { return a = a + b; }
+inline DualF& operator-=(DualF& a, DualF b) { return a = a - b; }
+inline bool operator>(DualF a, DualF b) { return a.v > b.v; }
+inline bool operator<(DualF a, DualF b) { return a.v < b.v; }
+inline DualF sqrt(DualF a) {
+    const float r = std::sqrt(a.v);
+    return {r, a.d / (2.0 * static_cast<double>(r))};
+}
+inline double val(DualF x) { return x.v; }
+inline double tan_of(DualF x) { return x.d; }
+inline Dual to_dual(DualF x) { return Dual(x.v, x.d); }
+inline Dual to_dual(Dual x) { return x; }
+
+inline double val(float x) { return x; }
+inline double val(double x) { return x; }
+inline double val(Dual x) { return x.v; }
+inline double tan_of(float) { return 0.0; }
+inline double tan_of(double) { return 0.0; }
+inline double tan_of(Dual x) { return x.d; }
+
+template <typename T>
+T damping_t(double e, T k, double m) {
+    using std::sqrt;
+    const double pi = 3.14159265358979323846;
+    if (e <= 0.0) return T(2.0) * sqrt(k * T(m));
+    if (e >= 1.0) return T(0.0);
+    const double ln_e = std::log(e);
+    return T(-2.0 * ln_e / std::sqrt(pi * pi + ln_e * ln_e)) * sqrt(T(m) * k);
+}
+
+struct RStep {
+    std::vector<int> off;  // n + 1
+    std::vector<RContact> c;
+};
+
+struct Recording {
+    wd_config c;
+    int n = 0;
+    std::vector<float> st0, stT;
+    std::vector<int> mat;
+    std::vector<RStep> steps;
+};
+
+void rec_forward(Recording& R, int nsteps, int threads) {
+    OrSim s;
+    s.c = R.c;
+    s.n = R.n;
+    s.st = R.st0;
+    s.mat = R.mat;
+    s.hist.assign(R.n, HList());
+    const Phys ph = phys_of(s.c);
+    const float dt = s.c.dt;
+    std::vector<float> acc(3 * static_cast<size_t>(s.n));
+    std::vector<HList> newh;
+    std::vector<std::vector<RContact>> per(R.n);
+    R.steps.assign(nsteps, RStep());
+    for (int step = 0; step < nsteps; ++step) {
+        Bins b;
+        build_bins(s.c, s.n, s.st.data(), b);
+        newh.assign(s.n, HList());
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+        for (int i = 0; i < s.n; ++i) {
+            per[i].clear();
+            grain_force<float>(s.c, ph, b, s.n, s.st.data(), s.mat.data(), i, &acc[3 * static_cast<size_t>(i)],
+                               nullptr, nullptr, &s.hist[i], &newh[i], &per[i]);
+        }
+        RStep& rs = R.steps[step];
+        rs.off.assign(s.n + 1, 0);
+        for (int i = 0; i < s.n; ++i) rs.off[i + 1] = rs.off[i] + static_cast<int>(per[i].size());
+        rs.c.resize(rs.off[s.n]);
+        for (int i = 0; i < s.n; ++i) std::copy(per[i].begin(), per[i].end(), rs.c.begin() + rs.off[i]);
+        for (int i = 0; i < s.n; ++i) {
+            float* q = s.st.data() + 6 * static_cast<size_t>(i);
+            for (int a = 0; a < 3; ++a) q[3 + a] += dt * acc[3 * i + a];
+            for (int a = 0; a < 3; ++a) q[a] += dt * q[3 + a];
+        }
+        s.hist.swap(newh);
+    }
+    R.stT = s.st;
+}
+
+template <typename T>
+struct RHist {
+    int partner;
+    T xi[3];
+};
+
+template <typename T>
+void rhist_find(const std::vector<RHist<T>>& h, int partner, T xi[3]) {
+    xi[0] = xi[1] = xi[2] = T(0.0);
+    for (const RHist<T>& e : h) {
+        if (e.partner == partner) {
+            for (int a = 0; a < 3; ++a) xi[a] = e.xi[a];
+            return;
+        }
+    }
+}
+
+// Parameters of the replayed function.  For T = float they are the forward's
+// own FP32 constants; otherwise mu and k are the inputs and every derived
+// constant (k_t, the damping table) is formed from k in T.
+template <typename T>
+struct RParams {
+    T mu, k, kneg, kt, fmax, r, dt, g;
+    T lower[3], upper[3];
+    std::vector<T> mass, c_wall, c_pair;
+    int nmat = 0;
+};
+
+template <typename T>
+RParams<T> rparams(const wd_config& c, const Phys& ph, T mu, T k) {
+    RParams<T> p;
+    p.nmat = c.num_masses;
+    p.mass.resize(p.nmat);
+    for (int m = 0; m < p.nmat; ++m) p.mass[m] = T(ph.mass[m]);
+    p.fmax = T(ph.fmax);
+    p.r = T(ph.r);
+    p.dt = T(ph.dt);
+    p.g = T(ph.g);
+    for (int a = 0; a < 3; ++a) {
+        p.lower[a] = T(ph.lower[a]);
+        p.upper[a] = T(ph.upper[a]);
+    }
+    p.c_wall.resize(p.nmat);
+    p.c_pair.resize(static_cast<size_t>(p.nmat) * p.nmat);
+    if constexpr (std::is_same<T, float>::value) {
+        p.mu = T(ph.mu);
+        p.k = T(ph.k);
+        p.kneg = T(ph.kneg);
+        p.kt = T(ph.kt);
+        for (int a = 0; a < p.nmat; ++a) {
+            p.c_wall[a] = T(damping32(ph.e, ph.k, ph.mass[a]));
+            for (int b = 0; b < p.nmat; ++b) {
+                const float msum = ph.mass[a] + ph.mass[b];
+                const float mred = (msum > 0.f) ? (ph.mass[a] * ph.mass[b]) / msum : 0.f;
+                p.c_pair[a * p.nmat + b] = T(damping32(ph.e, ph.k, mred));
+            }
+        }
+    } else if constexpr (std::is_same<T, DualF>::value) {
+        // The FP32 forward's constants; c ~ sqrt(k_n), so dc/dk_n = c / (2 k_n).
+        p.mu = mu;
+        p.k = k;
+        p.kneg = T(0.0) - k;
+        p.kt = T(static_cast<double>(c.tangential_stiffness_ratio)) * k;
+        const float kv = static_cast<float>(val(k));
+        const double dk = tan_of(k);
+        auto damp = [&](float m) {
+            const float cv = damping32(ph.e, kv, m);
+            return DualF(cv, static_cast<double>(cv) * dk / (2.0 * static_cast<double>(kv)));
+        };
+        for (int a = 0; a < p.nmat; ++a) {
+            p.c_wall[a] = damp(ph.mass[a]);
+            for (int b = 0; b < p.nmat; ++b) {
+                const float msum = ph.mass[a] + ph.mass[b];
+                const float mred = (msum > 0.f) ? (ph.mass[a] * ph.mass[b]) / msum : 0.f;
+                p.c_pair[a * p.nmat + b] = damp(mred);
+            }
+        }
+    } else {
+        p.mu = mu;
+        p.k = k;
+        p.kneg = T(0.0) - k;
+        p.kt = T(static_cast<double>(c.tangential_stiffness_ratio)) * k;
+        for (int a = 0; a < p.nmat; ++a) {
+            p.c_wall[a] = damping_t<T>(ph.e, k, ph.mass[a]);
+            for (int b = 0; b < p.nmat; ++b) {
+                const double ma = ph.mass[a], mb = ph.mass[b];
+                const double mred = (ma + mb > 0.0) ? ma * mb / (ma + mb) : 0.0;
+                p.c_pair[a * p.nmat + b] = damping_t<T>(ph.e, k, mred);
+            }
+        }
+    }
+    return p;
+}
+
+// One grain's acceleration with the recorded contacts of one step, in the
+// order and arithmetic of grain_force (contacts only: a candidate that does
+// not touch adds exact zeros there).
+template <typename T>
+void replay_grain(const RParams<T>& P, const T* st, const int* mat, int i, const RContact* c, int nc,
+                  const std::vector<RHist<T>>& oldh, std::vector<RHist<T>>& newh, T out_a[3]) {
+    using std::sqrt;
+    const T* si = st + 6 * static_cast<size_t>(i);
+    T pos[3], vel[3];
+    for (int a = 0; a < 3; ++a) {
+        pos[a] = si[a];
+        vel[a] = si[3 + a];
+    }
+    const int mi = mat[i];
+    const T m_i = P.mass[mi];
+    const T c_wall = P.c_wall[mi];
+    T f[3] = {T(0.0f), T(0.0f), T(0.0f)};
+    f[2] -= m_i * P.g;
+    newh.clear();
+    auto push = [&](int partner, const T xi[3]) {
+        if (static_cast<int>(newh.size()) >= kHistSlots) return;
+        RHist<T> e;
+        e.partner = partner;
+        for (int a = 0; a < 3; ++a) e.xi[a] = xi[a];
+        newh.push_back(e);
+    };
+    for (int k = 0; k < nc; ++k) {
+        const RContact& e = c[k];
+        if (e.partner < 0) {
+            const int a = (-e.partner - 1) / 2, side = (-e.partner - 1) % 2;
+            const T d = side == 0 ? pos[a] - P.lower[a] - P.r : P.upper[a] - pos[a] - P.r;
+            const T fw = e.capped ? P.fmax : P.kneg * d;
+            if (side == 0) f[a] += fw;
+            else f[a] -= fw;
+            f[a] -= c_wall * vel[a];
+            T nrm[3] = {T(0.0f), T(0.0f), T(0.0f)};
+            nrm[a] = side == 0 ? T(1.0f) : T(-1.0f);
+            T vn = T(0.0f);
+            for (int q = 0; q < 3; ++q) vn += vel[q] * nrm[q];
+            T xi[3];
+            rhist_find(oldh, e.partner, xi);
+            spring_fric<T>(f, nrm, vel, vn, fw, c_wall, xi, P.kt, P.mu, P.dt, nullptr, e.slip);
+            push(e.partner, xi);
+        } else {
+            const T* sj = st + 6 * static_cast<size_t>(e.partner);
+            T d[3];
+            T s = T(0.0f);
+            for (int a = 0; a < 3; ++a) {
+                d[a] = pos[a] - sj[a];
+                s += d[a] * d[a];
+            }
+            const T norm = sqrt(s);
+            const T dist = norm - P.r - P.r;
+            T nrm[3];
+            for (int a = 0; a < 3; ++a) nrm[a] = d[a] / norm;
+            const T fp = e.capped ? P.fmax : P.kneg * dist;
+            for (int a = 0; a < 3; ++a) f[a] += fp * nrm[a];
+            const T cp = P.c_pair[mi * P.nmat + mat[e.partner]];
+            T vr[3], vn = T(0.0f);
+            for (int a = 0; a < 3; ++a) {
+                vr[a] = vel[a] - sj[3 + a];
+                vn += vr[a] * nrm[a];
+            }
+            for (int a = 0; a < 3; ++a) f[a] -= cp * vn * nrm[a];
+            T xi[3];
+            rhist_find(oldh, e.partner, xi);
+            spring_fric<T>(f, nrm, vr, vn, fp, cp, xi, P.kt, P.mu, P.dt, nullptr, e.slip);
+            push(e.partner, xi);
+        }
+    }
+    for (int a = 0; a < 3; ++a) out_a[a] = f[a] / m_i;
+}
+
+template <typename A, typename T>
+A acc_of(T x) {
+    if constexpr (std::is_same<A, Dual>::value && std::is_same<T, DualF>::value) return to_dual(x);
+    else return A(x);
+}
+
+// CIC deposit of a T state (axis weights as axis_w), accumulated in A.
+template <typename T, typename A>
+void cic_t(const wd_config& c, int n, const T* st, const int* mat, A* out) {
+    const int size[3] = {c.density_grid_size_x, c.density_grid_size_y, c.density_grid_size_z};
+    const size_t nodes = static_cast<size_t>(size[0]) * size[1] * size[2];
+    for (size_t k = 0; k < 4 * nodes; ++k) out[k] = A(0.0);
+    const float lo[3] = {c.x_min, c.y_min, c.z_min}, hi[3] = {c.x_max, c.y_max, c.z_max};
+    for (int i = 0; i < n; ++i) {
+        const T* s = st + 6 * static_cast<size_t>(i);
+        const float mf = c.masses[mat[i]];
+        if (mf == 0.f) continue;
+        const T m = T(mf);
+        int idx[3][2];
+        T w[3][2];
+        for (int a = 0; a < 3; ++a) {
+            if (size[a] <= 1) {
+                idx[a][0] = idx[a][1] = 0;
+                w[a][0] = T(1.0f);
+                w[a][1] = T(0.0f);
+                continue;
+            }
+            const T h = (T(hi[a]) - T(lo[a])) / T(static_cast<float>(size[a] - 1));
+            const T u = (s[a] - T(lo[a])) / h;
+            const double bv = std::floor(val(u));
+            const T frac = u - T(static_cast<float>(bv));
+            long long bb;
+            if (sizeof(T) == 4 || std::is_same<T, DualF>::value) bb = to_int_sat(static_cast<float>(bv));
+            else bb = std::isnan(bv) ? 0 : static_cast<long long>(bv);
+            auto clampi = [&](long long v) { return static_cast<int>(v < 0 ? 0 : (v > size[a] - 1 ? size[a] - 1 : v)); };
+            idx[a][0] = clampi(bb);
+            idx[a][1] = clampi(bb + 1);
+            w[a][0] = T(1.0f) - frac;
+            w[a][1] = frac;
+        }
+        const T mom[3] = {m * s[3], m * s[4], m * s[5]};
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y)
+                for (int z = 0; z < 2; ++z) {
+                    const T ww = w[0][x] * w[1][y] * w[2][z];
+                    if (val(ww) == 0.0 && tan_of(ww) == 0.0) continue;
+                    const size_t node = (static_cast<size_t>(idx[0][x]) * size[1] + idx[1][y]) * size[2] + idx[2][z];
+                    out[node] += acc_of<A>(m * ww);
+                    for (int a = 0; a < 3; ++a) out[(a + 1) * nodes + node] += acc_of<A>(mom[a] * ww);
+                }
+    }
+}
+
+template <typename T>
+void replay_run(const Recording& R, const RParams<T>& P, std::vector<T>& st, int nsteps, int threads) {
+    const int n = R.n;
+    std::vector<std::vector<RHist<T>>> oldh(n), newh(n);
+    std::vector<T> acc(3 * static_cast<size_t>(n));
+    for (int step = 0; step < nsteps; ++step) {
+        const RStep& rs = R.steps[step];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+        for (int i = 0; i < n; ++i) {
+            replay_grain<T>(P, st.data(), R.mat.data(), i, rs.c.data() + rs.off[i], rs.off[i + 1] - rs.off[i],
+                            oldh[i], newh[i], &acc[3 * static_cast<size_t>(i)]);
+        }
+        for (int i = 0; i < n; ++i) {
+            T* q = st.data() + 6 * static_cast<size_t>(i);
+            for (int a = 0; a < 3; ++a) q[3 + a] += P.dt * acc[3 * i + a];
+            for (int a = 0; a < 3; ++a) q[a] += P.dt * q[3 + a];
+        }
+        oldh.swap(newh);
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -841,4 +1229,123 @@ void or_density_grid(const wd_config* c, int n, const float* st, const int* mat,
     }
 }
 
+
+// ---- frozen-branch replay ---------------------------------------------------
+
+// Records nsteps of the FP32 SIE forward with contact history.
+void* or_rec_create(const wd_config* c, int n, const float* st, const int* mat, int nsteps, int threads) {
+    Recording* R = new Recording();
+    R->c = *c;
+    R->n = n;
+    R->st0.assign(st, st + 6 * static_cast<size_t>(n));
+    R->mat.assign(mat, mat + n);
+    rec_forward(*R, nsteps, threads);
+    return R;
+}
+
+void or_rec_destroy(void* h) { delete static_cast<Recording*>(h); }
+
+void or_rec_state(void* h, float* out) {
+    const Recording* R = static_cast<Recording*>(h);
+    std::memcpy(out, R->stT.data(), sizeof(float) * R->stT.size());
+}
+
+// Contacts of one recorded step: per-grain counts, and (if partners is not
+// null) the partners and flags (bit 0 clamp, bit 1 slip) in order.
+long long or_rec_contacts(void* h, int step, int* counts, int* partners, int* flags) {
+    const Recording* R = static_cast<Recording*>(h);
+    const RStep& rs = R->steps[step];
+    for (int i = 0; i < R->n; ++i) counts[i] = rs.off[i + 1] - rs.off[i];
+    if (partners) {
+        for (size_t k = 0; k < rs.c.size(); ++k) {
+            partners[k] = rs.c[k].partner;
+            flags[k] = rs.c[k].capped | (rs.c[k].slip << 1);
+        }
+    }
+    return static_cast<long long>(rs.c.size());
+}
+
+// Replays the recorded trajectory from state (x0, v0) with parameters
+// (mu, k_n) and the recorded branches.  precision 32: FP32 with the
+// forward's own constants (mu and k_n ignored); 64: FP64; 65: FP64 with the
+// tangent (tx0, tv0, tmu, tk) carried forward (JVP); 66: the FP32 forward's
+// own primal values (bit for bit) with an FP64 tangent, i.e. the Jacobian at
+// the FP32 trajectory.  Writes the final state and its CIC grid [4][nodes];
+// for 65 and 66 also their tangents.
+void or_rec_replay(void* h, int precision, const double* x0, const double* v0, double mu, double k,
+                   const double* tx0, const double* tv0, double tmu, double tk, double* state_out,
+                   double* dstate_out, double* rho_out, double* drho_out, int threads) {
+    const Recording* R = static_cast<Recording*>(h);
+    const int n = R->n;
+    const Phys ph = phys_of(R->c);
+    const size_t nodes = static_cast<size_t>(R->c.density_grid_size_x) * R->c.density_grid_size_y *
+                         R->c.density_grid_size_z;
+    const int nsteps = static_cast<int>(R->steps.size());
+    if (precision == 32) {
+        std::vector<float> st(6 * static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                st[6 * i + a] = static_cast<float>(x0[3 * i + a]);
+                st[6 * i + 3 + a] = static_cast<float>(v0[3 * i + a]);
+            }
+        const RParams<float> P = rparams<float>(R->c, ph, 0.f, 0.f);
+        replay_run<float>(*R, P, st, nsteps, threads);
+        for (size_t q = 0; q < st.size(); ++q) state_out[q] = st[q];
+        if (rho_out) cic_t<float, double>(R->c, n, st.data(), R->mat.data(), rho_out);
+    } else if (precision == 64) {
+        std::vector<double> st(6 * static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                st[6 * i + a] = x0[3 * i + a];
+                st[6 * i + 3 + a] = v0[3 * i + a];
+            }
+        const RParams<double> P = rparams<double>(R->c, ph, mu, k);
+        replay_run<double>(*R, P, st, nsteps, threads);
+        std::memcpy(state_out, st.data(), sizeof(double) * st.size());
+        if (rho_out) cic_t<double, double>(R->c, n, st.data(), R->mat.data(), rho_out);
+    } else if (precision == 66) {
+        std::vector<DualF> st(6 * static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                st[6 * i + a] = DualF(static_cast<float>(x0[3 * i + a]), tx0 ? tx0[3 * i + a] : 0.0);
+                st[6 * i + 3 + a] = DualF(static_cast<float>(v0[3 * i + a]), tv0 ? tv0[3 * i + a] : 0.0);
+            }
+        const RParams<DualF> P = rparams<DualF>(R->c, ph, DualF(static_cast<float>(mu), tmu),
+                                                DualF(static_cast<float>(k), tk));
+        replay_run<DualF>(*R, P, st, nsteps, threads);
+        for (size_t q = 0; q < st.size(); ++q) {
+            state_out[q] = st[q].v;
+            if (dstate_out) dstate_out[q] = st[q].d;
+        }
+        if (rho_out || drho_out) {
+            std::vector<Dual> rho(4 * nodes);
+            cic_t<DualF, Dual>(R->c, n, st.data(), R->mat.data(), rho.data());
+            for (size_t q = 0; q < rho.size(); ++q) {
+                if (rho_out) rho_out[q] = rho[q].v;
+                if (drho_out) drho_out[q] = rho[q].d;
+            }
+        }
+    } else {
+        std::vector<Dual> st(6 * static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                st[6 * i + a] = Dual(x0[3 * i + a], tx0 ? tx0[3 * i + a] : 0.0);
+                st[6 * i + 3 + a] = Dual(v0[3 * i + a], tv0 ? tv0[3 * i + a] : 0.0);
+            }
+        const RParams<Dual> P = rparams<Dual>(R->c, ph, Dual(mu, tmu), Dual(k, tk));
+        replay_run<Dual>(*R, P, st, nsteps, threads);
+        for (size_t q = 0; q < st.size(); ++q) {
+            state_out[q] = st[q].v;
+            if (dstate_out) dstate_out[q] = st[q].d;
+        }
+        if (rho_out || drho_out) {
+            std::vector<Dual> rho(4 * nodes);
+            cic_t<Dual, Dual>(R->c, n, st.data(), R->mat.data(), rho.data());
+            for (size_t q = 0; q < rho.size(); ++q) {
+                if (rho_out) rho_out[q] = rho[q].v;
+                if (drho_out) drho_out[q] = rho[q].d;
+            }
+        }
+    }
+}
 }  // extern "C"

@@ -378,3 +378,78 @@ def test_history_spring_sticks_then_slips(oracle_lib):
     e_expected = (0.5 * m * float(np.dot(s[3:], s[3:])) + m * 9.81 * s[2]
                   + 0.5 * workloads.K_N * ov * ov + 0.5 * kt * float(np.dot(xi[0, 0], xi[0, 0])))
     assert e_with == pytest.approx(e_expected, rel=1e-6)
+
+
+# -- frozen-branch replay: the checker of the adjoint -------------------------
+
+def replay_case(oracle_lib, nsteps=120):
+    c = workloads.column(6, 6, 6, friction=0.3)
+    st, mat = oracle_lib.initial_state(c)
+    st = st.reshape(-1, 6).copy()
+    rng = np.random.default_rng(5)
+    st[:, :3] += rng.normal(0, 0.0015, (len(st), 3)).astype(np.float32)
+    st[:, 3:] = rng.normal(0, 0.2, (len(st), 3)).astype(np.float32)
+    return c, st, mat, oracle_lib.Recording(c, st, mat, nsteps)
+
+
+def test_replay_in_fp32_reproduces_the_forward(oracle_lib):
+    c, st, mat, R = replay_case(oracle_lib)
+    fin = R.final_state()
+    want, _, _ = oracle_lib.steps(c, st, mat, R.nsteps)
+    np.testing.assert_array_equal(fin, want)
+    s32, rho32 = R.replay(32, st[:, :3], st[:, 3:])
+    np.testing.assert_array_equal(s32.astype(np.float32), fin)
+    np.testing.assert_array_equal(rho32, oracle_lib.density_grid(c, fin, mat, 32))
+    cnt, partners, flags = R.contacts(R.nsteps - 1)
+    assert cnt.sum() == len(partners) > 0
+    assert (flags >> 1).any() and not (flags >> 1).all()  # both stick and slip occur
+
+
+def test_replay_tangent_matches_central_differences(oracle_lib):
+    """The forward-mode tangent of the FP64 replay against central differences
+    of the same frozen-branch function: the error falls as h^2."""
+    c, st, mat, R = replay_case(oracle_lib)
+    x0, v0 = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    mu, k = float(c.friction), float(c.normal_stiffness)
+    rng = np.random.default_rng(9)
+    tx, tv = rng.normal(size=x0.shape) * 1e-3, rng.normal(size=v0.shape) * 1e-2
+    tmu, tk = 0.1, 10.0
+    _, _, dst, drho = R.replay(65, x0, v0, mu, k, tx, tv, tmu, tk)
+    W = rng.normal(size=drho.shape)
+    errs = []
+    for h in (1e-6, 1e-7):
+        sp, rp = R.replay(64, x0 + h * tx, v0 + h * tv, mu + h * tmu, k + h * tk)
+        sm, rm = R.replay(64, x0 - h * tx, v0 - h * tv, mu - h * tmu, k - h * tk)
+        errs.append(abs((W @ rp - W @ rm) / (2 * h) - W @ drho) / abs(W @ drho))
+        fds = (sp - sm) / (2 * h)
+        assert np.abs(fds - dst).max() <= 0.05 * np.abs(dst).max()
+    assert errs[1] < 1e-7 and errs[1] < errs[0] / 30
+
+
+def test_replay_tangent_closed_form_free_flight(oracle_lib):
+    """No contacts: SIE gives x_T = x_0 + T dt v_0 + (gravity terms), so the
+    tangent of x_T along v_0 is T dt and along mu or k_n is zero."""
+    c = workloads.column(1, 1, 1)
+    c.init_type = _capi.WD_INIT_SINGLE_PARTICLE
+    st = np.array([[0.0, 0.0, 0.3, 0.1, -0.2, 0.5]], np.float32)
+    mat = np.array([1], np.int32)
+    R = oracle_lib.Recording(c, st, mat, 50)
+    _, _, dst, _ = R.replay(65, st[:, :3], st[:, 3:], 0.5, 150.0, np.zeros((1, 3)),
+                            np.array([[1.0, 2.0, 3.0]]), 1.0, 1.0)
+    np.testing.assert_allclose(dst[0, :3], np.array([1.0, 2.0, 3.0]) * 50 * c.dt, rtol=1e-12)
+    np.testing.assert_allclose(dst[0, 3:], [1.0, 2.0, 3.0], rtol=1e-12)
+
+
+def test_replay_fp32_primal_tangent(oracle_lib):
+    """Precision 66 keeps the FP32 forward's primal bit for bit; over a short
+    window its tangent equals the FP64 tangent to FP32 accuracy."""
+    c, st, mat, R = replay_case(oracle_lib, nsteps=10)
+    x0, v0 = st[:, :3].astype(np.float64), st[:, 3:].astype(np.float64)
+    rng = np.random.default_rng(4)
+    tx, tv = rng.normal(size=x0.shape), rng.normal(size=v0.shape)
+    s66, r66, d66, dr66 = R.replay(66, x0, v0, float(c.friction), float(c.normal_stiffness), tx, tv, 0.3, 150.0)
+    np.testing.assert_array_equal(s66.astype(np.float32), R.final_state())
+    _, _, d65, dr65 = R.replay(65, x0, v0, float(c.friction), float(c.normal_stiffness), tx, tv, 0.3, 150.0)
+    assert np.abs(d66 - d65).max() <= 1e-4 * np.abs(d65).max()
+    W = rng.normal(size=dr65.shape)
+    assert W @ dr66 == pytest.approx(W @ dr65, rel=1e-4)

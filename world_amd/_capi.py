@@ -134,6 +134,11 @@ _SIGNATURES = [
     ("wd_batch_member_range", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
     ("wd_batch_density_grid", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("wd_set_friction", C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+    ("wd_set_normal_stiffness", C.c_int, [C.c_void_p, C.c_float]),
+    ("wd_adjoint_record", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("wd_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]

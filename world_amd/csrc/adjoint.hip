@@ -1,0 +1,508 @@
+// gfx950 kernels of the discrete adjoint (extension; the reference has no
+// differentiation).
+//
+// The adjoint differentiates the semi-implicit Euler step with the
+// history-carrying tangential spring (dem_device.h: pair_contact /
+// spring_friction arithmetic) along a recorded forward trajectory.  The
+// contact set of every step is the one the forward used (the pass re-derives
+// it bit for bit from the restored state), and so are the clamp and
+// stick/slip branches: the adjoint is the exact derivative of that frozen
+// piecewise-smooth step.
+//
+// Per step s (state s -> s+1), with lambda = dL/d(state s+1) given:
+//   k_adj_contacts  one lane per grain in cell order: replays every contact
+//                   of the forward pass, applies the contact's vector-Jacobian
+//                   product to (dL/dF_i, dL/dxi_new), keeps the grain's own
+//                   share, writes dL/dxi_old into the previous spring
+//                   generation, and stores each pair contact's (d, v_r)
+//                   cotangents for the partner;
+//   k_adj_gather    one lane per grain: adds the partners' shares (the pair
+//                   law depends on x_i - x_j and v_i - v_j only, so the
+//                   partner's share is the negative of the stored one), in the
+//                   grain's own contact order: no atomics, deterministic.
+// CIC cotangents enter through k_adj_seed.  Everything is accumulated in FP32
+// except the parameter gradients (per-grain double sums over the window).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace wd {
+
+constexpr int kAdjContacts = 32;  // pair contacts per grain kept for the partner gather
+
+enum AdjError { kAdjErrContacts = 2 };
+
+namespace {
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    float s = 0.f;
+    s += a[0] * b[0];
+    s += a[1] * b[1];
+    s += a[2] * b[2];
+    return s;
+}
+
+// Reverse of one contact of the history model: the normal spring-dashpot
+// (f += fp n - c vn n) followed by spring_friction, recomputed in the same
+// FP32 order so that the slip decision is the forward's.  Inputs: unit
+// normal n, relative velocity vr, vn = vr . n, fp and whether fp is on its
+// linear branch (fp = kneg * dist), the damping c, the stored spring xi, and
+// the cotangents gF (of the grain's force) and gX (of the updated spring).
+// Outputs: cotangents of n, dist, vr, the stored spring, and the parameters
+// k_n and mu (added).
+__device__ __forceinline__ void contact_vjp(const Params& P, float mu, const float n[3],
+                                            const float vr[3], float vn, float fp, bool fp_linear,
+                                            float dist, float c, const float xi[3],
+                                            const float gF[3], const float gX[3], float g_n[3],
+                                            float& g_dist, float g_vr[3], float g_xi[3],
+                                            float& g_k, float& g_mu) {
+    // forward (spring_friction)
+    float vt[3], xi1[3], xi2[3], ft0[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * n[a];
+    float xn = 0.f;
+    xn += xi[0] * n[0];
+    xn += xi[1] * n[1];
+    xn += xi[2] * n[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xi1[a] = xi[a] - xn * n[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xi2[a] = xi1[a] + vt[a] * P.dt;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ft0[a] = -(P.kt * xi2[a]) - c * vt[a];
+    float s = 0.f;
+    s += ft0[0] * ft0[0];
+    s += ft0[1] * ft0[1];
+    s += ft0[2] * ft0[2];
+    const float fm = sqrtf(s);
+    const float lim = mu * fp;
+    const bool slip = fm > lim;
+
+    // reverse
+    float g_ft0[3], g_xi2[3], g_vt[3] = {0.f, 0.f, 0.f};
+    float g_c = 0.f, g_kt = 0.f, g_fp = 0.f;
+    if (slip) {
+        const float scale = lim / fm;
+        float ft[3], g_ft[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ft[a] = ft0[a] * scale;
+            // xi_new = -(ft + c vt) / kt
+            g_ft[a] = gF[a] - gX[a] / P.kt;
+            g_vt[a] = -c * gX[a] / P.kt;
+        }
+        float w[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[a] = ft[a] + c * vt[a];
+        g_c -= dot3(gX, vt) / P.kt;
+        g_kt += dot3(gX, w) / (P.kt * P.kt);
+        // ft = ft0 * lim / |ft0|
+        const float g_scale = dot3(g_ft, ft0);
+        const float g_lim = g_scale / fm;
+        const float g_fm = -g_scale * lim / (fm * fm);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            g_ft0[a] = scale * g_ft[a] + g_fm * ft0[a] / fm;
+            g_xi2[a] = 0.f;
+        }
+        g_mu += g_lim * fp;
+        g_fp += g_lim * mu;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            g_ft0[a] = gF[a];
+            g_xi2[a] = gX[a];
+        }
+    }
+    // ft0 = -(kt xi2) - c vt
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g_xi2[a] -= P.kt * g_ft0[a];
+        g_vt[a] -= c * g_ft0[a];
+    }
+    g_kt -= dot3(xi2, g_ft0);
+    g_c -= dot3(vt, g_ft0);
+    // xi2 = xi1 + vt dt
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g_vt[a] += P.dt * g_xi2[a];
+    // xi1 = xi - (xi . n) n
+    const float g_xn = -dot3(g_xi2, n);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g_xi[a] = g_xi2[a] + g_xn * n[a];
+        g_n[a] = -xn * g_xi2[a] + g_xn * xi[a];
+    }
+    // vt = vr - vn n
+    float g_vn = -dot3(g_vt, n);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g_vr[a] = g_vt[a];
+        g_n[a] -= vn * g_vt[a];
+    }
+    // f += fp n - c vn n
+    const float gFn = dot3(gF, n);
+    g_fp += gFn;
+    g_c -= vn * gFn;
+    g_vn -= c * gFn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g_n[a] += fp * gF[a] - c * vn * gF[a];
+    // vn = vr . n
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g_vr[a] += g_vn * n[a];
+        g_n[a] += g_vn * vr[a];
+    }
+    // fp = clamp(kneg * dist); kneg = -k_n, kt = ratio k_n, c ~ sqrt(k_n)
+    const float k = -P.kneg;
+    float g_kneg = 0.f;
+    g_dist = 0.f;
+    if (fp_linear) {
+        g_dist = P.kneg * g_fp;
+        g_kneg = dist * g_fp;
+    }
+    g_k += -g_kneg + (P.kt / k) * g_kt + g_c * c / (2.f * k);
+}
+
+}  // namespace
+
+// Snapshot of the current state and spring generation, by original id:
+// dP[id], dV[id] = the grain's position/velocity float4s, dH / dC = its
+// stored springs (slots below the count only).
+__global__ __launch_bounds__(kBlock) void k_snap_save(const Ctl* __restrict__ ctl,
+                                                      const float4* __restrict__ Tp,
+                                                      const float4* __restrict__ Tv,
+                                                      HistoryIO hist, float4* __restrict__ dP,
+                                                      float4* __restrict__ dV,
+                                                      float4* __restrict__ dH, int* __restrict__ dC,
+                                                      int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = Tp[i];
+    const int id = __float_as_int(p.w);
+    dP[id] = p;
+    dV[id] = Tv[i];
+    const int par = ctl->hist_parity;
+    const int c = hist.cnt[par][i];
+    dC[i] = c;
+    for (int t = 0; t < c; ++t) dH[static_cast<size_t>(t) * n + i] = hist.buf[par][static_cast<size_t>(t) * n + i];
+}
+
+// Restores a snapshot: T = original order, keys and block partials for the
+// next bin, springs into the current generation.
+__global__ __launch_bounds__(kBlock) void k_snap_load(const Params* __restrict__ Pp,
+                                                      const Ctl* __restrict__ ctl,
+                                                      BlockPart* __restrict__ parts,
+                                                      const float4* __restrict__ sP,
+                                                      const float4* __restrict__ sV,
+                                                      const float4* __restrict__ sH,
+                                                      const int* __restrict__ sC,
+                                                      float4* __restrict__ Tp, float4* __restrict__ Tv,
+                                                      uint32_t* __restrict__ key, HistoryIO hist,
+                                                      int n) {
+    const Params& P = *Pp;
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    int member = -1;
+    if (i < n) {
+        const float4 p = sP[i], v = sV[i];
+        Tp[i] = p;
+        Tv[i] = v;
+        member = member_of(P, v.w);
+        const int par = ctl->hist_parity;
+        const int c = sC[i];
+        hist.cnt[par][i] = c;
+        for (int t = 0; t < c; ++t) hist.buf[par][static_cast<size_t>(t) * n + i] = sH[static_cast<size_t>(t) * n + i];
+        int cx, cy, cz;
+        grain_cell(P, p.x, p.y, p.z, cx, cy, cz);
+        key[i] = pack_cell(cx, cy, cz);
+        lo[0] = hi[0] = cx;
+        lo[1] = hi[1] = cy;
+        lo[2] = hi[2] = cz;
+    }
+    store_block_part(parts, logical_block(), member, lo, hi, 0u, 0.0, false);
+}
+
+// dL/dx, dL/dv of the final state from the cotangent of its CIC grids
+// (4 channels per member, src/density_grid_kernel.cu:43-106) plus an optional
+// direct state cotangent (original order, 6 floats per grain).
+__global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ Pp,
+                                                     const float4* __restrict__ Tp,
+                                                     const float4* __restrict__ Tv,
+                                                     const float* __restrict__ grid_cot,
+                                                     const float* __restrict__ state_cot,
+                                                     float4* __restrict__ lx, float4* __restrict__ lv,
+                                                     int n) {
+    const Params& P = *Pp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = Tp[i], v = Tv[i];
+    const int id = __float_as_int(p.w);
+    float gx[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
+    if (state_cot) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            gx[a] = state_cot[6 * static_cast<size_t>(id) + a];
+            gv[a] = state_cot[6 * static_cast<size_t>(id) + 3 + a];
+        }
+    }
+    const float m = P.mass[grain_mat(v.w)];
+    if (grid_cot && m != 0.f) {
+        const int member = member_of(P, v.w);
+        const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
+        const float* G = grid_cot + static_cast<size_t>(member) * 4 * nodes;
+        const float x[3] = {p.x, p.y, p.z}, vel[3] = {v.x, v.y, v.z};
+        const int size[3] = {P.dgx, P.dgy, P.dgz};
+        int idx[3][2];
+        float w[3][2], dw[3][2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (size[a] <= 1) {
+                idx[a][0] = idx[a][1] = 0;
+                w[a][0] = 1.f; w[a][1] = 0.f;
+                dw[a][0] = dw[a][1] = 0.f;
+                continue;
+            }
+            const float h = (P.hi[a] - P.lo[a]) / static_cast<float>(size[a] - 1);
+            const float u = (x[a] - P.lo[a]) / h;
+            const float base = floorf(u);
+            const float frac = u - base;
+            const int b = sat_f2i(base);
+            idx[a][0] = min(size[a] - 1, max(0, b));
+            idx[a][1] = (b == 2147483647) ? (size[a] - 1) : min(size[a] - 1, max(0, b + 1));
+            w[a][0] = 1.f - frac;
+            w[a][1] = frac;
+            dw[a][0] = -1.f / h;
+            dw[a][1] = 1.f / h;
+        }
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    const size_t node = (static_cast<size_t>(idx[0][cx]) * P.dgy + idx[1][cy]) * P.dgz + idx[2][cz];
+                    const float g0 = G[node], g1 = G[nodes + node], g2 = G[2 * nodes + node],
+                                g3 = G[3 * nodes + node];
+                    // d(m w)/dx + sum_b d(m v_b w)/dx
+                    const float q = m * (g0 + vel[0] * g1 + vel[1] * g2 + vel[2] * g3);
+                    const float wxy = w[0][cx] * w[1][cy];
+                    gx[0] += q * dw[0][cx] * w[1][cy] * w[2][cz];
+                    gx[1] += q * w[0][cx] * dw[1][cy] * w[2][cz];
+                    gx[2] += q * wxy * dw[2][cz];
+                    const float mw = m * wxy * w[2][cz];
+                    gv[0] += mw * g1;
+                    gv[1] += mw * g2;
+                    gv[2] += mw * g3;
+                }
+    }
+    lx[id] = make_float4(gx[0], gx[1], gx[2], 0.f);
+    lv[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
+}
+
+// Pass A of the adjoint of one SIE step (see the file comment).  Grains in
+// cell order (Sp/Sv, start: the bin of state s); h_in = the springs the
+// forward step read; lxi_next[k][id] = dL/d(spring slot k written by the
+// step); lxi_prev[t][id] receives dL/d(spring slot t read by the step).
+__global__ __launch_bounds__(kBlock) void k_adj_contacts(
+        const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
+        const float4* __restrict__ Sv, const uint32_t* __restrict__ start, HistoryIO hist,
+        const float4* __restrict__ lx, const float4* __restrict__ lv,
+        const float4* __restrict__ lxi_next, float4* __restrict__ lxi_prev,
+        float4* __restrict__ own_x, float4* __restrict__ own_v, float4* __restrict__ cg_d,
+        float4* __restrict__ cg_v, int* __restrict__ cg_cnt, double2* __restrict__ gparam, int n) {
+    const Params& P = *Pp;
+    __shared__ uint32_t s_list[kContactList][kBlock];
+    const int p = logical_block() * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 pi = Sp[p];
+    const float4 vi4 = Sv[p];
+    const float x[3] = {pi.x, pi.y, pi.z};
+    const float v[3] = {vi4.x, vi4.y, vi4.z};
+    const int id = __float_as_int(pi.w);
+    const int mat = grain_mat(vi4.w);
+    const int member = member_of(P, vi4.w);
+    const float mu = P.members == 1 ? P.mu : P.mmu[member];
+    const float m = P.mass[mat];
+
+    // SIE: v' = v + dt F/m, x' = x + dt v'.
+    const float4 lx4 = lx[id], lv4 = lv[id];
+    const float lvt[3] = {lv4.x + P.dt * lx4.x, lv4.y + P.dt * lx4.y, lv4.z + P.dt * lx4.z};
+    float gF[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gF[a] = P.dt * lvt[a] / m;
+    float gx[3] = {lx4.x, lx4.y, lx4.z};
+    float gv[3] = {lvt[0], lvt[1], lvt[2]};
+    float gk = 0.f, gmu = 0.f;
+
+    const int par = ctl->hist_parity;
+    const float4* __restrict__ h_in = hist.buf[par];
+    const int h_old = hist.cnt[par][id];
+    for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    int h_new = 0, kc = 0;
+    // Stored spring of `partner` and its slot (-1: a new contact, zero spring).
+    auto h_find = [&](int partner, float xi[3]) __attribute__((always_inline)) {
+        xi[0] = xi[1] = xi[2] = 0.f;
+        for (int t = 0; t < h_old; ++t) {
+            const float4 h = h_in[static_cast<size_t>(t) * n + id];
+            if (__float_as_int(h.w) == partner) {
+                xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
+                return t;
+            }
+        }
+        return -1;
+    };
+    auto g_next = [&](float g[3]) __attribute__((always_inline)) {
+        if (h_new < kHist) {
+            const float4 q = lxi_next[static_cast<size_t>(h_new) * n + id];
+            g[0] = q.x; g[1] = q.y; g[2] = q.z;
+        } else {
+            g[0] = g[1] = g[2] = 0.f;
+        }
+    };
+
+    // Walls, lower then upper per axis, as the forward.
+    const float c_wall = P.c_wall[mat];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const float dist = side == 0 ? x[a] - P.wlo[a] - P.radius : P.whi[a] - x[a] - P.radius;
+            if (dist < 0.f) {
+                const float raw = P.kneg * dist;
+                const float fw = fmaxf(0.f, fminf(P.max_force, raw));
+                float nrm[3] = {0.f, 0.f, 0.f};
+                nrm[a] = side == 0 ? 1.f : -1.f;
+                float vn = 0.f;
+                vn += v[0] * nrm[0];
+                vn += v[1] * nrm[1];
+                vn += v[2] * nrm[2];
+                float xi[3], gX[3], g_n[3], g_vr[3], g_xi[3], g_dist;
+                const int t = h_find(wall_partner(a, side), xi);
+                g_next(gX);
+                contact_vjp(P, mu, nrm, v, vn, fw, fw == raw, dist, c_wall, xi, gF, gX, g_n, g_dist,
+                            g_vr, g_xi, gk, gmu);
+                if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+                gx[a] += side == 0 ? g_dist : -g_dist;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) gv[b] += g_vr[b];
+                ++h_new;
+            }
+        }
+    }
+
+    const MemberCells mc = member_cells(P, member);
+    int cx, cy, cz;
+    grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
+    const uint32_t kcap = static_cast<uint32_t>(P.kcap);
+    int listed = 0;
+    auto flush = [&]() __attribute__((always_inline)) {
+        for (int k = 0; k < listed; ++k) {
+            const uint32_t q = s_list[k][threadIdx.x];
+            const float4 pj = Sp[q];
+            float d0, d1, d2;
+            const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
+            const float dist = norm - P.radius - P.radius;
+            if (!(dist < 0.f)) continue;
+            const int jd = __float_as_int(pj.w);
+            const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
+            const float raw = P.kneg * dist;
+            const float fp = fmaxf(0.f, fminf(P.max_force, raw));
+            const float4 vj = Sv[q];
+            const float c = P.c_pair[mat * kMaxMat + grain_mat(vj.w)];
+            const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
+            float vn = 0.f;
+            vn += vr[0] * nrm[0];
+            vn += vr[1] * nrm[1];
+            vn += vr[2] * nrm[2];
+            float xi[3], gX[3], g_n[3], g_vr[3], g_xi[3], g_dist;
+            const int t = h_find(jd, xi);
+            g_next(gX);
+            contact_vjp(P, mu, nrm, vr, vn, fp, fp == raw, dist, c, xi, gF, gX, g_n, g_dist, g_vr,
+                        g_xi, gk, gmu);
+            if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+            // n = d / |d|, dist = |d| - 2r
+            const float gnn = dot3(g_n, nrm);
+            float gd[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) gd[a] = (g_n[a] - gnn * nrm[a]) / norm + g_dist * nrm[a];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                gx[a] += gd[a];
+                gv[a] += g_vr[a];
+            }
+            if (kc < kAdjContacts) {
+                cg_d[static_cast<size_t>(kc) * n + id] = make_float4(gd[0], gd[1], gd[2], __int_as_float(jd));
+                cg_v[static_cast<size_t>(kc) * n + id] = make_float4(g_vr[0], g_vr[1], g_vr[2], 0.f);
+            } else {
+                ctl->error = kAdjErrContacts;
+            }
+            ++kc;
+            ++h_new;
+        }
+        listed = 0;
+    };
+    // Candidate scan of the forward (touching candidates listed in LDS,
+    // drained in stencil order).
+    for (int c = 0; c < 9; ++c) {
+        uint4 q4;
+        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+        const uint32_t b3[4] = {q4.x, q4.y, q4.z, q4.w};
+        for (int z = 0; z < 3; ++z) {
+            const uint32_t e = min(b3[z + 1], b3[z] + kcap);
+            for (uint32_t q = b3[z]; q < e; ++q) {
+                const float4 pj = Sp[q];
+                const float e0 = x[0] - pj.x, e1 = x[1] - pj.y, e2 = x[2] - pj.z;
+                float s2 = 0.f;
+                s2 += e0 * e0;
+                s2 += e1 * e1;
+                s2 += e2 * e2;
+                if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
+                    s_list[listed][threadIdx.x] = q;
+                    if (++listed == kContactList) flush();
+                }
+            }
+        }
+    }
+    flush();
+    cg_cnt[id] = min(kc, kAdjContacts);
+    own_x[id] = make_float4(gx[0], gx[1], gx[2], 0.f);
+    own_v[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
+    double2 gp = gparam[id];
+    gp.x += static_cast<double>(gmu);
+    gp.y += static_cast<double>(gk);
+    gparam[id] = gp;
+}
+
+// Pass B: dL/d(state s) of grain j = its own share + minus the stored share
+// of every pair contact (i, j) that grain i evaluated, in j's contact order.
+__global__ __launch_bounds__(kBlock) void k_adj_gather(const float4* __restrict__ own_x,
+                                                       const float4* __restrict__ own_v,
+                                                       const float4* __restrict__ cg_d,
+                                                       const float4* __restrict__ cg_v,
+                                                       const int* __restrict__ cg_cnt,
+                                                       float4* __restrict__ lx, float4* __restrict__ lv,
+                                                       int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float4 gx = own_x[j], gv = own_v[j];
+    const int cj = cg_cnt[j];
+    for (int k = 0; k < cj; ++k) {
+        const int i = __float_as_int(cg_d[static_cast<size_t>(k) * n + j].w);
+        const int ci = cg_cnt[i];
+        for (int t = 0; t < ci; ++t) {
+            const float4 e = cg_d[static_cast<size_t>(t) * n + i];
+            if (__float_as_int(e.w) == j) {
+                const float4 f = cg_v[static_cast<size_t>(t) * n + i];
+                gx.x -= e.x; gx.y -= e.y; gx.z -= e.z;
+                gv.x -= f.x; gv.y -= f.y; gv.z -= f.z;
+                break;
+            }
+        }
+    }
+    lx[j] = gx;
+    lv[j] = gv;
+}
+
+}  // namespace wd

@@ -19,6 +19,7 @@
 
 // The kernels are compiled in this translation unit.
 #include "kernels.hip"
+#include "adjoint.hip"
 
 using namespace wd;
 
@@ -103,9 +104,38 @@ struct wd_sim {
     std::vector<TimedStage> pending;
     std::vector<hipEvent_t> spare;
 
+    // Discrete adjoint (extension): checkpoints of the recorded window and
+    // the cotangent buffers, all owned by `owned`.
+    struct Snap {
+        float4* P = nullptr;  // positions by original id
+        float4* V = nullptr;  // velocities by original id
+        float4* H = nullptr;  // springs [kHist][n] by original id
+        int* C = nullptr;     // spring counts by original id
+    };
+    struct AdjointState {
+        int nsteps = 0, every = 0, nseg = 0;
+        unsigned long long overflow0 = 0;
+        std::vector<Snap> ckpt, seg;
+        float4 *lx = nullptr, *lv = nullptr, *own_x = nullptr, *own_v = nullptr;
+        float4 *lxi[2] = {nullptr, nullptr}, *cg_d = nullptr, *cg_v = nullptr;
+        int* cg_cnt = nullptr;
+        double2* gparam = nullptr;
+        float *grid_cot = nullptr, *state_cot = nullptr;
+        std::vector<void*> owned;
+    };
+    AdjointState* adj = nullptr;
+
+    void drop_adjoint() {
+        if (!adj) return;
+        for (void* b : adj->owned) (void)hipFree(b);
+        delete adj;
+        adj = nullptr;
+    }
+
     ~wd_sim() { release(); }
 
     void release() {
+        drop_adjoint();
         if (graph_one) (void)hipGraphExecDestroy(graph_one);
         if (graph_chunk) (void)hipGraphExecDestroy(graph_chunk);
         graph_one = graph_chunk = nullptr;
@@ -352,6 +382,10 @@ struct wd_sim {
     int check_device() {
         int err = 0;
         WD_HIP(hipMemcpy(&err, &ctl->error, sizeof(int), hipMemcpyDeviceToHost));
+        if (err == kAdjErrContacts) {
+            (void)hipMemset(&ctl->error, 0, sizeof(int));
+            return fail(WD_ERR_DEVICE, "adjoint: a grain has more pair contacts than the adjoint keeps (32)");
+        }
         if (err) return fail(WD_ERR_DEVICE, "device: scan look-back wait exceeded its bound");
         return WD_OK;
     }
@@ -454,6 +488,21 @@ void fill_params(const wd_config& c, int n, Params& P) {
 int upload_params(wd_sim* s) {
     WD_HIP(hipMemcpy(s->dp, &s->hp, sizeof(Params), hipMemcpyHostToDevice));
     return WD_OK;
+}
+
+// Recomputes the derived constants from s->cfg (keeping the member tables
+// and the current dt) and uploads them; graphs read them through dp.
+int refresh_params(wd_sim* s) {
+    Params keep = s->hp;
+    fill_params(s->cfg, s->n, s->hp);
+    s->hp.dt = keep.dt;
+    s->hp.members = keep.members;
+    s->hp.mgoff = keep.mgoff;
+    s->hp.mbox = keep.mbox;
+    s->hp.mcell = keep.mcell;
+    s->hp.mmu = keep.mmu;
+    s->hp.mscale = keep.mscale;
+    return upload_params(s);
 }
 
 int allocate(wd_sim* s) {
@@ -1078,6 +1127,201 @@ extern "C" int wd_batch_density_grid(wd_sim* s, int member, float* out) {
     } else {
         WD_HIP(hipMemcpy(out, s->cic_out + per * static_cast<size_t>(member), sizeof(float) * per,
                          hipMemcpyDeviceToHost));
+    }
+    return WD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Parameter updates used by gradient-based callers (extension).
+
+extern "C" int wd_set_friction(wd_sim* s, int member, float mu) {
+    WD_CHECK_SIM(s);
+    if (member < 0 || member >= s->members) return fail(WD_ERR_ARG, "wd_set_friction: bad member");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->mcfg[member].friction = mu;
+    if (member == 0) s->cfg.friction = mu;
+    WD_HIP(hipMemcpy(s->d_mmu + member, &mu, sizeof(float), hipMemcpyHostToDevice));
+    return refresh_params(s);
+}
+
+extern "C" int wd_set_normal_stiffness(wd_sim* s, float k_n) {
+    WD_CHECK_SIM(s);
+    if (!(k_n > 0.f)) return fail(WD_ERR_ARG, "wd_set_normal_stiffness: k_n must be positive");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->cfg.normal_stiffness = k_n;
+    for (auto& c : s->mcfg) c.normal_stiffness = k_n;
+    return refresh_params(s);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of a window of SIE steps (extension).
+
+namespace {
+
+int adj_buf(wd_sim* s, void** p, size_t bytes) {
+    *p = nullptr;
+    WD_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
+    s->adj->owned.push_back(*p);
+    return WD_OK;
+}
+
+template <typename T>
+int adj_alloc(wd_sim* s, T** p, size_t count) {
+    return adj_buf(s, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+int adj_snap(wd_sim* s, wd_sim::Snap& sn) {
+    const size_t n = static_cast<size_t>(s->n);
+    int rc = adj_alloc(s, &sn.P, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &sn.V, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &sn.H, static_cast<size_t>(kHist) * n);
+    if (rc == WD_OK) rc = adj_alloc(s, &sn.C, n);
+    return rc;
+}
+
+HistoryIO hist_io(wd_sim* s) { return HistoryIO{{s->H[0], s->H[1]}, {s->Hc[0], s->Hc[1]}}; }
+
+void snap_save(wd_sim* s, const wd_sim::Snap& sn) {
+    hipLaunchKernelGGL(k_snap_save, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
+                       s->ctl, s->Tp, s->Tv, hist_io(s), sn.P, sn.V, sn.H, sn.C, s->n);
+}
+
+void snap_load(wd_sim* s, const wd_sim::Snap& sn) {
+    const int gb = grain_blocks(s->n);
+    hipLaunchKernelGGL(k_snap_load, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl, s->parts,
+                       sn.P, sn.V, sn.H, sn.C, s->Tp, s->Tv, s->key, hist_io(s), s->n);
+    s->finalize(gb, kFinBox);
+}
+
+int read_overflow(wd_sim* s, unsigned long long* out) {
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(out, &s->ctl->overflow_lifetime, sizeof(*out), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+}  // namespace
+
+extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
+    WD_CHECK_SIM(s);
+    if (nsteps < 1 || checkpoint_every < 1) return fail(WD_ERR_ARG, "wd_adjoint_record: bad window");
+    if (s->picard() || !s->history()) {
+        return fail(WD_ERR_STATE, "wd_adjoint_record: the adjoint covers semi-implicit Euler with spring_history");
+    }
+    if (s->n == 0) return fail(WD_ERR_STATE, "wd_adjoint_record: no grains");
+    s->drop_adjoint();
+    s->adj = new wd_sim::AdjointState();
+    auto* A = s->adj;
+    A->nsteps = nsteps;
+    A->every = std::min(checkpoint_every, nsteps);
+    A->nseg = (nsteps + A->every - 1) / A->every;
+    const size_t n = static_cast<size_t>(s->n);
+    int rc = WD_OK;
+    A->ckpt.resize(A->nseg + 1);
+    A->seg.resize(A->every);
+    for (auto& sn : A->ckpt) if (rc == WD_OK) rc = adj_snap(s, sn);
+    for (size_t k = 1; k < A->seg.size(); ++k) if (rc == WD_OK) rc = adj_snap(s, A->seg[k]);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lx, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lv, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->own_x, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->own_v, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lxi[0], static_cast<size_t>(kHist) * n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lxi[1], static_cast<size_t>(kHist) * n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->cg_d, static_cast<size_t>(kAdjContacts) * n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->cg_v, static_cast<size_t>(kAdjContacts) * n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->cg_cnt, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->gparam, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->grid_cot, s->cic_count);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->state_cot, 6 * n);
+    if (rc == WD_OK) rc = read_overflow(s, &A->overflow0);
+    if (rc != WD_OK) {
+        s->drop_adjoint();
+        return rc;
+    }
+    snap_save(s, A->ckpt[0]);
+    for (int c = 1; c <= A->nseg; ++c) {
+        rc = s->step(std::min(A->every, nsteps - (c - 1) * A->every));
+        if (rc != WD_OK) return rc;
+        snap_save(s, A->ckpt[c]);
+    }
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float* state_cot,
+                                   float* grad_state0, double* grad_params) {
+    WD_CHECK_SIM(s);
+    auto* A = s->adj;
+    if (!A) return fail(WD_ERR_STATE, "wd_adjoint_backward: no recorded window (wd_adjoint_record)");
+    unsigned long long over = 0;
+    int rc = read_overflow(s, &over);
+    if (rc != WD_OK) return rc;
+    if (over != A->overflow0) {
+        return fail(WD_ERR_STATE, "wd_adjoint_backward: cells over particles_per_cell in the window; "
+                                  "the neighbor lists are not symmetric");
+    }
+    const size_t n = static_cast<size_t>(s->n);
+    if (grid_cot) WD_HIP(hipMemcpy(A->grid_cot, grid_cot, sizeof(float) * s->cic_count, hipMemcpyHostToDevice));
+    if (state_cot) WD_HIP(hipMemcpy(A->state_cot, state_cot, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    WD_HIP(hipMemsetAsync(A->lxi[0], 0, sizeof(float4) * kHist * n, s->stream));
+    WD_HIP(hipMemsetAsync(A->lxi[1], 0, sizeof(float4) * kHist * n, s->stream));
+    WD_HIP(hipMemsetAsync(A->gparam, 0, sizeof(double2) * n, s->stream));
+    const int blocks = static_cast<int>((n + kBlock - 1) / kBlock);
+    const int gb = grain_blocks(s->n);
+    snap_load(s, A->ckpt[A->nseg]);
+    hipLaunchKernelGGL(k_adj_seed, dim3(blocks), dim3(kBlock), 0, s->stream, s->dp, s->Tp, s->Tv,
+                       grid_cot ? A->grid_cot : nullptr, state_cot ? A->state_cot : nullptr, A->lx,
+                       A->lv, s->n);
+    const bool energy = s->energy_on;
+    s->energy_on = false;  // recomputed steps append nothing to the energy log
+    int cur = 0;
+    for (int seg = A->nseg - 1; seg >= 0; --seg) {
+        const int s0 = seg * A->every;
+        const int K = std::min(A->nsteps, s0 + A->every) - s0;
+        // Recompute the segment's states from its checkpoint.
+        snap_load(s, A->ckpt[seg]);
+        for (int k = 1; k < K; ++k) {
+            s->enqueue_step();
+            snap_save(s, A->seg[k]);
+        }
+        for (int k = K - 1; k >= 0; --k) {
+            snap_load(s, k == 0 ? A->ckpt[seg] : A->seg[k]);
+            s->enqueue_bin_current();
+            hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
+                               s->Sp, s->Sv, s->start, hist_io(s), A->lx, A->lv, A->lxi[cur],
+                               A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt,
+                               A->gparam, s->n);
+            hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
+                               A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->lx, A->lv, s->n);
+            cur ^= 1;
+        }
+    }
+    s->energy_on = energy;
+    snap_load(s, A->ckpt[A->nseg]);  // the handle is left at the end of the window
+    WD_HIP(hipGetLastError());
+    rc = s->sync();
+    if (rc != WD_OK) return rc;
+    if (grad_state0) {
+        std::vector<float4> x(n), v(n);
+        WD_HIP(hipMemcpy(x.data(), A->lx, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        WD_HIP(hipMemcpy(v.data(), A->lv, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            float* g = grad_state0 + 6 * i;
+            g[0] = x[i].x; g[1] = x[i].y; g[2] = x[i].z;
+            g[3] = v[i].x; g[4] = v[i].y; g[5] = v[i].z;
+        }
+    }
+    if (grad_params) {
+        std::vector<double2> gp(n);
+        WD_HIP(hipMemcpy(gp.data(), A->gparam, sizeof(double2) * n, hipMemcpyDeviceToHost));
+        for (int m = 0; m < s->members; ++m) {
+            double gmu = 0.0, gk = 0.0;
+            for (int i = s->goff[m]; i < s->goff[m + 1]; ++i) {
+                gmu += gp[i].x;
+                gk += gp[i].y;
+            }
+            grad_params[2 * m] = gmu;
+            grad_params[2 * m + 1] = gk;
+        }
     }
     return WD_OK;
 }

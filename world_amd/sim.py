@@ -232,6 +232,36 @@ class ParticleDynamics:
         n = self.n
         return partner[: n * S].reshape(n, S), xi[: n * S * 3].reshape(n, S, 3), cnt[:n]
 
+    # -- parameters and the adjoint (extension) ----------------------------
+    def set_friction(self, mu: float, member: int = 0) -> None:
+        check(lib().wd_set_friction(self._h, int(member), C.c_float(mu)))
+
+    def set_normal_stiffness(self, k_n: float) -> None:
+        check(lib().wd_set_normal_stiffness(self._h, C.c_float(k_n)))
+
+    def adjoint_record(self, nsteps: int, checkpoint_every: int) -> None:
+        self._push_dt()
+        check(lib().wd_adjoint_record(self._h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, grid_cotangent=None, state_cotangent=None):
+        """dL/d(initial state) [n, 6] and per-member [dL/dmu, dL/dk_n]."""
+        members = getattr(self, "members", 1)
+        g = None
+        if grid_cotangent is not None:
+            g = np.ascontiguousarray(grid_cotangent, np.float32).reshape(-1)
+            if g.size != members * K_CHANNELS * self.density_grid_nodes():
+                raise ValueError("grid cotangent must hold members * 4 * nodes floats")
+        sc = None
+        if state_cotangent is not None:
+            sc = np.ascontiguousarray(state_cotangent, np.float32).reshape(-1)
+            if sc.size != K_STATE_STRIDE * self.n:
+                raise ValueError("state cotangent must hold 6n floats")
+        gs = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
+        gp = np.zeros(2 * members, np.float64)
+        check(lib().wd_adjoint_backward(self._h, None if g is None else ptr(g),
+                                        None if sc is None else ptr(sc), ptr(gs), ptr(gp)))
+        return gs[: K_STATE_STRIDE * self.n].reshape(self.n, K_STATE_STRIDE), gp.reshape(members, 2)
+
     def rhs(self) -> np.ndarray:
         out = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
         check(lib().wd_rhs(self._h, ptr(out)))
