@@ -1,0 +1,50 @@
+"""torch.autograd through the GPU adjoint: the density-loss gradient that
+autograd returns equals the engine's adjoint and the oracle's tangent at the
+FP32 trajectory (1e-4)."""
+import numpy as np
+import pytest
+
+from world_amd import workloads
+from world_amd.adjoint import DensityWindow, density_window
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autograd_density_loss_matches_adjoint_and_oracle(oracle_lib):
+    import torch
+
+    c = workloads.column(8, 8, 8, friction=0.3)
+    win = DensityWindow(c, 60)  # checkpoints every ceil(sqrt(60)) = 8 steps
+    try:
+        assert win.every == 8
+        x_init = win.x0.copy()
+        with torch.no_grad():
+            target = density_window(torch.tensor(x_init), torch.tensor(0.5, dtype=torch.float64),
+                                    torch.tensor(150.0), win)[:, 0].clone()
+        x0 = torch.tensor(x_init, requires_grad=True)
+        mu = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+        k_n = torch.tensor(150.0, requires_grad=True)
+        rho = density_window(x0, mu, k_n, win)
+        loss = ((rho[:, 0] - target) ** 2).sum()
+        loss.backward()
+        assert x0.grad.shape == x0.shape and mu.grad.shape == mu.shape and k_n.grad.shape == k_n.shape
+
+        G = np.zeros((1, 4, win.nodes), np.float32)
+        G[:, 0] = (2.0 * (rho[:, 0] - target)).detach().numpy()
+        fin = win.sim.state()
+        gx, gmu, gk = win.backward(G)
+        np.testing.assert_allclose(x0.grad.numpy(), gx, rtol=1e-6, atol=1e-12)
+        assert float(mu.grad) == pytest.approx(float(gmu[0]), rel=1e-6)
+        assert float(k_n.grad) == pytest.approx(gk, rel=1e-6)
+
+        st0 = np.concatenate([x_init, win.v0], axis=1)
+        R = oracle_lib.Recording(c, st0, np.ones(len(st0), np.int32), 60)
+        np.testing.assert_array_equal(R.final_state(), fin)
+        rng = np.random.default_rng(6)
+        tx = rng.normal(size=x_init.shape)
+        _, _, _, drho = R.replay(66, x_init, win.v0, float(np.float32(0.3)), 150.0, tx, None, 1.0, 10.0)
+        jvp = G.reshape(-1).astype(np.float64) @ drho
+        vjp = (x0.grad.numpy() * tx).sum() + float(mu.grad) + 10.0 * float(k_n.grad)
+        assert vjp == pytest.approx(jvp, rel=1e-4)
+    finally:
+        win.close()

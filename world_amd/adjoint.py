@@ -1,0 +1,92 @@
+"""Density-field gradients through the GPU adjoint, as a torch.autograd.Function.
+
+The function maps (initial positions, friction per member, normal stiffness)
+to the CIC grids at the end of a window of semi-implicit Euler steps with the
+spring_history tangential model.  Its backward is the engine's discrete
+adjoint (wd_adjoint_backward): the exact derivative of the recorded forward
+with its contact sets and stick/slip branches frozen, recomputed from
+checkpoints every K steps (K = ceil(sqrt(window)) by default).
+
+The reference solver has no differentiation; its surrogate side
+(surrogate/) consumes CIC grids written by the dataset driver, which is the
+quantity differentiated here.  Host buffers cross the C ABI; torch tensors
+are plumbing on either side.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .sim import Ensemble, ParticleDynamics
+
+
+class DensityWindow:
+    """A reusable window: one engine handle (single or batched members) whose
+    forward runs `nsteps` steps from given initial positions."""
+
+    def __init__(self, configs, nsteps: int, checkpoint_every: int | None = None):
+        if not isinstance(configs, (list, tuple)):
+            configs = [configs]
+        self.configs = [c.copy() for c in configs]
+        self.nsteps = int(nsteps)
+        self.every = int(checkpoint_every or math.ceil(math.sqrt(self.nsteps)))
+        self.sim = Ensemble(self.configs) if len(self.configs) > 1 else ParticleDynamics(self.configs[0])
+        self.members = len(self.configs)
+        st = self.sim.state()
+        self.v0 = st[:, 3:].copy()
+        self.x0 = st[:, :3].copy()
+        self.nodes = self.sim.density_grid_nodes()
+
+    def close(self) -> None:
+        self.sim.close()
+
+    def run(self, x0: np.ndarray, mu, k_n: float) -> np.ndarray:
+        """Final CIC grids [members, 4, nodes] from initial positions x0 [n, 3]."""
+        mu = np.broadcast_to(np.asarray(mu, np.float64), (self.members,))
+        st = np.concatenate([np.asarray(x0, np.float32).reshape(-1, 3), self.v0], axis=1)
+        for m in range(self.members):
+            self.sim.set_friction(float(mu[m]), m)
+        self.sim.set_normal_stiffness(float(k_n))
+        self.sim.set_state(st)  # also clears the contact history
+        self.sim.adjoint_record(self.nsteps, self.every)
+        if self.members > 1:
+            grids = self.sim.density_grids()
+        else:
+            self.sim.compute_density_grid()
+            grids = self.sim.host_density_grid[None, :].copy()
+        return grids.reshape(self.members, 4, self.nodes)
+
+    def backward(self, grid_cotangent: np.ndarray):
+        """dL/dx0 [n, 3], dL/dmu [members], dL/dk_n for the last run."""
+        gs, gp = self.sim.adjoint_backward(np.asarray(grid_cotangent, np.float32).reshape(-1))
+        return gs[:, :3].copy(), gp[:, 0].copy(), float(gp[:, 1].sum())
+
+
+def density_window_function():
+    """The torch.autograd.Function class (torch is imported on first use)."""
+    import torch
+
+    class DensityWindowFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x0, mu, k_n, window: DensityWindow):
+            grids = window.run(x0.detach().cpu().numpy(), mu.detach().cpu().numpy(),
+                               float(k_n.detach().cpu()))
+            ctx.window = window
+            ctx.meta = [(t.dtype, t.shape, t.device) for t in (x0, mu, k_n)]
+            return torch.from_numpy(grids).to(x0.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            gx, gmu, gk = ctx.window.backward(g.detach().cpu().numpy())
+            out = []
+            for val, (dtype, shape, dev) in zip((gx, gmu, np.array(gk)), ctx.meta):
+                out.append(torch.as_tensor(np.asarray(val)).reshape(shape).to(device=dev, dtype=dtype))
+            return out[0], out[1], out[2], None
+
+    return DensityWindowFunction
+
+
+def density_window(x0, mu, k_n, window: DensityWindow):
+    """Differentiable final CIC grids [members, 4, nodes] of `window`."""
+    return density_window_function().apply(x0, mu, k_n, window)
