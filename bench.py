@@ -8,9 +8,19 @@ total energy recorded every step (fused into the force pass) and a 64^3
 cloud-in-cell deposit every 40 steps (kept on the device).  Grains are
 synthetic (a seeded lattice), state resident in HBM before timing starts.
 
-Also measured, as secondary lines inside the same JSON object: the
-1,048,576-grain column (128 x 128 x 64, 4 x 4 m box, CIC 128^3 every 100
-steps) and the CPU oracle on the host's cores (cpu_baseline).
+Also measured, as secondary entries of the same JSON object:
+  variants  the same column at mu = 0 and with backward-Euler-Picard (3
+            iterations), mu = 0.5 and 0;
+  large     the 1,048,576-grain column (128 x 128 x 64, 4 x 4 m box, CIC 128^3
+            every 100 steps);
+  ensemble  64 uneven members (2,048..32,768 grains, aspect 1..4, mu in
+            [0, 0.6], 50 um Gaussian perturbation) advanced by one batched
+            handle, against looping the same members one at a time;
+  adjoint   the 8,192-grain window (mu 0.3, checkpoints every 20 steps): cost
+            of a backward step against a forward step;
+  sliding   one grain launched at 1 m/s on the floor at mu 0.5: fitted
+            deceleration against -mu g = -4.905 m/s^2;
+  cpu_baseline  the CPU oracle on the host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -35,6 +45,13 @@ B_STEP = 132.0
 # The force kernel performs the force + integrate pass (52 B) and writes the
 # next cell key (4 B).
 B_FORCE = 56.0
+# With the spring_history tangential model the force pass also reads and
+# writes each grain's stored springs: a 4 B count each way and 16 B per
+# contact each way, so 8 + 32 Z bytes per grain for Z contacts per grain.
+
+
+def history_bytes(z: float) -> float:
+    return 8.0 + 32.0 * z
 
 
 def cic_bytes_per_grain_step(n: int, nodes: int, every: int) -> float:
@@ -72,24 +89,156 @@ def run_gpu(cfg, steps: int, warmup: int, cic_every: int, energy: bool, profile_
     elapsed = t1 - t0
     grids = sim.density_log()
     elog = sim.energy_log() if energy else None
-    # Per-kernel durations: HIP events around every launch on the engine's
-    # stream, for profile_steps further steps of the same run.
-    kernels = sim.profile_step(profile_steps)
     stable = sim.is_stable()
     overflow = sim.grid_overflow_count()
+    block = sim.force_block()
+    passes, contacts = sim.contact_stats()
+    z = contacts / max(1, passes * n)
     sim.close()
+    # Per-kernel durations over the same work: the run is deterministic, so a
+    # second handle, with the force block size the first one tuned, replays
+    # the warmup + timed steps with kernel-stamped HIP events on every launch
+    # on the engine's stream (profile_steps of them).
+    kernels = {}
+    if profile_steps > 0:
+        prof = ParticleDynamics(cfg)
+        prof.set_force_block(block)
+        if energy:
+            prof.enable_energy_log(profile_steps + 8)
+        kernels = prof.profile_step(profile_steps)
+        prof.close()
     return {
         "n": n, "elapsed": elapsed, "kernels_ms": kernels, "stable": stable,
-        "overflow": overflow, "grids": grids, "energy": elog,
+        "overflow": overflow, "grids": grids, "energy": elog, "force_block": block,
+        "contacts_per_grain": z, "history": cfg.tangential_model == 1,
     }
 
 
-def roofline_of(kernels_ms: dict, n: int) -> dict:
-    ms = kernels_ms.get("k_force", 0.0)
-    achieved = B_FORCE * n / (ms * 1e-3) if ms > 0 else 0.0
+def roofline_of(run: dict) -> dict:
+    n = run["n"]
+    ms = run["kernels_ms"].get("k_force", 0.0)
+    bpg = B_FORCE + (history_bytes(run["contacts_per_grain"]) if run["history"] else 0.0)
+    achieved = bpg * n / (ms * 1e-3) if ms > 0 else 0.0
     return {"bound": "hbm", "kernel": "k_force", "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-            "traffic": None, "bytes_per_grain": B_FORCE, "avg_launch_ms": ms}
+            "traffic": traffic_of("k_force", n), "bytes_per_grain": bpg,
+            "contacts_per_grain": run["contacts_per_grain"], "avg_launch_ms": ms}
+
+
+def step_bytes(run: dict, nodes: int, every: int) -> float:
+    b = B_STEP + cic_bytes_per_grain_step(run["n"], nodes, every)
+    return b + (history_bytes(run["contacts_per_grain"]) if run["history"] else 0.0)
+
+
+def ensemble_leg(steps: int) -> dict:
+    """64-member uneven ensemble: one batched handle vs the members looped."""
+    from world_amd import _capi, workloads
+    from world_amd.sim import Ensemble, ParticleDynamics
+
+    cs = workloads.uneven_ensemble(64)
+    n = sum(_capi.grain_count(c) for c in cs)
+    ens = Ensemble(cs)
+    ens.take_step(20)
+    ens.sync()
+    t = time.perf_counter()
+    ens.take_step(steps)
+    ens.sync()
+    tb = time.perf_counter() - t
+    stable = ens.is_stable()
+    ens.close()
+    tl = 0.0
+    for c in cs:
+        sim = ParticleDynamics(c)
+        sim.take_step(20)
+        sim.sync()
+        t = time.perf_counter()
+        sim.take_step(steps)
+        sim.sync()
+        tl += time.perf_counter() - t
+        sim.close()
+    return {"workload": "uneven_ensemble_64", "members": len(cs), "grains": n, "steps": steps,
+            "batched_ms_per_step": 1e3 * tb / steps, "batched_grain_steps_per_s": n * steps / tb,
+            "looped_ms_per_step": 1e3 * tl / steps, "looped_grain_steps_per_s": n * steps / tl,
+            "speedup": tl / tb, "stable": stable}
+
+
+def adjoint_leg(window: int = 200, every: int = 20) -> dict:
+    """Backward-step cost of the 8,192-grain density-loss window against a
+    forward step of the same handle."""
+    import numpy as np
+
+    from world_amd import workloads
+    from world_amd.sim import ParticleDynamics
+
+    c = workloads.column_8k(friction=0.3)
+    sim = ParticleDynamics(c)
+    sim.take_step(20)
+    sim.sync()
+    t = time.perf_counter()
+    sim.take_step(window)
+    sim.sync()
+    fwd = (time.perf_counter() - t) / window
+    G = np.random.default_rng(0).normal(size=4 * sim.density_grid_nodes()).astype(np.float32)
+    rec = bwd = 0.0
+    for _ in range(2):  # the first round allocates the checkpoints
+        t = time.perf_counter()
+        sim.adjoint_record(window, every)
+        sim.sync()
+        rec = (time.perf_counter() - t) / window
+        t = time.perf_counter()
+        sim.adjoint_backward(G)
+        bwd = (time.perf_counter() - t) / window
+    sim.close()
+    return {"workload": "column_8k_window", "grains": 8192, "window_steps": window,
+            "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd,
+            "record_ms_per_step": 1e3 * rec, "backward_ms_per_step": 1e3 * bwd,
+            "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd}
+
+
+def sliding_leg() -> dict:
+    import numpy as np
+
+    from world_amd import workloads
+    from world_amd.sim import ParticleDynamics
+
+    c = workloads.sliding_grain(friction=0.5)
+    sim = ParticleDynamics(c)
+    t, vx = [], []
+    for k in range(180):
+        sim.take_step(100)
+        t.append((k + 1) * 100 * c.dt)
+        vx.append(float(sim.state()[0, 3]))
+    sim.close()
+    t, vx = np.array(t), np.array(vx)
+    sel = vx > 0.1  # fit where the grain still slides
+    decel = float(np.polyfit(t[sel], vx[sel], 1)[0])
+    return {"deceleration": decel, "analytic": -0.5 * 9.81, "rel_error": abs(decel / (-0.5 * 9.81) - 1)}
+
+
+def copy_probe(mib: int = 256) -> None:
+    """One device copy of known size, the calibration dispatch that HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) are scaled by."""
+    import torch
+
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)  # one elementwise kernel: reads and writes `mib` MiB
+    torch.cuda.synchronize()
+
+
+def traffic_of(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` at n grains, from the committed
+    per-grain counter summary (profiles/hbm_traffic.json), or None."""
+    p = ROOT / "profiles" / "hbm_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        per = json.loads(p.read_text())["bytes_per_grain"][kernel]
+    except (KeyError, ValueError):
+        return None
+    best = min(per, key=lambda e: abs(e["grains"] - n))
+    return best["bytes_per_grain"] * n if abs(best["grains"] - n) <= 0.05 * n else None
 
 
 def cpu_baseline(cfg, target_s: float = 12.0) -> dict:
@@ -120,7 +269,14 @@ def main() -> int:
     ap.add_argument("--large-steps", type=int, default=500,
                     help="steps of the 1,048,576-grain column (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--profile-steps", type=int, default=-1,
+                    help="steps replayed with HIP events around every kernel for the "
+                         "roofline (default: the warmup + timed steps of each workload)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the variants, ensemble, adjoint and sliding legs")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -131,12 +287,15 @@ def main() -> int:
 
     from world_amd import workloads
 
+    if args.copy_probe:
+        copy_probe()
     cfg = workloads.column_collapse_32k(friction=0.5)
-    main_run = run_gpu(cfg, args.steps, args.warmup, 40, True, args.profile_steps)
+    prof_main = args.warmup + args.steps if args.profile_steps < 0 else args.profile_steps
+    main_run = run_gpu(cfg, args.steps, args.warmup, 40, True, prof_main)
     n = main_run["n"]
     value = n * args.steps / main_run["elapsed"]
     nodes = 64 ** 3
-    b_total = B_STEP + cic_bytes_per_grain_step(n, nodes, 40)
+    b_total = step_bytes(main_run, nodes, 40)
     out = {
         "metric": "grain-steps/s (32,768-grain column collapse, SIE, mu 0.5, energy every step, "
                   "64^3 CIC every 40 steps)",
@@ -154,27 +313,42 @@ def main() -> int:
         "config": {"workload": "column_collapse_32k", "grains": n, "integrator": "semi_implicit_euler",
                    "friction": 0.5, "dt": 1e-3, "cic_nodes": "64^3", "cic_every": 40,
                    "energy_every": 1, "parallelism": "single_gpu"},
-        "roofline": roofline_of(main_run["kernels_ms"], n),
+        "roofline": roofline_of(main_run),
         "step_roofline": {"bytes_per_grain_step": b_total,
                           "achieved_GBps": value * b_total / 1e9,
                           "frac": value * b_total / HBM_PEAK},
         "kernels_ms": main_run["kernels_ms"],
+        "force_block": main_run["force_block"],
         "stable": main_run["stable"],
         "overflow_cells": main_run["overflow"],
     }
     if args.large_steps > 0:
         big = workloads.column_collapse_1m()
-        lr = run_gpu(big, args.large_steps, 10, 100, False, 20)
+        prof_big = 10 + args.large_steps if args.profile_steps < 0 else args.profile_steps
+        lr = run_gpu(big, args.large_steps, 10, 100, False, prof_big)
         lv = lr["n"] * args.large_steps / lr["elapsed"]
-        lb = B_STEP + cic_bytes_per_grain_step(lr["n"], 128 ** 3, 100)
+        lb = step_bytes(lr, 128 ** 3, 100)
         out["large"] = {
             "workload": "column_collapse_1m", "grains": lr["n"], "steps": args.large_steps,
             "value": lv, "unit": "grain-steps/s", "ms_per_step": 1e3 * lr["elapsed"] / args.large_steps,
             "step_roofline": {"bytes_per_grain_step": lb, "achieved_GBps": lv * lb / 1e9,
                               "frac": lv * lb / HBM_PEAK},
-            "roofline": roofline_of(lr["kernels_ms"], lr["n"]),
-            "kernels_ms": lr["kernels_ms"], "stable": lr["stable"], "overflow_cells": lr["overflow"],
+            "roofline": roofline_of(lr),
+            "kernels_ms": lr["kernels_ms"], "force_block": lr["force_block"], "stable": lr["stable"],
+            "overflow_cells": lr["overflow"],
         }
+    if not args.no_extras:
+        variants = {}
+        for name, mu, picard in (("sie_mu0", 0.0, False), ("picard3_mu0.5", 0.5, True),
+                                 ("picard3_mu0", 0.0, True)):
+            vr = run_gpu(workloads.column_collapse_32k(friction=mu, picard=picard), args.steps,
+                         args.warmup, 40, True, 10)
+            variants[name] = {"value": vr["n"] * args.steps / vr["elapsed"], "unit": "grain-steps/s",
+                              "ms_per_step": 1e3 * vr["elapsed"] / args.steps, "stable": vr["stable"]}
+        out["variants"] = variants
+        out["ensemble"] = ensemble_leg(args.ensemble_steps)
+        out["adjoint"] = adjoint_leg()
+        out["sliding"] = sliding_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

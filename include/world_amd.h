@@ -291,6 +291,21 @@ int wd_adjoint_record(wd_sim* sim, int nsteps, int checkpoint_every);
 int wd_adjoint_backward(wd_sim* sim, const float* grid_cotangent, const float* state_cotangent,
                         float* grad_state0, double* grad_params);
 
+/* ---- launch configuration (extension) -----------------------------------
+ * The force pass runs with 64, 128 or 256 grains per block.  By default the
+ * first wd_step of at least nine steps tunes it online (three round-robin
+ * rounds over the candidates, timed on the engine's stream); the results are
+ * bit-identical for every choice.  The reference fixes threads_per_block from the config
+ * (src/kernel.cu, blocks = ceil(n / threads_per_block)). */
+int wd_set_force_block(wd_sim* sim, int block); /* 0: autotune, or 64 / 128 / 256 */
+int wd_get_force_block(const wd_sim* sim, int* block_out); /* 0 until tuned */
+
+/* Completed force passes and the contacts (wall and pair, counted from each
+ * side) summed over them, since construction: contacts / (passes * n) is the
+ * mean contact count per grain that the force pass's history traffic scales
+ * with.  Extension; no counterpart in the reference. */
+int wd_contact_stats(wd_sim* sim, unsigned long long* passes_out, unsigned long long* contacts_out);
+
 /* ---- parity / inspection entry points ---------------------------------- */
 
 /* Extension, tangential_model spring_history: the stored contact springs in

@@ -108,7 +108,7 @@ def test_graph_replay_equals_eager_launches(col4k):
     eager = col4k.copy()
     eager.kernel_timing = 1
     with ParticleDynamics(col4k) as a, ParticleDynamics(eager) as b:
-        a.take_step(37)  # two 16-step graph chunks + 5 single-step replays
+        a.take_step(37)  # nine eager tuning steps, one 16-step graph chunk, 12 single-step replays
         for _ in range(37):
             b.take_step()
         np.testing.assert_array_equal(a.state(), b.state())
@@ -234,3 +234,45 @@ def test_sliding_grain_on_gpu_decelerates_at_mu_g():
     t, vx = np.array(t), np.array(vx)
     sel = vx > 0.1
     assert np.polyfit(t[sel], vx[sel], 1)[0] == pytest.approx(-0.5 * 9.81, rel=0.01)
+
+
+@pytest.mark.parametrize("picard", [False, True])
+def test_results_do_not_depend_on_the_force_block_size(picard):
+    """64, 128 and 256 grains per force block, and the autotuned choice, give
+    bit-identical states, contact springs and energy logs."""
+    c = workloads.column_collapse_4k(friction=0.4)
+    if picard:
+        c.time_integrator = _capi.WD_BACKWARD_EULER_PICARD
+    st, _ = jittered(c, seed=8)
+    out = {}
+    for block in (64, 128, 256, 0):
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.set_force_block(block)
+            sim.enable_energy_log(64)
+            sim.take_step(3)
+            sim.take_step(37)
+            assert sim.force_block() in (64, 128, 256)
+            out[block] = (sim.state(), sim.history(), sim.energy_log(), sim.compute_total_energy())
+    ref = out[256]
+    for block, got in out.items():
+        np.testing.assert_array_equal(got[0], ref[0], err_msg=str(block))
+        for a, b in zip(got[1], ref[1]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(got[2], ref[2])
+        assert got[3] == ref[3]
+
+
+def test_contact_count_matches_oracle(oracle_lib):
+    """The force pass's contact counter (walls and pairs, each side) equals the
+    contacts the oracle's FP32 forward takes, step by step."""
+    c = workloads.column_collapse_4k(friction=0.3)
+    st, mat = jittered(c, sigma_x=0.002, sigma_v=0.3, seed=13)
+    R = oracle_lib.Recording(c, st, mat, 30)
+    want = sum(int(R.contacts(k)[0].sum()) for k in range(30))
+    with ParticleDynamics(c) as sim:
+        sim.set_state(st)
+        sim.take_step(30)
+        passes, contacts = sim.contact_stats()
+        np.testing.assert_array_equal(sim.state(), R.final_state())
+    assert passes == 30 and contacts == want

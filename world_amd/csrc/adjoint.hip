@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_snap_save(const Ctl* __restrict__ ct
                                                       float4* __restrict__ dV,
                                                       float4* __restrict__ dH, int* __restrict__ dC,
                                                       int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // T position
     if (i >= n) return;
     const float4 p = Tp[i];
     const int id = __float_as_int(p.w);
@@ -185,12 +185,13 @@ __global__ __launch_bounds__(kBlock) void k_snap_save(const Ctl* __restrict__ ct
     dV[id] = Tv[i];
     const int par = ctl->hist_parity;
     const int c = hist.cnt[par][i];
-    dC[i] = c;
-    for (int t = 0; t < c; ++t) dH[static_cast<size_t>(t) * n + i] = hist.buf[par][static_cast<size_t>(t) * n + i];
+    dC[id] = c;
+    for (int t = 0; t < c; ++t) dH[static_cast<size_t>(t) * n + id] = hist.buf[par][static_cast<size_t>(t) * n + i];
 }
 
 // Restores a snapshot: T = original order, keys and block partials for the
-// next bin, springs into the current generation.
+// next bin, springs into the current generation (whose index is then the
+// original id).
 __global__ __launch_bounds__(kBlock) void k_snap_load(const Params* __restrict__ Pp,
                                                       const Ctl* __restrict__ ctl,
                                                       BlockPart* __restrict__ parts,
@@ -304,7 +305,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ 
 // Pass A of the adjoint of one SIE step (see the file comment).  Grains in
 // cell order (Sp/Sv, start: the bin of state s); h_in = the springs the
 // forward step read; lxi_next[k][id] = dL/d(spring slot k written by the
-// step); lxi_prev[t][id] receives dL/d(spring slot t read by the step).
+// step); lxi_prev[t][id] receives dL/d(spring slot t read by the step).  The
+// cotangent lists are indexed by original id.
 __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const uint32_t* __restrict__ start, HistoryIO hist,
@@ -338,14 +340,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
 
     const int par = ctl->hist_parity;
     const float4* __restrict__ h_in = hist.buf[par];
-    const int h_old = hist.cnt[par][id];
+    const int hs = hist.hidx[p];  // the grain's index in generation par
+    const int h_old = hist.cnt[par][hs];
     for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(0.f, 0.f, 0.f, 0.f);
     int h_new = 0, kc = 0;
     // Stored spring of `partner` and its slot (-1: a new contact, zero spring).
     auto h_find = [&](int partner, float xi[3]) __attribute__((always_inline)) {
         xi[0] = xi[1] = xi[2] = 0.f;
         for (int t = 0; t < h_old; ++t) {
-            const float4 h = h_in[static_cast<size_t>(t) * n + id];
+            const float4 h = h_in[static_cast<size_t>(t) * n + hs];
             if (__float_as_int(h.w) == partner) {
                 xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
                 return t;

@@ -5,6 +5,7 @@
 // object (src/particle_dynamics.cu:10-360); the step pipeline is the one
 // described in kernels.hip.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -63,8 +64,10 @@ struct wd_sim {
     float4 *Tp = nullptr, *Tv = nullptr, *Sp = nullptr, *Sv = nullptr;
     float4* NP[2] = {nullptr, nullptr};
     float4* NV[2] = {nullptr, nullptr};
-    float4* H[2] = {nullptr, nullptr};  // contact-history generations [kHist][n], by original id
+    float4* H[2] = {nullptr, nullptr};  // contact-history generations [kHist][n], by T position
     int* Hc[2] = {nullptr, nullptr};
+    int* hidx[2] = {nullptr, nullptr};  // spring index of each cell-sorted grain (two Picard buffers)
+    int hidx_cur = 0;                   // hidx buffer the last bin wrote
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
     int* tmp_id = nullptr;
     unsigned long long* status = nullptr;
@@ -84,6 +87,9 @@ struct wd_sim {
     int nblocks = 0;
     BlockPart* parts = nullptr;
     double* energy_log = nullptr;
+    double* ewave = nullptr;      // per-64-grain energy sums of the last energy pass
+    int force_bs = 0;             // force-pass block size; 0 until the autotuner has chosen
+    bool force_bs_fixed = false;  // set by wd_set_force_block
     int energy_cap = 0;
     bool energy_on = false;
     unsigned long long* cic_acc = nullptr;
@@ -149,7 +155,8 @@ struct wd_sim {
         void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], H[0], H[1], Hc[0],
                         Hc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
-                        cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale};
+                        cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
+                        hidx[0], hidx[1]};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -157,8 +164,9 @@ struct wd_sim {
         NP[0] = NP[1] = NV[0] = NV[1] = nullptr;
         H[0] = H[1] = nullptr;
         Hc[0] = Hc[1] = nullptr;
+        hidx[0] = hidx[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
-        st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr;
+        st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr; ewave = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
         d_mscale = nullptr;
@@ -206,17 +214,18 @@ struct wd_sim {
         pending.clear();
     }
 
-    // Runs one launch, bracketed by an event pair when profiling.
-    template <typename F>
-    void timed(const char* name, F&& launch) {
+    // One launch on the engine's stream.  When profiling, the launch carries
+    // a start / stop event pair that the runtime stamps at the kernel's own
+    // start and end (hipExtLaunchKernelGGL), so the pair times the kernel
+    // alone, without the dispatch gap around it.
+    template <typename K, typename... A>
+    void go(const char* name, K kernel, dim3 grid, dim3 block, A... args) {
         if (!prof) {
-            launch();
+            hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
             return;
         }
         hipEvent_t a = take_event(), b = take_event();
-        (void)hipEventRecord(a, stream);
-        launch();
-        (void)hipEventRecord(b, stream);
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, a, b, 0, args...);
         prof->push_back({name, a, b});
     }
 
@@ -225,49 +234,53 @@ struct wd_sim {
                      float4* nout_v) {
         const int gb = grain_blocks(n);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
-        timed("k_count", [&] {
-            hipLaunchKernelGGL(k_count, dim3(gb), dim3(kBlock), 0, stream, dp, ctl, Tv, key, slot, count,
-                               n);
-        });
-        timed("k_scan", [&] {
-            hipLaunchKernelGGL(k_scan, dim3(scan_grid), dim3(kScanThreads), 0, stream, dp, ctl,
-                               count, start, status);
-        });
-        timed("k_scatter", [&] {
-            hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kBlock), 0, stream, dp, key, slot, Tp, Tv,
-                               start, tmp_id, n);
-        });
-        const Carry carry{nin_p, nin_v, nout_p, nout_v};
-        timed("k_reorder", [&] {
-            if (nmode == 0) {
-                hipLaunchKernelGGL(k_reorder<0>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
-            } else if (nmode == 1) {
-                hipLaunchKernelGGL(k_reorder<1>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
-            } else {
-                hipLaunchKernelGGL(k_reorder<2>, dim3(gb), dim3(kBlock), 0, stream, dp, key, start,
-                                   tmp_id, Tp, Tv, Sp, Sv, carry, n);
-            }
-        });
+        go("k_count", k_count, dim3(gb), dim3(kBlock), (const Params*)dp, ctl, (const float4*)Tv, key,
+           slot, count, n);
+        go("k_scan", k_scan, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
+           status);
+        go("k_scatter", k_scatter, dim3(gb), dim3(kBlock), (const Params*)dp, (const uint32_t*)key,
+           (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
+        // Picard iterations after the first carry the spring index of the
+        // step's first bin through every re-sort (nmode 2).
+        const int hout = nmode == 2 ? hidx_cur ^ 1 : 0;
+        const Carry carry{nin_p, nin_v, nout_p, nout_v, hidx[hidx_cur], history() ? hidx[hout] : nullptr};
+        hidx_cur = hout;
+        auto reorder = [&](auto kern) {
+            go("k_reorder", kern, dim3(gb), dim3(kBlock), (const Params*)dp, (const uint32_t*)key,
+               (const uint32_t*)start, (const int*)tmp_id, (const float4*)Tp, (const float4*)Tv, Sp, Sv,
+               carry, n);
+        };
+        if (nmode == 0) reorder(k_reorder<0>);
+        else if (nmode == 1) reorder(k_reorder<1>);
+        else reorder(k_reorder<2>);
     }
 
     void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr); }
 
+    template <int MODE, bool ENERGY, int BS>
+    void launch_force_bs(const float4* np, const float4* nv, float* rhs) {
+        const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}, hidx[hidx_cur]};
+        auto force = [&](auto kern) {
+            go("k_force", kern, dim3(grain_blocks(n, BS)), dim3(BS), (const Params*)dp, ctl,
+               (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, rhs,
+               parts, ewave, hio, n);
+        };
+        if (history()) force(k_force<MODE, ENERGY, true, BS>);
+        else force(k_force<MODE, ENERGY, false, BS>);
+    }
+
     // last_pass: the pass completes a step (its springs become current).
     template <int MODE, bool ENERGY>
     void launch_force(const float4* np, const float4* nv, float* rhs, bool last_pass = true) {
-        const int gb = grain_blocks(n);
-        const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}};
-        timed("k_force", [&] {
-            if (history()) {
-                hipLaunchKernelGGL((k_force<MODE, ENERGY, true>), dim3(gb), dim3(kBlock), 0, stream,
-                                   dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, hio, n);
-            } else {
-                hipLaunchKernelGGL((k_force<MODE, ENERGY, false>), dim3(gb), dim3(kBlock), 0, stream,
-                                   dp, ctl, Sp, Sv, np, nv, start, Tp, Tv, key, rhs, parts, hio, n);
-            }
-        });
+        const int bs = (MODE == kModeRHS || force_bs == 0) ? kBlock : force_bs;
+        const int gb = grain_blocks(n, bs);
+        if constexpr (MODE == kModeRHS) {
+            launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
+        } else {
+            if (bs == 64) launch_force_bs<MODE, ENERGY, 64>(np, nv, rhs);
+            else if (bs == 128) launch_force_bs<MODE, ENERGY, 128>(np, nv, rhs);
+            else launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
+        }
         int flags = 0;
         if (MODE != kModeRHS) {
             flags = kFinBox | kFinAccel | (ENERGY ? kFinEnergyLog : 0);
@@ -275,37 +288,30 @@ struct wd_sim {
         } else if (ENERGY) {
             flags = kFinEnergyLast;
         }
-        if (flags) finalize(gb, flags);
+        if (flags) finalize(gb, flags, bs);
     }
 
-    void finalize(int nparts, int flags) {
-        timed("k_finalize", [&] {
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeBlock), 0, stream, dp, ctl, parts, nparts,
-                               key, Tp, Tv, energy_log, flags);
-        });
+    // gpp: grains per block partial of the pass being reduced.
+    void finalize(int nparts, int flags, int gpp = kBlock) {
+        go("k_finalize", k_finalize, dim3(1), dim3(kFinalizeBlock), (const Params*)dp, ctl,
+           (const BlockPart*)parts, nparts, gpp, (const uint32_t*)key, (const float4*)Tp,
+           (const float4*)Tv, (const double*)ewave, energy_log, flags);
     }
 
     // CIC deposit of the current state into `out` (4 * nodes floats), queued.
     void enqueue_cic(float* out) {
         const int zb = static_cast<int>(std::min<size_t>((cic_count + kBlock - 1) / kBlock, 4096));
-        timed("k_cic_zero", [&] {
-            hipLaunchKernelGGL(k_cic_zero, dim3(zb), dim3(kBlock), 0, stream, cic_acc, cic_count);
-        });
+        go("k_cic_zero", k_cic_zero, dim3(zb), dim3(kBlock), cic_acc, cic_count);
         if (n > 0) {
             const int sb = (n + kBlock - 1) / kBlock;
-            timed("k_cic_scale", [&] {
-                hipLaunchKernelGGL(k_cic_scale, dim3(sb), dim3(kBlock), 0, stream, dp, parts, Tp, Tv, n);
-            });
+            go("k_cic_scale", k_cic_scale, dim3(sb), dim3(kBlock), (const Params*)dp, parts,
+               (const float4*)Tp, (const float4*)Tv, n);
             finalize(sb, kFinCicScale);
-            timed("k_cic_deposit", [&] {
-                hipLaunchKernelGGL(k_cic_deposit, dim3(grain_blocks(n)), dim3(kBlock), 0, stream, dp,
-                                   Tp, Tv, cic_acc, n);
-            });
+            go("k_cic_deposit", k_cic_deposit, dim3(grain_blocks(n)), dim3(kBlock), (const Params*)dp,
+               (const float4*)Tp, (const float4*)Tv, cic_acc, n);
         }
-        timed("k_cic_convert", [&] {
-            hipLaunchKernelGGL(k_cic_convert, dim3(zb), dim3(kBlock), 0, stream, dp, cic_acc, out,
-                               cic_count);
-        });
+        go("k_cic_convert", k_cic_convert, dim3(zb), dim3(kBlock), (const Params*)dp,
+           (const unsigned long long*)cic_acc, out, cic_count);
     }
 
     void enqueue_step() {
@@ -354,7 +360,54 @@ struct wd_sim {
         return WD_OK;
     }
 
+    // Online autotuning of the force pass's block size: the first steps run
+    // eagerly, cycling through the candidates round-robin (so that the
+    // evolving state biases none of them), with the force kernels timed by
+    // kernel-stamped events; the first round warms up, the fastest of the
+    // rest is kept.  These are real steps of the simulation (the results do
+    // not depend on the block size).
+    int autotune(int& nsteps) {
+        if (force_bs != 0 || n == 0) return WD_OK;
+        constexpr int kRounds = 3;
+        constexpr int kCand = static_cast<int>(sizeof(kForceBlocks) / sizeof(kForceBlocks[0]));
+        if (nsteps < kRounds * kCand) return WD_OK;  // tune on a longer call
+        double best_ms[kCand];
+        for (int c = 0; c < kCand; ++c) best_ms[c] = 1e30;
+        for (int r = 0; r < kRounds; ++r) {
+            for (int c = 0; c < kCand; ++c) {
+                force_bs = kForceBlocks[c];
+                std::vector<ProfRec> recs;
+                prof = &recs;
+                enqueue_step();
+                prof = nullptr;
+                WD_HIP(hipStreamSynchronize(stream));
+                double t = 0.0;
+                for (auto& rec : recs) {
+                    float e = 0.f;
+                    if (std::strcmp(rec.name, "k_force") == 0 &&
+                        hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) {
+                        t += e;
+                    }
+                    spare.push_back(rec.a);
+                    spare.push_back(rec.b);
+                }
+                if (r > 0) best_ms[c] = std::min(best_ms[c], t);
+                time += hp.dt;
+                --nsteps;
+            }
+        }
+        int best = 0;
+        for (int c = 1; c < kCand; ++c) if (best_ms[c] < best_ms[best]) best = c;
+        force_bs = kForceBlocks[best];
+        drop_graphs();
+        return check_device();
+    }
+
     int step(int nsteps) {
+        if (!force_bs_fixed) {
+            const int rc = autotune(nsteps);
+            if (rc != WD_OK) return rc;
+        }
         if (n > 0) {
             if (timing) {
                 for (int s = 0; s < nsteps; ++s) enqueue_step();
@@ -519,7 +572,11 @@ int allocate(wd_sim* s) {
     }
     if (s->history()) {
         const size_t hs = static_cast<size_t>(kHist) * n;
-        for (int b = 0; b < 2; ++b) { WD_TRY(dalloc(&s->H[b], hs)); WD_TRY(dalloc(&s->Hc[b], n)); }
+        for (int b = 0; b < 2; ++b) {
+            WD_TRY(dalloc(&s->H[b], hs));
+            WD_TRY(dalloc(&s->Hc[b], n));
+            WD_TRY(dalloc(&s->hidx[b], n));
+        }
     }
     WD_TRY(dalloc(&s->key, n)); WD_TRY(dalloc(&s->slot, n)); WD_TRY(dalloc(&s->tmp_id, n));
     const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
@@ -532,8 +589,9 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->status, s->ntiles_max));
     WD_TRY(dalloc(&s->st6, static_cast<size_t>(6) * n));
     WD_TRY(dalloc(&s->mat_d, n));
-    s->nblocks = grain_blocks(n);
+    s->nblocks = grain_blocks(n, kForceBlocks[0]);  // the most partials any pass writes
     WD_TRY(dalloc(&s->parts, s->nblocks));
+    WD_TRY(dalloc(&s->ewave, static_cast<size_t>((n + 63) / 64)));
     s->nodes = static_cast<size_t>(c.density_grid_size_x) * c.density_grid_size_y * c.density_grid_size_z;
     s->cic_count = static_cast<size_t>(4) * s->nodes * static_cast<size_t>(s->members);
     const int E = s->members;
@@ -1059,7 +1117,7 @@ extern "C" int wd_get_history(wd_sim* s, int* partner, float* xi, int* count) {
         }
         if (rc == WD_OK) {
             hipLaunchKernelGGL(k_export_history, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                               s->stream, s->H[par], s->Hc[par], dp_, dx, dc, s->n);
+                               s->stream, s->Tp, s->H[par], s->Hc[par], dp_, dx, dc, s->n);
             rc = s->sync();
         }
     }
@@ -1179,7 +1237,9 @@ int adj_snap(wd_sim* s, wd_sim::Snap& sn) {
     return rc;
 }
 
-HistoryIO hist_io(wd_sim* s) { return HistoryIO{{s->H[0], s->H[1]}, {s->Hc[0], s->Hc[1]}}; }
+HistoryIO hist_io(wd_sim* s) {
+    return HistoryIO{{s->H[0], s->H[1]}, {s->Hc[0], s->Hc[1]}, s->hidx[s->hidx_cur]};
+}
 
 void snap_save(wd_sim* s, const wd_sim::Snap& sn) {
     hipLaunchKernelGGL(k_snap_save, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
@@ -1323,5 +1383,36 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
             grad_params[2 * m + 1] = gk;
         }
     }
+    return WD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch configuration (extension).
+
+extern "C" int wd_set_force_block(wd_sim* s, int block) {
+    WD_CHECK_SIM(s);
+    if (block != 0 && block != 64 && block != 128 && block != 256) {
+        return fail(WD_ERR_ARG, "wd_set_force_block: block must be 0 (autotune), 64, 128 or 256");
+    }
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->force_bs = block;
+    s->force_bs_fixed = block != 0;
+    s->drop_graphs();
+    return WD_OK;
+}
+
+extern "C" int wd_get_force_block(const wd_sim* s, int* block) {
+    WD_CHECK_SIM(s);
+    if (!block) return fail(WD_ERR_ARG, "wd_get_force_block: null output");
+    *block = s->force_bs;
+    return WD_OK;
+}
+
+extern "C" int wd_contact_stats(wd_sim* s, unsigned long long* passes, unsigned long long* contacts) {
+    WD_CHECK_SIM(s);
+    if (!passes || !contacts) return fail(WD_ERR_ARG, "wd_contact_stats: null output");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(passes, &s->ctl->force_passes, sizeof(*passes), hipMemcpyDeviceToHost));
+    WD_HIP(hipMemcpy(contacts, &s->ctl->contact_sum, sizeof(*contacts), hipMemcpyDeviceToHost));
     return WD_OK;
 }

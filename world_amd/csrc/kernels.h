@@ -14,7 +14,7 @@ constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 constexpr int kFinalizeBlock = 1024;  // one block; one wave per ensemble member at a time
-constexpr int kContactList = 16;     // touching candidates buffered per grain in LDS
+constexpr int kContactList = 24;     // touching candidates buffered per grain in LDS
 
 // Small device-side control block: counters, the occupied-cell bounding box
 // and per-pass results.  Only k_count (block 0) and k_finalize write it
@@ -29,6 +29,8 @@ struct Ctl {
     int energy_capacity;
     int hist_parity;           // current contact-history generation; flipped by k_finalize
     double energy_last;        // last standalone energy
+    unsigned long long force_passes;  // completed force passes (k_finalize)
+    unsigned long long contact_sum;   // contacts (walls + pairs) summed over those passes
 };
 
 // Per-block partial results of a grain pass, reduced by k_finalize.  A block
@@ -41,8 +43,8 @@ struct BlockPart {
     int box0[6];      // min cx, cy, cz / max cx, cy, cz of member m0's grains
     int box1[6];      // the same for member m1 (when m1 != m0)
     unsigned accel;   // float bits of the block's max |a| (or deposit magnitude)
-    unsigned pad;
-    double energy;    // deterministic block sum of per-grain energies
+    unsigned contacts;  // contacts (walls + pairs) of the block's grains
+    double energy;    // unused (energies are summed per wave, see k_force)
 };
 
 // What k_finalize publishes.
@@ -58,10 +60,14 @@ enum FinalizeFlags {
 // Launch geometry helpers: grid rounded up to a multiple of the XCD count,
 // logical block b maps to a contiguous slab so neighbouring grains share an
 // XCD's L2.
-inline int grain_blocks(int n) {
-    int b = (n + kBlock - 1) / kBlock;
+inline int grain_blocks(int n, int bs = kBlock) {
+    int b = (n + bs - 1) / bs;
     return ((b + kXcds - 1) / kXcds) * kXcds;
 }
+
+// Force-pass block sizes the autotuner chooses from (grains per block; the
+// results do not depend on the choice).
+constexpr int kForceBlocks[3] = {64, 128, 256};
 
 __device__ __forceinline__ int logical_block() {
     const int per = gridDim.x / kXcds;
@@ -70,21 +76,26 @@ __device__ __forceinline__ int logical_block() {
 
 enum ForceMode { kModeSIE = 0, kModeBEP = 1, kModeRHS = 2 };
 
-// Contact-history buffers (slot-major [kHist][n], indexed by ORIGINAL grain
-// id, so they never move with the cell order).  Two generations: the force
-// pass reads generation ctl->hist_parity and writes the other one;
-// k_finalize flips the parity once a step is complete.
+// Contact-history buffers (slot-major [kHist][n]).  Two generations: the
+// force pass reads generation ctl->hist_parity and writes the other one;
+// k_finalize flips the parity once a step is complete.  A generation is
+// indexed by the T position of its grain (the cell-sorted position at which
+// the pass that wrote it found the grain), so that consecutive grains of the
+// next pass read and write consecutive entries; hidx[p] (written by
+// k_reorder) is the index of cell-sorted grain p in the generation it reads.
 struct HistoryIO {
     float4* buf[2];
     int* cnt[2];
+    const int* hidx;
 };
 
-// What k_reorder carries along the permutation besides the grain.
 struct Carry {
     const float4* nin_p;  // Picard base state (NMODE 2)
     const float4* nin_v;
     float4* nout_p;       // Picard base state out (NMODE 1, 2)
     float4* nout_v;
+    const int* hidx_in;   // spring index of T grain i (NMODE 2; else i itself)
+    int* hidx_out;        // spring index of cell-sorted grain p (null: no springs)
 };
 
 }  // namespace wd

@@ -76,9 +76,10 @@ __device__ double block_sum(double v) {
 // with member < 0 contribute nothing), the max of `accel` and the sum of
 // `energy`, and stores them with plain stores in parts[slot] (thread 0).
 __device__ void store_block_part(BlockPart* parts, int slot, int member, const int lo[3],
-                                 const int hi[3], unsigned accel, double energy, bool with_energy) {
+                                 const int hi[3], unsigned accel, double energy, bool with_energy,
+                                 unsigned contacts = 0u) {
     __shared__ int s_m[16][2];
-    __shared__ int s_box[16][13];
+    __shared__ int s_box[16][14];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     // block's member range
@@ -106,15 +107,19 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
         v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
     }
     v[12] = static_cast<int>(wave_max_u(accel));
+    const unsigned cw = wave_sum_u(contacts);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 13; ++k) s_box[wave][k] = v[k];
+        s_box[wave][13] = static_cast<int>(cw);
     }
     const double e = with_energy ? block_sum(energy) : 0.0;  // contains __syncthreads
     if (!with_energy) __syncthreads();
     if (threadIdx.x == 0) {
         unsigned am = static_cast<unsigned>(s_box[0][12]);
+        unsigned ct = static_cast<unsigned>(s_box[0][13]);
         for (int w = 1; w < nw; ++w) {
+            ct += static_cast<unsigned>(s_box[w][13]);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 v[k] = min(v[k], s_box[w][k]);
@@ -133,7 +138,7 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
             bp.box1[k] = v[6 + k];
         }
         bp.accel = am;
-        bp.pad = 0;
+        bp.contacts = ct;
         bp.energy = e;
         parts[slot] = bp;
     }
@@ -372,6 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t dst = b + rank;
     Sp[dst] = p;
     Sv[dst] = v;
+    if (carry.hidx_out) carry.hidx_out[dst] = NMODE == 2 ? carry.hidx_in[i] : i;
     if (NMODE == 1) {
         carry.nout_p[dst] = p;
         carry.nout_v[dst] = v;
@@ -388,22 +394,24 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // rhs = [v, F/m] in original order.  ENERGY adds the per-grain energy of S
 // (pair terms counted once, from the lower original index) to the block
 // partials.  HIST selects the tangential spring with history: the stored
-// springs of S are read from generation ctl->hist_parity (indexed by original
-// id) and the updated springs of this step are written to the other
-// generation in contact order (walls, then pairs in stencil order).
-template <int MODE, bool ENERGY, bool HIST>
-__global__ __launch_bounds__(kBlock) void k_force(
+// springs of S are read from generation ctl->hist_parity (at hist.hidx[p])
+// and the updated springs of this step are written to the other generation
+// at p, in contact order (walls, then pairs in stencil order).
+template <int MODE, bool ENERGY, bool HIST, int BS>
+__global__ __launch_bounds__(BS) void k_force(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
-        float* __restrict__ rhs_out, BlockPart* __restrict__ parts, HistoryIO hist, int n) {
+        float* __restrict__ rhs_out, BlockPart* __restrict__ parts, double* __restrict__ ewave,
+        HistoryIO hist, int n) {
     const Params& P = *Pp;
-    __shared__ uint32_t s_list[kContactList][kBlock];
+    __shared__ uint32_t s_list[kContactList][BS];
     const int lb = logical_block();
     const int p = lb * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
     unsigned accel_bits = 0;
+    unsigned ncont = 0;
     float e_grain = 0.f;
     int lane_member = -1;
     if (p < n) {
@@ -425,10 +433,11 @@ __global__ __launch_bounds__(kBlock) void k_force(
         const int par = HIST ? ctl->hist_parity : 0;
         const float4* __restrict__ h_in = hist.buf[par];
         float4* __restrict__ h_out = hist.buf[par ^ 1];
-        const int h_old = HIST ? hist.cnt[par][id] : 0;
+        const int hs = HIST ? hist.hidx[p] : 0;  // this grain's index in generation par
+        const int h_old = HIST ? hist.cnt[par][hs] : 0;
         int h_cur = 0, h_new = 0;
         float4 h_next = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (HIST && h_old > 0) h_next = h_in[id];
+        if (HIST && h_old > 0) h_next = h_in[hs];
         auto h_take = [&](int partner, float xi[3]) __attribute__((always_inline)) {
             if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
                 xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
@@ -436,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 xi[0] = xi[1] = xi[2] = 0.f;
                 int found = -1;
                 for (int t = 0; t < h_old; ++t) {
-                    const float4 h = h_in[static_cast<size_t>(t) * n + id];
+                    const float4 h = h_in[static_cast<size_t>(t) * n + hs];
                     if (__float_as_int(h.w) == partner) {
                         xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
                         found = t;
@@ -447,11 +456,11 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 h_cur = found;
             }
             ++h_cur;
-            if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + id];
+            if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + hs];
         };
         auto h_put = [&](int partner, const float xi[3]) __attribute__((always_inline)) {
             if (MODE != kModeRHS && h_new < kHist) {
-                h_out[static_cast<size_t>(h_new) * n + id] =
+                h_out[static_cast<size_t>(h_new) * n + p] =
                         make_float4(xi[0], xi[1], xi[2], __int_as_float(partner));
             }
             ++h_new;
@@ -470,6 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 if (side == 0) f[a] += fw;
                 else f[a] -= fw;
                 if (dist < 0.f) {
+                    ++ncont;
                     f[a] -= c_wall * v[a];
                     if (HIST) {
                         float nrm[3] = {0.f, 0.f, 0.f};
@@ -531,6 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 const int jd = __float_as_int(pj.w);
                 if (ENERGY && jd > id) e_grain += contact_energy(P, 2.f * P.radius - norm);
                 if (!HIST) {
+                    ncont += (norm - P.radius - P.radius < 0.f) ? 1u : 0u;
                     pair_contact(P, mu, v, mat, d0, d1, d2, norm, Sv, q, f);
                     continue;
                 }
@@ -541,6 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
                 f[1] += fp * nrm[1];
                 f[2] += fp * nrm[2];
                 if (dist < 0.f) {
+                    ++ncont;
                     const float4 vj = Sv[q];
                     const float c = P.c_pair[mat * kMaxMat + grain_mat(vj.w)];
                     const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
@@ -567,8 +579,10 @@ __global__ __launch_bounds__(kBlock) void k_force(
         };
         // Phase 1: squared distances only.  A candidate beyond contact adds
         // exact zeros in the reference's arithmetic, so only touching (or
-        // NaN) candidates are listed.  The scan stops when the list is full,
-        // phase 2 drains it, and the scan resumes where it stopped.
+        // NaN) candidates are listed, in stencil order, up to kContactList
+        // of them; phase 2 drains the list.  A grain with more touching
+        // candidates scans again and lists the ones after those drained.
+        int consumed = 0, seen = 0;
         auto probe = [&](uint32_t q, float4 pj) __attribute__((always_inline)) {
             const float d0 = x[0] - pj.x, d1 = x[1] - pj.y, d2 = x[2] - pj.z;
             float s2 = 0.f;
@@ -576,43 +590,45 @@ __global__ __launch_bounds__(kBlock) void k_force(
             s2 += d1 * d1;
             s2 += d2 * d2;
             if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
-                s_list[listed][threadIdx.x] = q;
-                ++listed;
+                if (seen >= consumed && listed < kContactList) {
+                    s_list[listed][threadIdx.x] = q;
+                    ++listed;
+                }
+                ++seen;
             }
         };
-        int c = 0, z = 0;
-        uint4 quad = column(0);
-        uint4 next = column(1);
-        uint32_t q = seg_begin(quad, 0), e = seg_end(quad, 0);
+        // Candidates [b, e) in order, four loads in flight.
+        auto scan = [&](uint32_t b, uint32_t e) __attribute__((always_inline)) {
+            uint32_t q = b;
+            for (; q + 4 <= e; q += 4) {
+                const float4 a0 = Sp[q], a1 = Sp[q + 1], a2 = Sp[q + 2], a3 = Sp[q + 3];
+                probe(q, a0);
+                probe(q + 1, a1);
+                probe(q + 2, a2);
+                probe(q + 3, a3);
+            }
+            for (; q < e; ++q) probe(q, Sp[q]);
+        };
         for (;;) {
-            while (listed <= kContactList - 2) {
-                if (q + 1 < e) {  // two candidates in flight
-                    const float4 a0 = Sp[q], a1 = Sp[q + 1];
-                    probe(q, a0);
-                    probe(q + 1, a1);
-                    q += 2;
-                } else if (q < e) {
-                    probe(q, Sp[q]);
-                    ++q;
-                } else if (z < 2) {
-                    ++z;
-                    q = seg_begin(quad, z);
-                    e = seg_end(quad, z);
-                } else if (c < 8) {
-                    ++c;
-                    quad = next;
-                    if (c < 8) next = column(c + 1);
-                    z = 0;
-                    q = seg_begin(quad, 0);
-                    e = seg_end(quad, 0);
+            seen = 0;
+            uint4 next = column(0);
+            for (int c = 0; c < 9; ++c) {
+                const uint4 quad = next;
+                if (c < 8) next = column(c + 1);
+                // The column's three cells are contiguous in cell order; only
+                // a cell above particles_per_cell splits the range.
+                if (quad.y - quad.x <= kcap && quad.z - quad.y <= kcap && quad.w - quad.z <= kcap) {
+                    scan(quad.x, quad.w);
                 } else {
-                    break;
+#pragma unroll
+                    for (int z = 0; z < 3; ++z) scan(seg_begin(quad, z), seg_end(quad, z));
                 }
             }
+            consumed += listed;
             flush();
-            if (c == 8 && z == 2 && q >= e) break;
+            if (seen <= consumed) break;
         }
-        if (HIST && MODE != kModeRHS) hist.cnt[par ^ 1][id] = min(h_new, kHist);
+        if (HIST && MODE != kModeRHS) hist.cnt[par ^ 1][p] = min(h_new, kHist);
 
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
         if (MODE == kModeRHS) {
@@ -654,10 +670,13 @@ __global__ __launch_bounds__(kBlock) void k_force(
             if (acc == acc) accel_bits = __float_as_uint(acc);
         }
     }
-    if (MODE != kModeRHS || ENERGY) {
-        store_block_part(parts, lb, lane_member, lo, hi, accel_bits, static_cast<double>(e_grain),
-                         ENERGY);
+    if (ENERGY) {
+        // One deterministic double sum per 64 grains, whatever the block size.
+        const double ew = wave_sum(static_cast<double>(e_grain));
+        const int wbase = p & ~63;
+        if ((threadIdx.x & 63) == 0 && wbase < n) ewave[wbase >> 6] = ew;
     }
+    if (MODE != kModeRHS) store_block_part(parts, lb, lane_member, lo, hi, accel_bits, 0.0, false, ncont);
 }
 
 // Largest CIC deposit magnitude of one grain as float bits (0 for a
@@ -679,10 +698,10 @@ __device__ __forceinline__ unsigned cic_grain_bits(const Params& P, float4 p, fl
 // both other than m (it spans three or more members) is folded grain by
 // grain with `grain(i, lo, hi)`.
 template <class Grain>
-__device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict__ parts, int m,
-                                   int v[6], Grain grain) {
+__device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict__ parts, int gpp,
+                                   int m, int v[6], Grain grain) {
     const int lane = threadIdx.x & 63;
-    const int bs = kBlock;  // grains per partial
+    const int bs = gpp;  // grains per partial
     const int g0 = P.mgoff[m], g1 = P.mgoff[m + 1];
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
     if (g1 > g0) {
@@ -709,19 +728,27 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
 
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __restrict__ Pp, Ctl* ctl,
                                                      const BlockPart* __restrict__ parts,
-                                                     int nparts, const uint32_t* __restrict__ key,
+                                                     int nparts, int gpp,
+                                                     const uint32_t* __restrict__ key,
                                                      const float4* __restrict__ Tp,
                                                      const float4* __restrict__ Tv,
+                                                     const double* __restrict__ ewave,
                                                      double* __restrict__ energy_log, int flags) {
     const Params& P = *Pp;
     __shared__ int s_box[16][1];
     unsigned am = 0u;
     double e = 0.0;
+    unsigned long long csum = 0;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         am = max(am, parts[b].accel);
-        e += parts[b].energy;
+        csum += parts[b].contacts;
+    }
+    if (flags & (kFinEnergyLog | kFinEnergyLast)) {
+        const int nw = (P.n + 63) >> 6;
+        for (int w = threadIdx.x; w < nw; w += blockDim.x) e += ewave[w];
     }
     const double total = block_sum(e);
+    const double contacts = block_sum(static_cast<double>(csum));  // exact below 2^53
     const unsigned amax_w = wave_max_u(am);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -735,7 +762,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
         for (int m = wave; m < P.members; m += nw) {
             int v[6];
             if (flags & kFinBox) {
-                wave_member_reduce(P, parts, m, v, [&](int i, int lo[3], int hi[3]) {
+                wave_member_reduce(P, parts, gpp, m, v, [&](int i, int lo[3], int hi[3]) {
                     int c3[3];
                     unpack_cell(key[i], c3[0], c3[1], c3[2]);
 #pragma unroll
@@ -747,7 +774,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
                 if (lane < 6) P.mbox[6 * m + lane] = v[lane];
             }
             if (flags & kFinCicScale) {
-                wave_member_reduce(P, parts, m, v, [&](int i, int lo[3], int hi[3]) {
+                wave_member_reduce(P, parts, gpp, m, v, [&](int i, int lo[3], int hi[3]) {
                     hi[0] = max(hi[0], static_cast<int>(cic_grain_bits(P, Tp[i], Tv[i])));
                 });
                 if (lane == 0) P.mscale[m] = v[3] < 0 ? 0u : static_cast<unsigned>(v[3]);
@@ -767,7 +794,11 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
         }
     }
     if (threadIdx.x != 0) return;
-    if (flags & kFinAccel) ctl->accel_last = amax;
+    if (flags & kFinAccel) {
+        ctl->accel_last = amax;
+        ctl->force_passes = ctl->force_passes + 1;
+        ctl->contact_sum = ctl->contact_sum + static_cast<unsigned long long>(contacts);
+    }
     if (flags & kFinEnergyLast) ctl->energy_last = total;
     if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
     if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
@@ -785,19 +816,23 @@ WD_INST_REORDER(1)
 WD_INST_REORDER(2)
 #undef WD_INST_REORDER
 
-#define WD_INST_FORCE(M, E, H)                                                                  \
-    template __global__ void k_force<M, E, H>(const Params*, Ctl*, const float4*, const float4*,    \
-                                              const float4*, const float4*, const uint32_t*,        \
-                                              float4*, float4*, uint32_t*, float*, BlockPart*,      \
-                                              HistoryIO, int);
-#define WD_INST_FORCE_H(M, E) WD_INST_FORCE(M, E, false) WD_INST_FORCE(M, E, true)
+#define WD_INST_FORCE(M, E, H, B)                                                               \
+    template __global__ void k_force<M, E, H, B>(const Params*, Ctl*, const float4*, const float4*, \
+                                                 const float4*, const float4*, const uint32_t*,     \
+                                                 float4*, float4*, uint32_t*, float*, BlockPart*,   \
+                                                 double*, HistoryIO, int);
+#define WD_INST_FORCE_B(M, E, H) WD_INST_FORCE(M, E, H, 64) WD_INST_FORCE(M, E, H, 128) WD_INST_FORCE(M, E, H, 256)
+#define WD_INST_FORCE_H(M, E) WD_INST_FORCE_B(M, E, false) WD_INST_FORCE_B(M, E, true)
 WD_INST_FORCE_H(kModeSIE, false)
 WD_INST_FORCE_H(kModeSIE, true)
 WD_INST_FORCE_H(kModeBEP, false)
 WD_INST_FORCE_H(kModeBEP, true)
-WD_INST_FORCE_H(kModeRHS, false)
-WD_INST_FORCE_H(kModeRHS, true)
+WD_INST_FORCE(kModeRHS, false, false, 256)
+WD_INST_FORCE(kModeRHS, false, true, 256)
+WD_INST_FORCE(kModeRHS, true, false, 256)
+WD_INST_FORCE(kModeRHS, true, true, 256)
 #undef WD_INST_FORCE_H
+#undef WD_INST_FORCE_B
 #undef WD_INST_FORCE
 
 // ---------------------------------------------------------------------------
@@ -852,21 +887,24 @@ __global__ __launch_bounds__(kBlock) void k_neighbor_rows(const Params* __restri
     row[w] = -1;
 }
 
-// Contact-history lists of one generation, transposed to per-grain rows:
-// partner[id * kHist + k], xi[(id * kHist + k) * 3 + a], count[id].
-__global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restrict__ hist,
+// Contact-history lists of one generation (indexed by T position),
+// transposed to per-grain rows in original order: partner[id * kHist + k],
+// xi[(id * kHist + k) * 3 + a], count[id].
+__global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restrict__ Tp,
+                                                           const float4* __restrict__ hist,
                                                            const int* __restrict__ cnt,
                                                            int* __restrict__ partner,
                                                            float* __restrict__ xi,
                                                            int* __restrict__ count, int n) {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int h = cnt[id];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // T position
+    if (i >= n) return;
+    const int id = __float_as_int(Tp[i].w);
+    const int h = cnt[i];
     count[id] = h;
     for (int k = 0; k < kHist; ++k) {
         const size_t o = static_cast<size_t>(id) * kHist + k;
         if (k < h) {
-            const float4 e = hist[static_cast<size_t>(k) * n + id];
+            const float4 e = hist[static_cast<size_t>(k) * n + i];
             partner[o] = __float_as_int(e.w);
             xi[3 * o] = e.x; xi[3 * o + 1] = e.y; xi[3 * o + 2] = e.z;
         } else {

@@ -159,7 +159,7 @@ class ParticleDynamics:
         check(lib().wd_density_log_read(self._h, None, 0, C.byref(rec)))
         out = np.zeros(max(rec.value, 1) * self.host_density_grid.size, np.float32)
         check(lib().wd_density_log_read(self._h, ptr(out), rec.value, C.byref(rec)))
-        return out[: rec.value * self.host_density_grid.size].reshape(rec.value, -1)
+        return out[: rec.value * self.host_density_grid.size].reshape(rec.value, self.host_density_grid.size)
 
     def profile_step(self, nsteps: int) -> dict[str, float]:
         """Mean ms per launch of each kernel of a step (HIP events on the
@@ -231,6 +231,22 @@ class ParticleDynamics:
         check(lib().wd_get_history(self._h, ptr(partner), ptr(xi), ptr(cnt)))
         n = self.n
         return partner[: n * S].reshape(n, S), xi[: n * S * 3].reshape(n, S, 3), cnt[:n]
+
+    # -- launch configuration (extension) ---------------------------------
+    def set_force_block(self, block: int) -> None:
+        """0: autotune on the next long take_step; 64 / 128 / 256: fixed."""
+        check(lib().wd_set_force_block(self._h, int(block)))
+
+    def force_block(self) -> int:
+        v = C.c_int(0)
+        check(lib().wd_get_force_block(self._h, C.byref(v)))
+        return v.value
+
+    def contact_stats(self) -> tuple[int, int]:
+        """(force passes, contacts summed over them) since construction."""
+        p, c = C.c_ulonglong(0), C.c_ulonglong(0)
+        check(lib().wd_contact_stats(self._h, C.byref(p), C.byref(c)))
+        return p.value, c.value
 
     # -- parameters and the adjoint (extension) ----------------------------
     def set_friction(self, mu: float, member: int = 0) -> None:
