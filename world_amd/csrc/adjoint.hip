@@ -309,11 +309,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ 
 // cotangent lists are indexed by original id.
 __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
-        const float4* __restrict__ Sv, const uint32_t* __restrict__ start, HistoryIO hist,
-        const float4* __restrict__ lx, const float4* __restrict__ lv,
-        const float4* __restrict__ lxi_next, float4* __restrict__ lxi_prev,
-        float4* __restrict__ own_x, float4* __restrict__ own_v, float4* __restrict__ cg_d,
-        float4* __restrict__ cg_v, int* __restrict__ cg_cnt, double2* __restrict__ gparam, int n) {
+        const float4* __restrict__ Sv, const uint32_t* __restrict__ start,
+        const int* __restrict__ hidx, const float4* __restrict__ h_in,
+        const int* __restrict__ h_cnt, const int* __restrict__ mbox,
+        const uint32_t* __restrict__ mcell, const float4* __restrict__ lx,
+        const float4* __restrict__ lv, const float4* __restrict__ lxi_next,
+        float4* __restrict__ lxi_prev, float4* __restrict__ own_x, float4* __restrict__ own_v,
+        float4* __restrict__ cg_d, float4* __restrict__ cg_v, int* __restrict__ cg_cnt,
+        int* __restrict__ cg_p, double2* __restrict__ gparam, int n) {
     const Params& P = *Pp;
     __shared__ uint32_t s_list[kContactList][kBlock];
     const int p = logical_block() * blockDim.x + threadIdx.x;
@@ -338,23 +341,36 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     float gv[3] = {lvt[0], lvt[1], lvt[2]};
     float gk = 0.f, gmu = 0.f;
 
-    const int par = ctl->hist_parity;
-    const float4* __restrict__ h_in = hist.buf[par];
-    const int hs = hist.hidx[p];  // the grain's index in generation par
-    const int h_old = hist.cnt[par][hs];
+    const int hs = hidx[p];  // the grain's index in the springs the step read
+    const int h_old = h_cnt[hs];
     for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(0.f, 0.f, 0.f, 0.f);
     int h_new = 0, kc = 0;
     // Stored spring of `partner` and its slot (-1: a new contact, zero spring).
+    // The stored list is in contact order, so its next entry (prefetched one
+    // contact ahead) normally matches; a miss searches the whole list.
+    int h_cur = 0;
+    float4 h_next = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (h_old > 0) h_next = h_in[hs];
     auto h_find = [&](int partner, float xi[3]) __attribute__((always_inline)) {
-        xi[0] = xi[1] = xi[2] = 0.f;
-        for (int t = 0; t < h_old; ++t) {
-            const float4 h = h_in[static_cast<size_t>(t) * n + hs];
-            if (__float_as_int(h.w) == partner) {
-                xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
-                return t;
+        int found = -1;
+        if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
+            xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
+            found = h_cur;
+        } else {
+            xi[0] = xi[1] = xi[2] = 0.f;
+            for (int t = 0; t < h_old; ++t) {
+                const float4 h = h_in[static_cast<size_t>(t) * n + hs];
+                if (__float_as_int(h.w) == partner) {
+                    xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
+                    found = t;
+                    break;
+                }
             }
+            if (found < 0) return -1;
         }
-        return -1;
+        h_cur = found + 1;
+        if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + hs];
+        return found;
     };
     auto g_next = [&](float g[3]) __attribute__((always_inline)) {
         if (h_new < kHist) {
@@ -395,7 +411,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         }
     }
 
-    const MemberCells mc = member_cells(P, member);
+    MemberCells mc;  // the member's cells as the step's bin laid them out
+    mc.g = make_subgrid(mbox + 6 * member);
+    mc.base = mcell[member];
     int cx, cy, cz;
     grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
@@ -438,6 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             if (kc < kAdjContacts) {
                 cg_d[static_cast<size_t>(kc) * n + id] = make_float4(gd[0], gd[1], gd[2], __int_as_float(jd));
                 cg_v[static_cast<size_t>(kc) * n + id] = make_float4(g_vr[0], g_vr[1], g_vr[2], 0.f);
+                cg_p[static_cast<size_t>(id) * kAdjContacts + kc] = jd;
             } else {
                 ctl->error = kAdjErrContacts;
             }
@@ -446,29 +465,51 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         }
         listed = 0;
     };
-    // Candidate scan of the forward (touching candidates listed in LDS,
-    // drained in stencil order).
-    for (int c = 0; c < 9; ++c) {
-        uint4 q4;
-        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
-        const uint32_t b3[4] = {q4.x, q4.y, q4.z, q4.w};
-        for (int z = 0; z < 3; ++z) {
-            const uint32_t e = min(b3[z + 1], b3[z] + kcap);
-            for (uint32_t q = b3[z]; q < e; ++q) {
-                const float4 pj = Sp[q];
-                const float e0 = x[0] - pj.x, e1 = x[1] - pj.y, e2 = x[2] - pj.z;
-                float s2 = 0.f;
-                s2 += e0 * e0;
-                s2 += e1 * e1;
-                s2 += e2 * e2;
-                if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
-                    s_list[listed][threadIdx.x] = q;
-                    if (++listed == kContactList) flush();
-                }
+    // Candidate scan of the forward pass (touching candidates listed in LDS
+    // in stencil order, drained by flush; a grain with more than
+    // kContactList of them scans again past the ones drained).
+    int consumed = 0, seen = 0;
+    auto probe = [&](uint32_t q, float4 pj) __attribute__((always_inline)) {
+        const float e0 = x[0] - pj.x, e1 = x[1] - pj.y, e2 = x[2] - pj.z;
+        float s2 = 0.f;
+        s2 += e0 * e0;
+        s2 += e1 * e1;
+        s2 += e2 * e2;
+        if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
+            if (seen >= consumed && listed < kContactList) {
+                s_list[listed][threadIdx.x] = q;
+                ++listed;
+            }
+            ++seen;
+        }
+    };
+    auto scan = [&](uint32_t b, uint32_t e) __attribute__((always_inline)) {
+        uint32_t q = b;
+        for (; q + 4 <= e; q += 4) {
+            const float4 a0 = Sp[q], a1 = Sp[q + 1], a2 = Sp[q + 2], a3 = Sp[q + 3];
+            probe(q, a0);
+            probe(q + 1, a1);
+            probe(q + 2, a2);
+            probe(q + 3, a3);
+        }
+        for (; q < e; ++q) probe(q, Sp[q]);
+    };
+    for (;;) {
+        seen = 0;
+        for (int c = 0; c < 9; ++c) {
+            uint4 q4;
+            __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+            if (q4.y - q4.x <= kcap && q4.z - q4.y <= kcap && q4.w - q4.z <= kcap) {
+                scan(q4.x, q4.w);
+            } else {
+                const uint32_t b3[4] = {q4.x, q4.y, q4.z, q4.w};
+                for (int z = 0; z < 3; ++z) scan(b3[z], min(b3[z + 1], b3[z] + kcap));
             }
         }
+        consumed += listed;
+        flush();
+        if (seen <= consumed) break;
     }
-    flush();
     cg_cnt[id] = min(kc, kAdjContacts);
     own_x[id] = make_float4(gx[0], gx[1], gx[2], 0.f);
     own_v[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
@@ -480,27 +521,42 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
 
 // Pass B: dL/d(state s) of grain j = its own share + minus the stored share
 // of every pair contact (i, j) that grain i evaluated, in j's contact order.
+// Partner lists are grain-major (cg_p[i * kAdjContacts + t]) so that a
+// search reads four partners per load.
 __global__ __launch_bounds__(kBlock) void k_adj_gather(const float4* __restrict__ own_x,
                                                        const float4* __restrict__ own_v,
                                                        const float4* __restrict__ cg_d,
                                                        const float4* __restrict__ cg_v,
                                                        const int* __restrict__ cg_cnt,
+                                                       const int* __restrict__ cg_p,
                                                        float4* __restrict__ lx, float4* __restrict__ lv,
                                                        int n) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     float4 gx = own_x[j], gv = own_v[j];
     const int cj = cg_cnt[j];
-    for (int k = 0; k < cj; ++k) {
-        const int i = __float_as_int(cg_d[static_cast<size_t>(k) * n + j].w);
-        const int ci = cg_cnt[i];
-        for (int t = 0; t < ci; ++t) {
-            const float4 e = cg_d[static_cast<size_t>(t) * n + i];
-            if (__float_as_int(e.w) == j) {
+    const int4* pj = reinterpret_cast<const int4*>(cg_p + static_cast<size_t>(j) * kAdjContacts);
+    for (int k0 = 0; k0 < cj; k0 += 4) {
+        const int4 mine = pj[k0 >> 2];
+        const int part[4] = {mine.x, mine.y, mine.z, mine.w};
+        for (int u = 0; u < 4 && k0 + u < cj; ++u) {
+            const int i = part[u];
+            const int ci = cg_cnt[i];
+            const int4* pi = reinterpret_cast<const int4*>(cg_p + static_cast<size_t>(i) * kAdjContacts);
+            int t = -1;
+            for (int b = 0; b < ci && t < 0; b += 8) {
+                const int4 w0 = pi[b >> 2], w1 = pi[(b >> 2) + 1];
+                const int c8[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (t < 0 && b + q < ci && c8[q] == j) t = b + q;
+                }
+            }
+            if (t >= 0) {
+                const float4 e = cg_d[static_cast<size_t>(t) * n + i];
                 const float4 f = cg_v[static_cast<size_t>(t) * n + i];
                 gx.x -= e.x; gx.y -= e.y; gx.z -= e.z;
                 gv.x -= f.x; gv.y -= f.y; gv.z -= f.z;
-                break;
             }
         }
     }

@@ -118,13 +118,26 @@ struct wd_sim {
         float4* H = nullptr;  // springs [kHist][n] by original id
         int* C = nullptr;     // spring counts by original id
     };
+    // The bin of one recomputed step and the springs it read: what the
+    // adjoint of that step consumes.
+    struct BinSlot {
+        float4 *Sp = nullptr, *Sv = nullptr;
+        uint32_t* start = nullptr;
+        int* hidx = nullptr;
+        float4* H = nullptr;  // springs read by the step [kHist][n]
+        int* Hc = nullptr;
+        int* mbox = nullptr;
+        uint32_t* mcell = nullptr;
+    };
     struct AdjointState {
         int nsteps = 0, every = 0, nseg = 0;
         unsigned long long overflow0 = 0;
-        std::vector<Snap> ckpt, seg;
+        std::vector<Snap> ckpt;
+        std::vector<BinSlot> slot;  // every + 1 (the last one receives the final springs)
         float4 *lx = nullptr, *lv = nullptr, *own_x = nullptr, *own_v = nullptr;
         float4 *lxi[2] = {nullptr, nullptr}, *cg_d = nullptr, *cg_v = nullptr;
         int* cg_cnt = nullptr;
+        int* cg_p = nullptr;  // partners of the stored pair contacts, grain-major
         double2* gparam = nullptr;
         float *grid_cot = nullptr, *state_cot = nullptr;
         std::vector<void*> owned;
@@ -1259,6 +1272,65 @@ int read_overflow(wd_sim* s, unsigned long long* out) {
     return WD_OK;
 }
 
+// The forward window with a checkpoint every `every` steps.
+int adj_run_record(wd_sim* s) {
+    auto* A = s->adj;
+    snap_save(s, A->ckpt[0]);
+    for (int c = 1; c <= A->nseg; ++c) {
+        const int rc = s->step(std::min(A->every, A->nsteps - (c - 1) * A->every));
+        if (rc != WD_OK) return rc;
+        snap_save(s, A->ckpt[c]);
+    }
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+// Recomputes `steps` steps from checkpoint `ck`, with the engine's bin and
+// spring buffers redirected into the slots: step k bins into slot k and
+// reads its springs from slot k, writing the next ones into slot k + 1.
+int adj_recompute(wd_sim* s, const wd_sim::Snap& ck, int steps) {
+    auto* A = s->adj;
+    int par = 0;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(&par, &s->ctl->hist_parity, sizeof(int), hipMemcpyDeviceToHost));
+    float4* const Sp0 = s->Sp;
+    float4* const Sv0 = s->Sv;
+    uint32_t* const start0 = s->start;
+    int* const hidx0 = s->hidx[0];
+    float4* const H0[2] = {s->H[0], s->H[1]};
+    int* const Hc0[2] = {s->Hc[0], s->Hc[1]};
+    s->H[par] = A->slot[0].H;
+    s->Hc[par] = A->slot[0].Hc;
+    snap_load(s, ck);
+    for (int k = 0; k < steps; ++k) {
+        wd_sim::BinSlot& b = A->slot[k];
+        WD_HIP(hipMemcpyAsync(b.mbox, s->d_mbox, sizeof(int) * 6 * s->members, hipMemcpyDeviceToDevice,
+                              s->stream));
+        WD_HIP(hipMemcpyAsync(b.mcell, s->d_mcell, sizeof(uint32_t) * (s->members + 1),
+                              hipMemcpyDeviceToDevice, s->stream));
+        s->Sp = b.Sp;
+        s->Sv = b.Sv;
+        s->start = b.start;
+        s->hidx[0] = b.hidx;
+        s->H[par] = b.H;
+        s->Hc[par] = b.Hc;
+        s->H[par ^ 1] = A->slot[k + 1].H;
+        s->Hc[par ^ 1] = A->slot[k + 1].Hc;
+        s->enqueue_step();
+        par ^= 1;
+    }
+    s->Sp = Sp0;
+    s->Sv = Sv0;
+    s->start = start0;
+    s->hidx[0] = hidx0;
+    s->H[0] = H0[0];
+    s->H[1] = H0[1];
+    s->Hc[0] = Hc0[0];
+    s->Hc[1] = Hc0[1];
+    WD_HIP(hipGetLastError());
+    return WD_OK;
+}
+
 }  // namespace
 
 extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
@@ -1268,18 +1340,34 @@ extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
         return fail(WD_ERR_STATE, "wd_adjoint_record: the adjoint covers semi-implicit Euler with spring_history");
     }
     if (s->n == 0) return fail(WD_ERR_STATE, "wd_adjoint_record: no grains");
+    const int every = std::min(checkpoint_every, nsteps);
+    int rc = WD_OK;
+    if (s->adj && s->adj->nsteps == nsteps && s->adj->every == every) {
+        rc = read_overflow(s, &s->adj->overflow0);  // same window shape: keep the buffers
+        if (rc != WD_OK) return rc;
+        return adj_run_record(s);
+    }
     s->drop_adjoint();
     s->adj = new wd_sim::AdjointState();
     auto* A = s->adj;
     A->nsteps = nsteps;
-    A->every = std::min(checkpoint_every, nsteps);
+    A->every = every;
     A->nseg = (nsteps + A->every - 1) / A->every;
     const size_t n = static_cast<size_t>(s->n);
-    int rc = WD_OK;
     A->ckpt.resize(A->nseg + 1);
-    A->seg.resize(A->every);
+    A->slot.resize(A->every + 1);
     for (auto& sn : A->ckpt) if (rc == WD_OK) rc = adj_snap(s, sn);
-    for (size_t k = 1; k < A->seg.size(); ++k) if (rc == WD_OK) rc = adj_snap(s, A->seg[k]);
+    for (auto& b : A->slot) {
+        if (rc == WD_OK) rc = adj_alloc(s, &b.Sp, n);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.Sv, n);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.start, s->scan_cells);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.hidx, n);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.H, static_cast<size_t>(kHist) * n);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.Hc, n);
+        if (rc == WD_OK) rc = adj_alloc(s, &b.mbox, 6 * static_cast<size_t>(s->members));
+        if (rc == WD_OK) rc = adj_alloc(s, &b.mcell, static_cast<size_t>(s->members) + 1);
+    }
+    if (rc == WD_OK) rc = adj_alloc(s, &A->cg_p, static_cast<size_t>(kAdjContacts) * n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->lx, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->lv, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->own_x, n);
@@ -1297,14 +1385,7 @@ extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
         s->drop_adjoint();
         return rc;
     }
-    snap_save(s, A->ckpt[0]);
-    for (int c = 1; c <= A->nseg; ++c) {
-        rc = s->step(std::min(A->every, nsteps - (c - 1) * A->every));
-        if (rc != WD_OK) return rc;
-        snap_save(s, A->ckpt[c]);
-    }
-    WD_HIP(hipGetLastError());
-    return s->sync();
+    return adj_run_record(s);
 }
 
 extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float* state_cot,
@@ -1337,25 +1418,24 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
     for (int seg = A->nseg - 1; seg >= 0; --seg) {
         const int s0 = seg * A->every;
         const int K = std::min(A->nsteps, s0 + A->every) - s0;
-        // Recompute the segment's states from its checkpoint.
-        snap_load(s, A->ckpt[seg]);
-        for (int k = 1; k < K; ++k) {
-            s->enqueue_step();
-            snap_save(s, A->seg[k]);
-        }
+        rc = adj_recompute(s, A->ckpt[seg], K);
+        if (rc != WD_OK) break;
         for (int k = K - 1; k >= 0; --k) {
-            snap_load(s, k == 0 ? A->ckpt[seg] : A->seg[k]);
-            s->enqueue_bin_current();
+            const wd_sim::BinSlot& b = A->slot[k];
             hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
-                               s->Sp, s->Sv, s->start, hist_io(s), A->lx, A->lv, A->lxi[cur],
-                               A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt,
-                               A->gparam, s->n);
+                               (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
+                               (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
+                               (const int*)b.mbox, (const uint32_t*)b.mcell, (const float4*)A->lx,
+                               (const float4*)A->lv, (const float4*)A->lxi[cur], A->lxi[cur ^ 1],
+                               A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->gparam,
+                               s->n);
             hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
-                               A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->lx, A->lv, s->n);
+                               A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->lx, A->lv, s->n);
             cur ^= 1;
         }
     }
     s->energy_on = energy;
+    if (rc != WD_OK) return rc;
     snap_load(s, A->ckpt[A->nseg]);  // the handle is left at the end of the window
     WD_HIP(hipGetLastError());
     rc = s->sync();

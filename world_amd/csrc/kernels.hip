@@ -694,18 +694,19 @@ __device__ __forceinline__ unsigned cic_grain_bits(const Params& P, float4 p, fl
 
 // Reduces, with one wave, the boxes that the block partials hold for member
 // m over its grain range [mgoff[m], mgoff[m+1]) into v (valid in every lane:
-// min lo[3], max hi[3]).  A partial block whose first and last members are
+// min lo[3], max hi[3]).  With first / stride the wave takes partials
+// first, first + stride, ... (its share of a block-wide reduction).  A partial block whose first and last members are
 // both other than m (it spans three or more members) is folded grain by
 // grain with `grain(i, lo, hi)`.
 template <class Grain>
 __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict__ parts, int gpp,
-                                   int m, int v[6], Grain grain) {
-    const int lane = threadIdx.x & 63;
+                                   int m, int v[6], Grain grain, int first = -1, int stride = 64) {
+    const int lane = first < 0 ? static_cast<int>(threadIdx.x & 63) : first;
     const int bs = gpp;  // grains per partial
     const int g0 = P.mgoff[m], g1 = P.mgoff[m + 1];
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
     if (g1 > g0) {
-        for (int b = g0 / bs + lane; b <= (g1 - 1) / bs; b += 64) {
+        for (int b = g0 / bs + lane; b <= (g1 - 1) / bs; b += stride) {
             const BlockPart& bp = parts[b];
             if (bp.m0 == m || bp.m1 == m) {
                 const int* bx = bp.m0 == m ? bp.box0 : bp.box1;
@@ -757,9 +758,58 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     unsigned amax = 0u;
     for (int w = 0; w < nw; ++w) amax = max(amax, static_cast<unsigned>(s_box[w][0]));
     __syncthreads();
-    // One wave per member.
+    // A member with many partials is reduced by the whole block (members in
+    // order), the others by one wave each.
+    __shared__ int s_big[16][6];
+    const int tid = threadIdx.x;
+    auto big = [&](int m) { return (P.mgoff[m + 1] - P.mgoff[m]) / gpp > 256; };
+    auto block_combine = [&](int v[6]) {
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s_big[wave][k] = v[k];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < nw; ++w) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = min(v[k], s_big[w][k]);
+                    v[3 + k] = max(v[3 + k], s_big[w][3 + k]);
+                }
+            }
+        }
+        __syncthreads();
+    };
     if (flags & (kFinBox | kFinCicScale)) {
+        for (int m = 0; m < P.members; ++m) {
+            if (!big(m)) continue;
+            int v[6];
+            if (flags & kFinBox) {
+                wave_member_reduce(P, parts, gpp, m, v, [&](int i, int lo[3], int hi[3]) {
+                    int c3[3];
+                    unpack_cell(key[i], c3[0], c3[1], c3[2]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = min(lo[k], c3[k]);
+                        hi[k] = max(hi[k], c3[k]);
+                    }
+                }, tid, static_cast<int>(blockDim.x));
+                block_combine(v);
+                if (tid == 0) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) P.mbox[6 * m + k] = v[k];
+                }
+            }
+            if (flags & kFinCicScale) {
+                wave_member_reduce(P, parts, gpp, m, v, [&](int i, int lo[3], int hi[3]) {
+                    hi[0] = max(hi[0], static_cast<int>(cic_grain_bits(P, Tp[i], Tv[i])));
+                }, tid, static_cast<int>(blockDim.x));
+                block_combine(v);
+                if (tid == 0) P.mscale[m] = v[3] < 0 ? 0u : static_cast<unsigned>(v[3]);
+            }
+        }
         for (int m = wave; m < P.members; m += nw) {
+            if (big(m)) continue;
             int v[6];
             if (flags & kFinBox) {
                 wave_member_reduce(P, parts, gpp, m, v, [&](int i, int lo[3], int hi[3]) {
