@@ -375,8 +375,9 @@ struct wd_sim {
 
     // Online autotuning of the force pass's block size: the first steps run
     // eagerly, cycling through the candidates round-robin (so that the
-    // evolving state biases none of them), with the force kernels timed by
-    // kernel-stamped events; the first round warms up, the fastest of the
+    // evolving state biases none of them), with the force kernels and the
+    // reductions of their partials (more partials for smaller blocks) timed
+    // by kernel-stamped events; the first round warms up, the fastest of the
     // rest is kept.  These are real steps of the simulation (the results do
     // not depend on the block size).
     int autotune(int& nsteps) {
@@ -397,10 +398,10 @@ struct wd_sim {
                 double t = 0.0;
                 for (auto& rec : recs) {
                     float e = 0.f;
-                    if (std::strcmp(rec.name, "k_force") == 0 &&
-                        hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) {
-                        t += e;
-                    }
+                    // the force pass and the reduction of its block partials
+                    const bool counted = std::strcmp(rec.name, "k_force") == 0 ||
+                                         std::strcmp(rec.name, "k_finalize") == 0;
+                    if (counted && hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) t += e;
                     spare.push_back(rec.a);
                     spare.push_back(rec.b);
                 }
