@@ -239,7 +239,8 @@ int wd_stage_ms(wd_sim* sim, float out[4]);
  * (src/chaos.cpp:163-171; the dataset driver's row loop, src/dataset.cpp:
  * 189-232).  A batch handle advances E members with one launch sequence per
  * step.  Members may differ in their initialization (grain counts, seeds,
- * perturbation) and in physics.friction; every other field must match.
+ * perturbation, initial velocity), physics.friction, physics.restitution and
+ * the material masses; every other field must match.
  * Each member evolves exactly as it would alone: the same cell order, the
  * same force sums and therefore the same bits.
  *
@@ -258,6 +259,12 @@ int wd_batch_member_range(const wd_sim* sim, int member, int* first_out, int* co
 /* CIC grid (4 * nodes floats, src/particle_dynamics.cu:289-293) of one
  * member, or of all members back to back when member is -1. */
 int wd_batch_density_grid(wd_sim* sim, int member, float* grid_out);
+/* Per member: the lifetime over-capacity cell count
+ * (grid_overflow_count, src/find_neighbors_kernel.cu:230-234) and whether
+ * every grain is finite and inside the domain (is_stable,
+ * src/particle_dynamics.cu:311-325).  One int per member each. */
+int wd_batch_overflow_counts(wd_sim* sim, int* counts_out);
+int wd_batch_is_stable(wd_sim* sim, int* stable_out);
 
 /* ---- parameters and the discrete adjoint (extension) ---------------------
  * The reference has no differentiation.  These entry points give gradients

@@ -80,7 +80,7 @@ def test_batch_rejects_incompatible_members_before_device_work():
 
     a = workloads.column(3, 3, 3, friction=0.1)
     b = workloads.column(4, 4, 4, friction=0.5)
-    b.restitution = 0.7  # members may differ only in initialization and friction
+    b.gravity = 3.7  # members may differ only in initialization, friction, restitution, masses
     arr = (_capi.WdConfig * 2)(a, b)
     h = C.c_void_p()
     rc = _capi.lib().wd_batch_create(arr, 2, C.byref(h))

@@ -24,6 +24,11 @@ def members(integrator=_capi.WD_SEMI_IMPLICIT_EULER):
         workloads.perturbed(workloads.column_collapse_4k(), 50e-6, 1001),
         workloads.column(7, 7, 20, friction=0.4, seed=77),
     ]
+    # members may also differ in restitution and grain mass (the dataset
+    # driver's design columns)
+    cs[2].restitution = 0.8
+    cs[4].set_masses([0.0, 2.5 * workloads.GRAIN_MASS])
+    cs[4].restitution = 0.3
     for c in cs:
         c.time_integrator = integrator
     return cs
@@ -102,3 +107,29 @@ def test_batch_density_grids_block():
             np.testing.assert_array_equal(allg[m], ens.member_density_grid(m))
         ens.compute_density_grid()
         np.testing.assert_array_equal(ens.host_density_grid, allg[0])
+
+
+def test_member_overflow_and_stability():
+    """Per-member lifetime overflow counts and stability flags equal each
+    member's single-run diagnostics."""
+    cs = members()
+    # one grain per cell: cells that two grains share as the columns settle
+    # overflow
+    for c in cs:
+        c.particles_per_cell = 1
+    states = [jittered(c, 40 + m) for m, c in enumerate(cs)]
+    states[0][0, 0] = 10.0  # member 0 leaves the domain: unstable
+    want_over, want_stable = [], []
+    for c, st in zip(cs, states):
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.take_step(30)
+            want_over.append(sim.grid_overflow_count())
+            want_stable.append(sim.is_stable())
+    with Ensemble(cs) as ens:
+        ens.set_state(np.concatenate(states))
+        ens.take_step(30)
+        np.testing.assert_array_equal(ens.member_overflow_counts(), want_over)
+        np.testing.assert_array_equal(ens.member_stable(), want_stable)
+        assert ens.grid_overflow_count() == sum(want_over)
+    assert sum(want_over) > 0 and not want_stable[0] and all(want_stable[1:])

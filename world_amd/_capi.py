@@ -134,6 +134,8 @@ _SIGNATURES = [
     ("wd_batch_member_range", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
     ("wd_batch_density_grid", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("wd_batch_overflow_counts", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("wd_batch_is_stable", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_set_friction", C.c_int, [C.c_void_p, C.c_int, C.c_float]),
     ("wd_set_normal_stiffness", C.c_int, [C.c_void_p, C.c_float]),
     ("wd_adjoint_record", C.c_int, [C.c_void_p, C.c_int, C.c_int]),

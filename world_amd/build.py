@@ -67,10 +67,30 @@ def build_cpp_tests() -> Path:
     return out
 
 
+DRIVERS = ("dataset", "chaos", "model_problem")
+
+
+def build_drivers() -> list[Path]:
+    """The batched counterparts of the reference's ensemble / dataset /
+    model-problem drivers, linked against the library (world_amd/bin/)."""
+    src_dir = PKG / "drivers"
+    out_dir = PKG / "bin"
+    out_dir.mkdir(exist_ok=True)
+    outs = []
+    for name in DRIVERS:
+        out = out_dir / name
+        _run([CXX, "-O2", "-std=c++17", "-I", str(ROOT / "include"), "-I", str(ROOT / "include" / "world"),
+              "-I", str(src_dir), str(src_dir / f"{name}.cpp"), "-o", str(out), "-L", str(PKG),
+              "-lworld_amd", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."])
+        outs.append(out)
+    return outs
+
+
 def main() -> int:
     build_library()
     if (ROOT / "tests" / "cpp" / "particle_dynamics_test.cpp").exists():
         build_cpp_tests()
+    build_drivers()
     return 0
 
 

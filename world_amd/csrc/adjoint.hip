@@ -248,9 +248,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ 
             gv[a] = state_cot[6 * static_cast<size_t>(id) + 3 + a];
         }
     }
-    const float m = P.mass[grain_mat(v.w)];
+    const int member = member_of(P, v.w);
+    const float m = mat_mass(P, member, grain_mat(v.w));
     if (grid_cot && m != 0.f) {
-        const int member = member_of(P, v.w);
         const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
         const float* G = grid_cot + static_cast<size_t>(member) * 4 * nodes;
         const float x[3] = {p.x, p.y, p.z}, vel[3] = {v.x, v.y, v.z};
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     const int mat = grain_mat(vi4.w);
     const int member = member_of(P, vi4.w);
     const float mu = P.members == 1 ? P.mu : P.mmu[member];
-    const float m = P.mass[mat];
+    const float m = mat_mass(P, member, mat);
 
     // SIE: v' = v + dt F/m, x' = x + dt v'.
     const float4 lx4 = lx[id], lv4 = lv[id];
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     };
 
     // Walls, lower then upper per axis, as the forward.
-    const float c_wall = P.c_wall[mat];
+    const float c_wall = mat_cwall(P, member, mat);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             const float raw = P.kneg * dist;
             const float fp = fmaxf(0.f, fminf(P.max_force, raw));
             const float4 vj = Sv[q];
-            const float c = P.c_pair[mat * kMaxMat + grain_mat(vj.w)];
+            const float c = mat_cpair(P, member, mat, grain_mat(vj.w));
             const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
             float vn = 0.f;
             vn += vr[0] * nrm[0];

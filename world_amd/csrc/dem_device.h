@@ -22,6 +22,7 @@ namespace wd {
 
 constexpr int kMaxMat = WD_MAX_MATERIALS;
 constexpr int kHist = 24;  // contact-history slots per grain (walls + pairs)
+constexpr int kMatTab = 2 * kMaxMat + kMaxMat * kMaxMat;  // one member's material tables
 
 // Device-resident solver constants (one copy per handle, read through a
 // pointer so that dt can change without re-capturing the step graph).
@@ -52,6 +53,9 @@ struct Params {
     uint32_t* mcell;       // [members + 1]
     const float* mmu;      // [members] friction coefficient
     unsigned* mscale;      // [members] float bits of the largest CIC deposit magnitude
+    const float* mtab;     // [members][kMatTab] mass, c_wall, c_pair per member (null: the
+                           // tables below serve every grain)
+    unsigned long long* movf;  // [members] over-capacity cells, summed over passes
     float mass[kMaxMat];
     float c_wall[kMaxMat];
     float c_pair[kMaxMat * kMaxMat];
@@ -88,6 +92,19 @@ __device__ __forceinline__ void grain_cell(const Params& P, float x, float y, fl
     cx = cell_axis(x, P.lo[0], P.hi[0], P.gx);
     cy = cell_axis(y, P.lo[1], P.hi[1], P.gy);
     cz = cell_axis(z, P.lo[2], P.hi[2], P.gz);
+}
+
+// Material constants of a grain of `member` (members of a batch may differ
+// in grain mass and restitution).
+__device__ __forceinline__ float mat_mass(const Params& P, int member, int mat) {
+    return P.mtab ? P.mtab[member * kMatTab + mat] : P.mass[mat];
+}
+__device__ __forceinline__ float mat_cwall(const Params& P, int member, int mat) {
+    return P.mtab ? P.mtab[member * kMatTab + kMaxMat + mat] : P.c_wall[mat];
+}
+__device__ __forceinline__ float mat_cpair(const Params& P, int member, int a, int b) {
+    return P.mtab ? P.mtab[member * kMatTab + 2 * kMaxMat + a * kMaxMat + b]
+                  : P.c_pair[a * kMaxMat + b];
 }
 
 // Member index and material id of a grain, packed in its velocity's .w.
@@ -172,8 +189,8 @@ __device__ __forceinline__ float pair_norm(const float xi[3], float xj0, float x
 // One neighbor j (cell-sorted slot q) acting on grain i, given the
 // separation d and its norm.  A pair beyond contact adds a zero spring force,
 // as in the reference.
-__device__ __forceinline__ void pair_contact(const Params& P, float mu, const float vi[3], int mat_i,
-                                             float d0, float d1, float d2, float norm,
+__device__ __forceinline__ void pair_contact(const Params& P, float mu, int member, const float vi[3],
+                                             int mat_i, float d0, float d1, float d2, float norm,
                                              const float4* __restrict__ vel, int q, float f[3]) {
     const float dist = norm - P.radius - P.radius;
     const float n0 = d0 / norm, n1 = d1 / norm, n2 = d2 / norm;
@@ -184,7 +201,7 @@ __device__ __forceinline__ void pair_contact(const Params& P, float mu, const fl
     if (dist < 0.f) {
         const float4 vj = vel[q];
         const int mat_j = grain_mat(vj.w);
-        const float c = P.c_pair[mat_i * kMaxMat + mat_j];
+        const float c = mat_cpair(P, member, mat_i, mat_j);
         const float r0 = vi[0] - vj.x, r1 = vi[1] - vj.y, r2 = vi[2] - vj.z;
         float vn = 0.f;
         vn += r0 * n0;

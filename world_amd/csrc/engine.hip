@@ -83,6 +83,8 @@ struct wd_sim {
     uint32_t* d_mcell = nullptr;
     float* d_mmu = nullptr;
     unsigned* d_mscale = nullptr;
+    float* d_mtab = nullptr;                 // per-member material tables (members > 1)
+    unsigned long long* d_movf = nullptr;    // per-member over-capacity cells
     size_t nodes = 0;                  // CIC nodes per member channel
     int nblocks = 0;
     BlockPart* parts = nullptr;
@@ -169,7 +171,7 @@ struct wd_sim {
                         Hc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
-                        hidx[0], hidx[1]};
+                        hidx[0], hidx[1], d_mtab, d_movf};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -182,7 +184,7 @@ struct wd_sim {
         st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr; ewave = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
-        d_mscale = nullptr;
+        d_mscale = nullptr; d_mtab = nullptr; d_movf = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -557,6 +559,23 @@ int upload_params(wd_sim* s) {
     return WD_OK;
 }
 
+// Each member's mass and damping tables, formed by the same code as a single
+// simulation's (members > 1 only).
+int upload_member_tables(wd_sim* s) {
+    if (!s->d_mtab) return WD_OK;
+    std::vector<float> tab(static_cast<size_t>(kMatTab) * s->members);
+    for (int m = 0; m < s->members; ++m) {
+        Params q;
+        fill_params(s->mcfg[m], s->n, q);
+        float* t = tab.data() + static_cast<size_t>(kMatTab) * m;
+        std::memcpy(t, q.mass, sizeof(float) * kMaxMat);
+        std::memcpy(t + kMaxMat, q.c_wall, sizeof(float) * kMaxMat);
+        std::memcpy(t + 2 * kMaxMat, q.c_pair, sizeof(float) * kMaxMat * kMaxMat);
+    }
+    WD_HIP(hipMemcpy(s->d_mtab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+    return WD_OK;
+}
+
 // Recomputes the derived constants from s->cfg (keeping the member tables
 // and the current dt) and uploads them; graphs read them through dp.
 int refresh_params(wd_sim* s) {
@@ -569,6 +588,10 @@ int refresh_params(wd_sim* s) {
     s->hp.mcell = keep.mcell;
     s->hp.mmu = keep.mmu;
     s->hp.mscale = keep.mscale;
+    s->hp.movf = keep.movf;
+    s->hp.mtab = keep.mtab;
+    const int rc = upload_member_tables(s);
+    if (rc != WD_OK) return rc;
     return upload_params(s);
 }
 
@@ -614,6 +637,9 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->d_mcell, E + 1));
     WD_TRY(dalloc(&s->d_mmu, E));
     WD_TRY(dalloc(&s->d_mscale, E));
+    WD_TRY(dalloc(&s->d_movf, E));
+    WD_HIP(hipMemset(s->d_movf, 0, sizeof(unsigned long long) * E));
+    if (E > 1) WD_TRY(dalloc(&s->d_mtab, static_cast<size_t>(kMatTab) * E));
     std::vector<float> mus(E);
     for (int m = 0; m < E; ++m) mus[m] = s->mcfg[m].friction;
     WD_HIP(hipMemcpy(s->d_mgoff, s->goff.data(), sizeof(int) * (E + 1), hipMemcpyHostToDevice));
@@ -626,6 +652,9 @@ int allocate(wd_sim* s) {
     s->hp.mcell = s->d_mcell;
     s->hp.mmu = s->d_mmu;
     s->hp.mscale = s->d_mscale;
+    s->hp.movf = s->d_movf;
+    s->hp.mtab = s->d_mtab;
+    WD_TRY(upload_member_tables(s));
     WD_TRY(dalloc(&s->cic_acc, s->cic_count));  // int64 fixed point, all members
     WD_TRY(dalloc(&s->cic_out, s->cic_count));
 #undef WD_TRY
@@ -673,9 +702,9 @@ int export_state(wd_sim* s) {
 
 namespace {
 
-// Fields every ensemble member must share (the grid, the box, the material
-// and the integrator); members may differ in their initial grains and in
-// friction.
+// Fields every ensemble member must share (the grid, the box, the spring and
+// the integrator); members may differ in their initial grains, friction,
+// restitution and material masses.
 int check_compatible(const wd_config& a, const wd_config& b) {
     const bool same =
             a.dt == b.dt && a.collision_grid_size_x == b.collision_grid_size_x &&
@@ -690,12 +719,13 @@ int check_compatible(const wd_config& a, const wd_config& b) {
             a.max_force == b.max_force && a.floor_z == b.floor_z && a.ceiling_z == b.ceiling_z &&
             a.wall_x_min == b.wall_x_min && a.wall_x_max == b.wall_x_max &&
             a.wall_y_min == b.wall_y_min && a.wall_y_max == b.wall_y_max &&
-            a.restitution == b.restitution && a.tangential_model == b.tangential_model &&
+            a.tangential_model == b.tangential_model &&
             a.tangential_stiffness_ratio == b.tangential_stiffness_ratio &&
             a.normal_stiffness == b.normal_stiffness && a.num_masses == b.num_masses &&
-            std::memcmp(a.masses, b.masses, sizeof(float) * a.num_masses) == 0 &&
             a.time_integrator == b.time_integrator && a.picard_iterations == b.picard_iterations;
-    return same ? WD_OK : fail(WD_ERR_ARG, "ensemble members differ in more than initial state and friction");
+    return same ? WD_OK
+                : fail(WD_ERR_ARG, "ensemble members differ in more than initial state, friction, "
+                                   "restitution and material masses");
 }
 
 int create_with(const wd_config* cfgs, int members, const std::vector<int>& goff, const float* st,
@@ -1495,5 +1525,44 @@ extern "C" int wd_contact_stats(wd_sim* s, unsigned long long* passes, unsigned 
     WD_HIP(hipStreamSynchronize(s->stream));
     WD_HIP(hipMemcpy(passes, &s->ctl->force_passes, sizeof(*passes), hipMemcpyDeviceToHost));
     WD_HIP(hipMemcpy(contacts, &s->ctl->contact_sum, sizeof(*contacts), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-member diagnostics of a batch.
+
+extern "C" int wd_batch_overflow_counts(wd_sim* s, int* counts) {
+    WD_CHECK_SIM(s);
+    if (!counts) return fail(WD_ERR_ARG, "wd_batch_overflow_counts: null output");
+    int rc = s->sync();
+    if (rc != WD_OK) return rc;
+    std::vector<unsigned long long> v(static_cast<size_t>(s->members));
+    WD_HIP(hipMemcpy(v.data(), s->d_movf, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost));
+    for (int m = 0; m < s->members; ++m) counts[m] = static_cast<int>(std::min<unsigned long long>(v[m], 0x7FFFFFFFull));
+    return WD_OK;
+}
+
+extern "C" int wd_batch_is_stable(wd_sim* s, int* stable) {
+    WD_CHECK_SIM(s);
+    if (!stable) return fail(WD_ERR_ARG, "wd_batch_is_stable: null output");
+    std::vector<float> st(static_cast<size_t>(6) * s->n);
+    const int rc = s->n ? wd_get_state(s, st.data()) : WD_OK;
+    if (rc != WD_OK) return rc;
+    for (int m = 0; m < s->members; ++m) {
+        const wd_config& c = s->mcfg[m];
+        const float lo[3] = {c.x_min, c.y_min, c.z_min};
+        const float hi[3] = {c.x_max, c.y_max, c.z_max};
+        int ok = 1;
+        for (int i = s->goff[m]; i < s->goff[m + 1] && ok; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                const float p = st[6 * static_cast<size_t>(i) + a];
+                if (!std::isfinite(p) || p < lo[a] || p > hi[a]) {
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        stable[m] = ok;
+    }
     return WD_OK;
 }

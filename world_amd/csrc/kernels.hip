@@ -260,7 +260,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
         uint32_t sum = 0, over = 0;
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
-            over += v[k] > kcap ? 1u : 0u;
+            if (v[k] > kcap) {
+                ++over;
+                // The member owning the cell (rare: over-capacity cells only).
+                const uint32_t cell = static_cast<uint32_t>(base) + k;
+                int lo = 0, hi = Pp->members - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (Pp->mcell[mid] <= cell) lo = mid;
+                    else hi = mid - 1;
+                }
+                atomicAdd(&Pp->movf[lo], 1ull);
+            }
             const uint32_t c = v[k];
             v[k] = sum;  // exclusive within the thread
             sum += c;
@@ -423,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_force(
         const int mat = grain_mat(vi4.w);
         const int member = member_of(P, vi4.w);
         const float mu = P.members == 1 ? P.mu : P.mmu[member];
-        const float m = P.mass[mat];
+        const float m = mat_mass(P, member, mat);
         float f[3] = {0.f, 0.f, 0.f};
         if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
 
@@ -468,7 +479,7 @@ __global__ __launch_bounds__(BS) void k_force(
 
         // Gravity and the six walls, lower then upper per axis
         // (src/compute_rhs_kernel.cu:105-141).
-        const float c_wall = P.c_wall[mat];
+        const float c_wall = mat_cwall(P, member, mat);
         f[2] -= m * P.gravity;
         #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -542,7 +553,7 @@ __global__ __launch_bounds__(BS) void k_force(
                 if (ENERGY && jd > id) e_grain += contact_energy(P, 2.f * P.radius - norm);
                 if (!HIST) {
                     ncont += (norm - P.radius - P.radius < 0.f) ? 1u : 0u;
-                    pair_contact(P, mu, v, mat, d0, d1, d2, norm, Sv, q, f);
+                    pair_contact(P, mu, member, v, mat, d0, d1, d2, norm, Sv, q, f);
                     continue;
                 }
                 const float dist = norm - P.radius - P.radius;
@@ -554,7 +565,7 @@ __global__ __launch_bounds__(BS) void k_force(
                 if (dist < 0.f) {
                     ++ncont;
                     const float4 vj = Sv[q];
-                    const float c = P.c_pair[mat * kMaxMat + grain_mat(vj.w)];
+                    const float c = mat_cpair(P, member, mat, grain_mat(vj.w));
                     const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
                     float vn = 0.f;
                     vn += vr[0] * nrm[0];
@@ -682,7 +693,7 @@ __global__ __launch_bounds__(BS) void k_force(
 // Largest CIC deposit magnitude of one grain as float bits (0 for a
 // massless grain, +inf bits for a non-finite one).
 __device__ __forceinline__ unsigned cic_grain_bits(const Params& P, float4 p, float4 v) {
-    const float m = P.mass[grain_mat(v.w)];
+    const float m = mat_mass(P, member_of(P, v.w), grain_mat(v.w));
     float mag = fabsf(m);
     mag = fmaxf(mag, fabsf(m * v.x));
     mag = fmaxf(mag, fabsf(m * v.y));
@@ -1031,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict
     const int member = member_of(P, v.w);
     const unsigned sb = P.mscale[member];
     if (sb >= 0x7F800000u) return;  // non-finite input: the member's grid is reported as NaN
-    const float m = P.mass[grain_mat(v.w)];
+    const float m = mat_mass(P, member, grain_mat(v.w));
     if (m == 0.f) return;
     const double scale =
             ldexp(1.0, cic_shift(__uint_as_float(sb), P.mgoff[member + 1] - P.mgoff[member]));

@@ -325,6 +325,16 @@ class Ensemble(ParticleDynamics):
         check(lib().wd_batch_density_grid(self._h, int(m), ptr(out)))
         return out
 
+    def member_overflow_counts(self) -> np.ndarray:
+        out = np.zeros(self.members, np.int32)
+        check(lib().wd_batch_overflow_counts(self._h, ptr(out)))
+        return out
+
+    def member_stable(self) -> np.ndarray:
+        out = np.zeros(self.members, np.int32)
+        check(lib().wd_batch_is_stable(self._h, ptr(out)))
+        return out.astype(bool)
+
     def density_grids(self) -> np.ndarray:
         """All members' CIC grids, [members, 4 * nodes]."""
         out = np.zeros(self.members * K_CHANNELS * self.density_grid_nodes(), np.float32)
