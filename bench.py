@@ -91,25 +91,26 @@ def run_gpu(cfg, steps: int, warmup: int, cic_every: int, energy: bool, profile_
     elog = sim.energy_log() if energy else None
     stable = sim.is_stable()
     overflow = sim.grid_overflow_count()
-    block = sim.force_block()
+    variant = sim.force_variant()
     passes, contacts = sim.contact_stats()
     z = contacts / max(1, passes * n)
     sim.close()
     # Per-kernel durations over the same work: the run is deterministic, so a
-    # second handle, with the force block size the first one tuned, replays
+    # second handle, with the force-pass variant the first one tuned, replays
     # the warmup + timed steps with kernel-stamped HIP events on every launch
     # on the engine's stream (profile_steps of them).
     kernels = {}
     if profile_steps > 0:
         prof = ParticleDynamics(cfg)
-        prof.set_force_block(block)
+        prof.set_force_variant(*variant)
         if energy:
             prof.enable_energy_log(profile_steps + 8)
         kernels = prof.profile_step(profile_steps)
         prof.close()
     return {
         "n": n, "elapsed": elapsed, "kernels_ms": kernels, "stable": stable,
-        "overflow": overflow, "grids": grids, "energy": elog, "force_block": block,
+        "overflow": overflow, "grids": grids, "energy": elog,
+        "force_variant": {"block": variant[0], "lanes_per_grain": variant[1]},
         "contacts_per_grain": z, "history": cfg.tangential_model == 1,
     }
 
@@ -318,7 +319,7 @@ def main() -> int:
                           "achieved_GBps": value * b_total / 1e9,
                           "frac": value * b_total / HBM_PEAK},
         "kernels_ms": main_run["kernels_ms"],
-        "force_block": main_run["force_block"],
+        "force_variant": main_run["force_variant"],
         "stable": main_run["stable"],
         "overflow_cells": main_run["overflow"],
     }
@@ -334,7 +335,7 @@ def main() -> int:
             "step_roofline": {"bytes_per_grain_step": lb, "achieved_GBps": lv * lb / 1e9,
                               "frac": lv * lb / HBM_PEAK},
             "roofline": roofline_of(lr),
-            "kernels_ms": lr["kernels_ms"], "force_block": lr["force_block"], "stable": lr["stable"],
+            "kernels_ms": lr["kernels_ms"], "force_variant": lr["force_variant"], "stable": lr["stable"],
             "overflow_cells": lr["overflow"],
         }
     if not args.no_extras:

@@ -299,12 +299,17 @@ int wd_adjoint_backward(wd_sim* sim, const float* grid_cotangent, const float* s
                         float* grad_state0, double* grad_params);
 
 /* ---- launch configuration (extension) -----------------------------------
- * The force pass runs with 64, 128 or 256 grains per block.  By default the
- * first wd_step of at least nine steps tunes it online (three round-robin
- * rounds over the candidates, timed on the engine's stream); the results are
- * bit-identical for every choice.  The reference fixes threads_per_block from the config
- * (src/kernel.cu, blocks = ceil(n / threads_per_block)). */
-int wd_set_force_block(wd_sim* sim, int block); /* 0: autotune, or 64 / 128 / 256 */
+ * The force pass has six variants: one lane per grain with 64, 128 or 256
+ * grains per block (lanes 0), or a one-wave block whose contacts are spread
+ * over its lanes with 1, 2 or 4 lanes per grain (block 64, lanes 1 / 2 / 4).
+ * By default the first wd_step of at least 18 steps tunes it online (three
+ * round-robin rounds over the candidates, timed on the engine's stream); the
+ * results are bit-identical for every choice.  The reference fixes
+ * threads_per_block from the config (src/kernel.cu:31-38, blocks =
+ * ceil(n / threads_per_block)). */
+int wd_set_force_variant(wd_sim* sim, int block, int lanes); /* (0, 0): autotune */
+int wd_get_force_variant(const wd_sim* sim, int* block_out, int* lanes_out); /* (0, 0) until tuned */
+int wd_set_force_block(wd_sim* sim, int block); /* 0: autotune, or 64 / 128 / 256 with lanes 0 */
 int wd_get_force_block(const wd_sim* sim, int* block_out); /* 0 until tuned */
 
 /* Completed force passes and the contacts (wall and pair, counted from each

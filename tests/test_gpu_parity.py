@@ -108,7 +108,7 @@ def test_graph_replay_equals_eager_launches(col4k):
     eager = col4k.copy()
     eager.kernel_timing = 1
     with ParticleDynamics(col4k) as a, ParticleDynamics(eager) as b:
-        a.take_step(37)  # nine eager tuning steps, one 16-step graph chunk, 12 single-step replays
+        a.take_step(37)  # 18 eager tuning steps, one 16-step graph chunk, 3 single-step replays
         for _ in range(37):
             b.take_step()
         np.testing.assert_array_equal(a.state(), b.state())
@@ -236,31 +236,86 @@ def test_sliding_grain_on_gpu_decelerates_at_mu_g():
     assert np.polyfit(t[sel], vx[sel], 1)[0] == pytest.approx(-0.5 * 9.81, rel=0.01)
 
 
+FORCE_VARIANTS = [(64, 0), (128, 0), (256, 0), (64, 1), (64, 2), (64, 4)]
+
+
+def assert_springs_equal(got, want, msg=""):
+    """Equal spring counts, and equal partners and displacements in the
+    slots in use."""
+    (gp, gx, gc), (wp, wx, wc) = got, want
+    np.testing.assert_array_equal(gc, wc, err_msg=msg)
+    for i in range(len(gc)):
+        k = int(gc[i])
+        np.testing.assert_array_equal(gp[i, :k], wp[i, :k], err_msg=f"{msg} grain {i}")
+        np.testing.assert_array_equal(gx[i, :k], wx[i, :k], err_msg=f"{msg} grain {i}")
+
+
 @pytest.mark.parametrize("picard", [False, True])
-def test_results_do_not_depend_on_the_force_block_size(picard):
-    """64, 128 and 256 grains per force block, and the autotuned choice, give
-    bit-identical states, contact springs and energy logs."""
+@pytest.mark.parametrize("model", ["history", "viscous"])
+def test_results_do_not_depend_on_the_force_variant(picard, model):
+    """Every force-pass variant (64, 128 or 256 grains per block with one lane
+    each; one-wave blocks with 1, 2 or 4 lanes per grain), and the autotuned
+    choice, give bit-identical states, contact springs and energy logs."""
     c = workloads.column_collapse_4k(friction=0.4)
+    if model == "viscous":
+        c = viscous_cfg(c)
     if picard:
         c.time_integrator = _capi.WD_BACKWARD_EULER_PICARD
     st, _ = jittered(c, seed=8)
     out = {}
-    for block in (64, 128, 256, 0):
+    for v in FORCE_VARIANTS + [(0, 0)]:
         with ParticleDynamics(c) as sim:
             sim.set_state(st)
-            sim.set_force_block(block)
+            sim.set_force_variant(*v)
             sim.enable_energy_log(64)
             sim.take_step(3)
             sim.take_step(37)
-            assert sim.force_block() in (64, 128, 256)
-            out[block] = (sim.state(), sim.history(), sim.energy_log(), sim.compute_total_energy())
-    ref = out[256]
-    for block, got in out.items():
-        np.testing.assert_array_equal(got[0], ref[0], err_msg=str(block))
-        for a, b in zip(got[1], ref[1]):
-            np.testing.assert_array_equal(a, b)
-        np.testing.assert_array_equal(got[2], ref[2])
-        assert got[3] == ref[3]
+            assert sim.force_variant() in FORCE_VARIANTS
+            springs = sim.history() if model == "history" else None
+            out[v] = (sim.state(), springs, sim.energy_log(), sim.compute_total_energy())
+    ref = out[(256, 0)]
+    for v, got in out.items():
+        np.testing.assert_array_equal(got[0], ref[0], err_msg=str(v))
+        if model == "history":
+            assert_springs_equal(got[1], ref[1], str(v))
+        np.testing.assert_array_equal(got[2], ref[2], err_msg=str(v))
+        assert got[3] == ref[3], v
+
+
+@pytest.mark.parametrize("kcap", [8, 128])
+@pytest.mark.parametrize("model", ["history", "viscous"])
+def test_dense_cluster_matches_oracle_for_every_variant(oracle_lib, kcap, model):
+    """80 grains inside a 3.2-radius cube: each touches about 40 others, more
+    than one lane lists per pass and more than the 24 spring slots, and with
+    8 grains per cell most cells are over capacity.  Every variant gives the
+    oracle's FP32 states, springs and energy."""
+    c = workloads.column(4, 4, 5, friction=0.5)
+    c.particles_per_cell = kcap
+    if model == "viscous":
+        c = viscous_cfg(c)
+    st, mat = _capi.initial_state(c)
+    st = st.reshape(-1, 6).copy()
+    rng = np.random.default_rng(77)
+    r = c.particle_radius
+    st[:, :3] = (rng.uniform(-1.6 * r, 1.6 * r, (len(st), 3)) + [0.0, 0.0, 0.3]).astype(np.float32)
+    st[:, 3:] = rng.normal(0, 0.05, (len(st), 3)).astype(np.float32)
+    ora = oracle_lib.Sim(c, st, mat)
+    ora.steps(4, threads=1)
+    want_state, want_hist, want_e = ora.state(), ora.history(), ora.energy()
+    ora.close()
+    with ParticleDynamics(c) as sim:
+        sim.set_state(st)
+        touching = [np.count_nonzero(row >= 0) for row in sim.neighbor_rows()]
+    assert max(touching) > 30
+    for v in FORCE_VARIANTS:
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.set_force_variant(*v)
+            sim.take_step(4)
+            np.testing.assert_array_equal(sim.state(), want_state, err_msg=str(v))
+            if model == "history":
+                assert_springs_equal(sim.history(), want_hist, str(v))
+            assert sim.compute_total_energy() == pytest.approx(want_e, rel=1e-6), v
 
 
 def test_contact_count_matches_oracle(oracle_lib):

@@ -139,6 +139,8 @@ _SIGNATURES = [
     ("wd_set_friction", C.c_int, [C.c_void_p, C.c_int, C.c_float]),
     ("wd_set_normal_stiffness", C.c_int, [C.c_void_p, C.c_float]),
     ("wd_adjoint_record", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("wd_set_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("wd_get_force_variant", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("wd_set_force_block", C.c_int, [C.c_void_p, C.c_int]),
     ("wd_get_force_block", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_contact_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),

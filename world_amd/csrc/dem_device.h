@@ -38,6 +38,8 @@ struct Params {
     float spring_k;        // k of the energy's quadratic branch
     float clamp_delta;     // overlap where the force clamp engages
     float cut2;            // squared centre distance above which a pair cannot touch
+    float touch2;          // smallest FP32 squared distance with (sqrt - r) - r >= 0: a pair
+                           // touches exactly when its squared distance is below it
     float mu;
     float dt;
     float kt;              // tangential spring stiffness (history model)

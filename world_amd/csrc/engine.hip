@@ -89,9 +89,12 @@ struct wd_sim {
     int nblocks = 0;
     BlockPart* parts = nullptr;
     double* energy_log = nullptr;
-    double* ewave = nullptr;      // per-64-grain energy sums of the last energy pass
-    int force_bs = 0;             // force-pass block size; 0 until the autotuner has chosen
-    bool force_bs_fixed = false;  // set by wd_set_force_block
+    double* ewave = nullptr;      // per-kEnergyGroup-grain energy sums of the last energy pass
+    int force_var = -1;           // index into kForceVariants; -1 until the autotuner has chosen
+    bool force_fixed = false;     // set by wd_set_force_block / wd_set_force_variant
+    long long steps_run = 0;      // steps taken by this handle
+    long long next_tune = 0;      // step count at which the autotuner runs (again)
+    double tune_ms[sizeof(kForceVariants) / sizeof(kForceVariants[0])] = {};  // last tuning's time per variant (force pass + its reduction)
     int energy_cap = 0;
     bool energy_on = false;
     unsigned long long* cic_acc = nullptr;
@@ -284,16 +287,36 @@ struct wd_sim {
         else force(k_force<MODE, ENERGY, false, BS>);
     }
 
+    template <int MODE, bool ENERGY, int L>
+    void launch_force_q(const float4* np, const float4* nv) {
+        const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}, hidx[hidx_cur]};
+        auto force = [&](auto kern) {
+            go("k_force", kern, dim3(grain_blocks(n, 64 / L)), dim3(64), (const Params*)dp, ctl,
+               (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, parts,
+               ewave, hio, n);
+        };
+        if (history()) force(k_force_q<MODE, ENERGY, true, L>);
+        else force(k_force_q<MODE, ENERGY, false, L>);
+    }
+
+    // The variant the force pass runs with: the tuned or pinned one, else
+    // 256 grains per block, one lane each.
+    ForceVariant variant() const { return kForceVariants[force_var < 0 ? 2 : force_var]; }
+
     // last_pass: the pass completes a step (its springs become current).
     template <int MODE, bool ENERGY>
     void launch_force(const float4* np, const float4* nv, float* rhs, bool last_pass = true) {
-        const int bs = (MODE == kModeRHS || force_bs == 0) ? kBlock : force_bs;
+        const ForceVariant fv = MODE == kModeRHS ? ForceVariant{kBlock, 0} : variant();
+        const int bs = variant_grains(fv);
         const int gb = grain_blocks(n, bs);
         if constexpr (MODE == kModeRHS) {
             launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
         } else {
-            if (bs == 64) launch_force_bs<MODE, ENERGY, 64>(np, nv, rhs);
-            else if (bs == 128) launch_force_bs<MODE, ENERGY, 128>(np, nv, rhs);
+            if (fv.lanes == 1) launch_force_q<MODE, ENERGY, 1>(np, nv);
+            else if (fv.lanes == 2) launch_force_q<MODE, ENERGY, 2>(np, nv);
+            else if (fv.lanes == 4) launch_force_q<MODE, ENERGY, 4>(np, nv);
+            else if (fv.block == 64) launch_force_bs<MODE, ENERGY, 64>(np, nv, rhs);
+            else if (fv.block == 128) launch_force_bs<MODE, ENERGY, 128>(np, nv, rhs);
             else launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
         }
         int flags = 0;
@@ -375,23 +398,40 @@ struct wd_sim {
         return WD_OK;
     }
 
-    // Online autotuning of the force pass's block size: the first steps run
-    // eagerly, cycling through the candidates round-robin (so that the
-    // evolving state biases none of them), with the force kernels and the
-    // reductions of their partials (more partials for smaller blocks) timed
-    // by kernel-stamped events; the first round warms up, the fastest of the
-    // rest is kept.  These are real steps of the simulation (the results do
-    // not depend on the block size).
+    // Online autotuning of the force pass's variant (block size, lanes per
+    // grain): the first steps run eagerly, cycling through the candidates
+    // round-robin (so that the evolving state biases none of them), with the
+    // force kernels and the reductions of their partials (more partials for
+    // fewer grains per block) timed by kernel-stamped events; the first
+    // round warms up, the fastest of the rest is kept.  These are real steps
+    // of the simulation (the results do not depend on the variant).  The
+    // best variant moves as contacts build up (a settling pile has several
+    // times the contacts of the released lattice), so tuning runs again
+    // after 256, 2048, 16384, ... steps.
     int autotune(int& nsteps) {
-        if (force_bs != 0 || n == 0) return WD_OK;
-        constexpr int kRounds = 3;
-        constexpr int kCand = static_cast<int>(sizeof(kForceBlocks) / sizeof(kForceBlocks[0]));
-        if (nsteps < kRounds * kCand) return WD_OK;  // tune on a longer call
+        if (n == 0 || steps_run < next_tune) return WD_OK;
+        constexpr int kCand = static_cast<int>(sizeof(kForceVariants) / sizeof(kForceVariants[0]));
+        // The first tuning warms every variant up for a round and keeps the
+        // best of two more; a later one times each remaining candidate once
+        // and drops those that were more than 1.5x the best last time.
+        const bool first = force_var < 0 || tune_ms[0] == 0.0;
+        const int rounds = first ? 3 : 1;
+        bool cand[kCand];
+        int ncand = 0;
+        double last_best = 1e30;
+        for (int c = 0; c < kCand; ++c) last_best = std::min(last_best, tune_ms[c]);
+        for (int c = 0; c < kCand; ++c) {
+            cand[c] = first || tune_ms[c] <= 1.5 * last_best;
+            ncand += cand[c] ? 1 : 0;
+        }
+        if (nsteps < rounds * ncand) return WD_OK;  // tune on a longer call
+        const int before = force_var;
         double best_ms[kCand];
         for (int c = 0; c < kCand; ++c) best_ms[c] = 1e30;
-        for (int r = 0; r < kRounds; ++r) {
+        for (int r = 0; r < rounds; ++r) {
             for (int c = 0; c < kCand; ++c) {
-                force_bs = kForceBlocks[c];
+                if (!cand[c]) continue;
+                force_var = c;
                 std::vector<ProfRec> recs;
                 prof = &recs;
                 enqueue_step();
@@ -407,20 +447,25 @@ struct wd_sim {
                     spare.push_back(rec.a);
                     spare.push_back(rec.b);
                 }
-                if (r > 0) best_ms[c] = std::min(best_ms[c], t);
+                if (r > 0 || !first) best_ms[c] = std::min(best_ms[c], t);
                 time += hp.dt;
                 --nsteps;
+                ++steps_run;
             }
         }
-        int best = 0;
-        for (int c = 1; c < kCand; ++c) if (best_ms[c] < best_ms[best]) best = c;
-        force_bs = kForceBlocks[best];
-        drop_graphs();
+        int best = -1;
+        for (int c = 0; c < kCand; ++c) {
+            if (cand[c] && (best < 0 || best_ms[c] < best_ms[best])) best = c;
+        }
+        for (int c = 0; c < kCand; ++c) tune_ms[c] = best_ms[c];
+        force_var = best;
+        next_tune = next_tune == 0 ? 256 : 8 * next_tune;
+        if (best != before) drop_graphs();  // captured with another variant
         return check_device();
     }
 
     int step(int nsteps) {
-        if (!force_bs_fixed) {
+        if (!force_fixed) {
             const int rc = autotune(nsteps);
             if (rc != WD_OK) return rc;
         }
@@ -445,6 +490,7 @@ struct wd_sim {
         }
         // The reference accumulates time in FP32, one dt per step.
         for (int s = 0; s < nsteps; ++s) time += hp.dt;
+        steps_run += nsteps;
         return WD_OK;
     }
 
@@ -505,6 +551,29 @@ int validate(const wd_config& c) {
     return WD_OK;
 }
 
+// The pair test of the contact law, (sqrtf(s2) - r) - r < 0 in FP32, is
+// monotone in s2 (correctly rounded sqrt and subtraction), so it holds
+// exactly below one FP32 value: the smallest s >= 0 for which it fails, found
+// by bisection over the bit patterns of the non-negative floats (ordered as
+// integers).  0 when nothing touches (r <= 0).
+float touch_threshold(float r) {
+    auto touching = [r](uint32_t bits) {
+        float s;
+        std::memcpy(&s, &bits, sizeof(s));
+        return (std::sqrt(s) - r) - r < 0.f;
+    };
+    uint32_t lo = 0u, hi = 0x7F800000u;  // +inf never touches
+    if (!touching(lo)) return 0.f;
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (touching(mid)) lo = mid;
+        else hi = mid;
+    }
+    float s;
+    std::memcpy(&s, &hi, sizeof(s));
+    return s;
+}
+
 void fill_params(const wd_config& c, int n, Params& P) {
     std::memset(&P, 0, sizeof(P));
     P.n = n;
@@ -534,6 +603,7 @@ void fill_params(const wd_config& c, int n, Params& P) {
     // exceeds it has norm - r - r > 0 after rounding, so its spring, dashpot,
     // friction and energy terms are all exactly zero.
     P.cut2 = static_cast<float>(4.0 * c.particle_radius * static_cast<double>(c.particle_radius) * (1.0 + 1e-5));
+    P.touch2 = touch_threshold(c.particle_radius);
     P.mu = c.friction;
     P.dt = c.dt;
     P.history = c.tangential_model == WD_TANGENTIAL_SPRING_HISTORY ? 1 : 0;
@@ -626,9 +696,9 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->status, s->ntiles_max));
     WD_TRY(dalloc(&s->st6, static_cast<size_t>(6) * n));
     WD_TRY(dalloc(&s->mat_d, n));
-    s->nblocks = grain_blocks(n, kForceBlocks[0]);  // the most partials any pass writes
+    s->nblocks = grain_blocks(n, 16);  // the most partials any pass writes (k_force_q, 4 lanes)
     WD_TRY(dalloc(&s->parts, s->nblocks));
-    WD_TRY(dalloc(&s->ewave, static_cast<size_t>((n + 63) / 64)));
+    WD_TRY(dalloc(&s->ewave, static_cast<size_t>((n + kEnergyGroup - 1) / kEnergyGroup)));
     s->nodes = static_cast<size_t>(c.density_grid_size_x) * c.density_grid_size_y * c.density_grid_size_z;
     s->cic_count = static_cast<size_t>(4) * s->nodes * static_cast<size_t>(s->members);
     const int E = s->members;
@@ -1500,23 +1570,47 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
 // ---------------------------------------------------------------------------
 // Launch configuration (extension).
 
+extern "C" int wd_set_force_variant(wd_sim* s, int block, int lanes) {
+    WD_CHECK_SIM(s);
+    int v = -1;
+    for (int k = 0; k < static_cast<int>(sizeof(kForceVariants) / sizeof(kForceVariants[0])); ++k) {
+        if (kForceVariants[k].block == block && kForceVariants[k].lanes == lanes) v = k;
+    }
+    if (!(block == 0 && lanes == 0) && v < 0) {
+        return fail(WD_ERR_ARG, "wd_set_force_variant: (block, lanes) must be (0, 0) (autotune), "
+                                "(64|128|256, 0) or (64, 1|2|4)");
+    }
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->force_var = v;
+    s->force_fixed = v >= 0;
+    s->next_tune = s->steps_run;  // (0, 0): tune on the next long enough call
+    for (double& t : s->tune_ms) t = 0.0;
+    s->drop_graphs();
+    return WD_OK;
+}
+
+extern "C" int wd_get_force_variant(const wd_sim* s, int* block, int* lanes) {
+    WD_CHECK_SIM(s);
+    if (!block || !lanes) return fail(WD_ERR_ARG, "wd_get_force_variant: null output");
+    const ForceVariant fv = s->force_var < 0 ? ForceVariant{0, 0} : kForceVariants[s->force_var];
+    *block = fv.block;
+    *lanes = fv.lanes;
+    return WD_OK;
+}
+
 extern "C" int wd_set_force_block(wd_sim* s, int block) {
     WD_CHECK_SIM(s);
     if (block != 0 && block != 64 && block != 128 && block != 256) {
         return fail(WD_ERR_ARG, "wd_set_force_block: block must be 0 (autotune), 64, 128 or 256");
     }
-    WD_HIP(hipStreamSynchronize(s->stream));
-    s->force_bs = block;
-    s->force_bs_fixed = block != 0;
-    s->drop_graphs();
-    return WD_OK;
+    return wd_set_force_variant(s, block, 0);
 }
 
 extern "C" int wd_get_force_block(const wd_sim* s, int* block) {
     WD_CHECK_SIM(s);
     if (!block) return fail(WD_ERR_ARG, "wd_get_force_block: null output");
-    *block = s->force_bs;
-    return WD_OK;
+    int lanes = 0;
+    return wd_get_force_variant(s, block, &lanes);
 }
 
 extern "C" int wd_contact_stats(wd_sim* s, unsigned long long* passes, unsigned long long* contacts) {

@@ -15,6 +15,8 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 constexpr int kFinalizeBlock = 1024;  // one block; one wave per ensemble member at a time
 constexpr int kContactList = 24;     // touching candidates buffered per grain in LDS
+constexpr int kQueueCap = 16;        // contacts listed per lane and pass by the queued force pass
+constexpr int kEnergyGroup = 16;     // grains per energy partial (ewave entry)
 
 // Small device-side control block: counters, the occupied-cell bounding box
 // and per-pass results.  Only k_count (block 0) and k_finalize write it
@@ -44,7 +46,7 @@ struct BlockPart {
     int box1[6];      // the same for member m1 (when m1 != m0)
     unsigned accel;   // float bits of the block's max |a| (or deposit magnitude)
     unsigned contacts;  // contacts (walls + pairs) of the block's grains
-    double energy;    // unused (energies are summed per wave, see k_force)
+    double energy;    // unused (energies are summed per kEnergyGroup grains into ewave)
 };
 
 // What k_finalize publishes.
@@ -65,9 +67,18 @@ inline int grain_blocks(int n, int bs = kBlock) {
     return ((b + kXcds - 1) / kXcds) * kXcds;
 }
 
-// Force-pass block sizes the autotuner chooses from (grains per block; the
-// results do not depend on the choice).
+// Force-pass variants the autotuner chooses from; the results do not depend
+// on the choice.  lanes 0: k_force, one lane per grain, `block` grains per
+// block.  lanes L > 0: k_force_q, L lanes per grain, one wave (64 / L grains)
+// per block.
+struct ForceVariant {
+    int block;
+    int lanes;
+};
+constexpr ForceVariant kForceVariants[6] = {{64, 0}, {128, 0}, {256, 0}, {64, 1}, {64, 2}, {64, 4}};
 constexpr int kForceBlocks[3] = {64, 128, 256};
+// Grains per block of a variant (grains per block partial).
+constexpr int variant_grains(ForceVariant v) { return v.lanes == 0 ? v.block : 64 / v.lanes; }
 
 __device__ __forceinline__ int logical_block() {
     const int per = gridDim.x / kXcds;

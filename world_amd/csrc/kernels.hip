@@ -50,9 +50,105 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// Deterministic sum of one double per grain over kEnergyGroup consecutive
+// grains held L lanes apart: a butterfly over the grain index, so the bits do
+// not depend on L or on the block size.
+template <int L>
+__device__ __forceinline__ double group_energy(double v) {
+#pragma unroll
+    for (int o = L; o < kEnergyGroup * L; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
     for (int o = 32; o > 0; o >>= 1) v += static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64));
     return v;
+}
+
+
+// True when a column's cell-start quad (starts of cells dz = -1, 0, +1 and of
+// the next cell) has a cell holding more than kcap grains.
+__device__ __forceinline__ bool column_capped(const uint4& q4, uint32_t kcap) {
+    return q4.y - q4.x > kcap || q4.z - q4.y > kcap || q4.w - q4.z > kcap;
+}
+
+// Number of candidates of a column: its three cells' grains, at most kcap
+// from each.
+__device__ __forceinline__ uint32_t column_length(const uint4& q4, uint32_t kcap) {
+    return column_capped(q4, kcap) ? min(q4.y - q4.x, kcap) + min(q4.z - q4.y, kcap) + min(q4.w - q4.z, kcap)
+                                   : q4.w - q4.x;
+}
+
+// Walks one grain's stencil candidates in the reference's neighbor-row order
+// (columns dx, then dy; the three cells dz = -1, 0, +1 of a column; at most
+// kcap grains of a cell) and calls probe(q, x, y, z) with the cell-sorted slot
+// and position of those at positions [s_lo, s_hi) of that sequence.  A run
+// (one column, or with GROUPED the three columns of one dx) is walked with
+// BATCH position loads in flight, while the next run's cell starts load.
+// column(c) returns column c's cell-start quad.
+template <int BATCH, bool GROUPED, class Column, class Probe>
+__device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint32_t kcap, uint32_t s_lo,
+                                             uint32_t s_hi, Column column, Probe probe) {
+    uint32_t off = 0;  // sequence position of the run's first candidate
+    // Candidates b0 + [0, l0), b1 + [0, l1), b2 + [0, l2) in that order.
+    auto run3 = [&](uint32_t b0, uint32_t l0, uint32_t b1, uint32_t l1, uint32_t b2, uint32_t l2)
+            __attribute__((always_inline)) {
+        const uint32_t tot = l0 + l1 + l2;
+        const uint32_t i0 = min(s_lo > off ? s_lo - off : 0u, tot);
+        const uint32_t i1 = s_hi > off ? min(s_hi - off, tot) : 0u;
+        for (uint32_t i = i0; i < i1; i += BATCH) {
+            uint32_t q[BATCH];
+            float px[BATCH], py[BATCH], pz[BATCH];
+            // Every slot loads (a slot past the run reloads the first
+            // candidate), so that no load sits behind a branch and all are
+            // in flight together.
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const uint32_t t = i + k;
+                q[k] = t < l0 ? b0 + t : (t < l0 + l1 ? b1 + (t - l0) : b2 + (t - l0 - l1));
+                const uint32_t ql = t < i1 ? q[k] : q[0];
+                const float* s = reinterpret_cast<const float*>(Sp + ql);
+                px[k] = s[0];
+                py[k] = s[1];
+                pz[k] = s[2];
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                if (i + k < i1) probe(q[k], px[k], py[k], pz[k]);
+            }
+        }
+        off += tot;
+    };
+    if constexpr (!GROUPED) {
+        uint4 next = column(0);
+#pragma unroll 1
+        for (int c = 0; c < 9; ++c) {
+            const uint4 a = next;
+            if (c < 8) next = column(c + 1);
+            if (!column_capped(a, kcap)) {
+                run3(a.x, a.w - a.x, a.w, 0u, a.w, 0u);
+            } else {
+                run3(a.x, min(a.y - a.x, kcap), a.y, min(a.z - a.y, kcap), a.z, min(a.w - a.z, kcap));
+            }
+        }
+        return;
+    }
+    uint4 n0 = column(0), n1 = column(1), n2 = column(2);
+#pragma unroll 1
+    for (int gx = 0; gx < 3; ++gx) {
+        const uint4 a = n0, b = n1, c = n2;
+        if (gx < 2) {
+            n0 = column(3 * gx + 3);
+            n1 = column(3 * gx + 4);
+            n2 = column(3 * gx + 5);
+        }
+        if (!column_capped(a, kcap) && !column_capped(b, kcap) && !column_capped(c, kcap)) {
+            run3(a.x, a.w - a.x, b.x, b.w - b.x, c.x, c.w - c.x);
+        } else {
+            run3(a.x, min(a.y - a.x, kcap), a.y, min(a.z - a.y, kcap), a.z, min(a.w - a.z, kcap));
+            run3(b.x, min(b.y - b.x, kcap), b.y, min(b.z - b.y, kcap), b.z, min(b.w - b.z, kcap));
+            run3(c.x, min(c.y - c.x, kcap), c.y, min(c.z - c.y, kcap), c.z, min(c.w - c.z, kcap));
+        }
+    }
 }
 
 // Deterministic block sum of one double per thread (fixed butterfly and wave
@@ -295,37 +391,56 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
             agg += s_wave[w];
             over_total += s_over[w];
         }
-        if (threadIdx.x == 0) {
-            if (over_total) atomicAdd(&ctl->overflow_lifetime, static_cast<unsigned long long>(over_total));
+        if (threadIdx.x == 0 && over_total) {
+            atomicAdd(&ctl->overflow_lifetime, static_cast<unsigned long long>(over_total));
+        }
+        // Look-back by the first wave: lane k reads the status of tile
+        // j - k, so one round covers 64 predecessors; the nearest one with
+        // its inclusive prefix ends the walk, and the aggregates between are
+        // added.  A round repeats until every status it needs is published.
+        if (wave == 0) {
             uint32_t prefix = 0;
             if (tile == 0) {
-                __hip_atomic_store(&status[0], tag | (2ull << 32) | agg, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&status[0], tag | (2ull << 32) | agg, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
             } else {
-                __hip_atomic_store(&status[tile], tag | (1ull << 32) | agg, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&status[tile], tag | (1ull << 32) | agg, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
                 int j = tile - 1;
                 unsigned spins = 0;
-                while (j >= 0) {
-                    const unsigned long long w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_AGENT);
+                for (;;) {
+                    const int idx = j - lane;
+                    unsigned long long w = 0ull;
+                    if (idx >= 0) {
+                        w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                     const unsigned flag = static_cast<unsigned>((w >> 32) & 3u);
-                    if ((w & ~((1ull << 34) - 1)) != tag || flag == 0) {
+                    const bool ok = idx >= 0 && (w & ~((1ull << 34) - 1)) == tag && flag != 0u;
+                    const unsigned long long inc = __ballot(ok && flag == 2u);
+                    const int stop = inc ? __builtin_ctzll(inc) : 64;  // nearest inclusive
+                    const unsigned long long need = stop >= 63 ? ~0ull : ((2ull << stop) - 1ull);
+                    if ((__ballot(ok) & need) != need) {
                         if (++spins > (1u << 22)) {
-                            ctl->error = 1;
+                            if (lane == 0) ctl->error = 1;
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    prefix += static_cast<uint32_t>(w);
-                    if (flag == 2) break;
-                    --j;
+                    prefix += wave_sum_u(lane <= stop && ok ? static_cast<uint32_t>(w) : 0u);
+                    if (stop < 64) break;
+                    j -= 64;
                 }
-                __hip_atomic_store(&status[tile], tag | (2ull << 32) | (prefix + agg),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&status[tile], tag | (2ull << 32) | (prefix + agg),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
-            s_prefix = prefix;
+            if (lane == 0) s_prefix = prefix;
         }
         __syncthreads();
         const uint32_t off = s_prefix + wave_off + (incl - sum);
@@ -408,8 +523,17 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // springs of S are read from generation ctl->hist_parity (at hist.hidx[p])
 // and the updated springs of this step are written to the other generation
 // at p, in contact order (walls, then pairs in stencil order).
+//
+// Phase 1 walks the stencil with batched position loads and lists the
+// touching candidates (squared distance below touch2, exactly the contact
+// law's dist < 0), up to kContactList of them; phase 2 evaluates them in that
+// order with the next contact's loads in flight.  A grain with more contacts
+// walks again and lists the ones after those evaluated.  A candidate beyond
+// contact adds exact zeros in the reference's arithmetic and is skipped; a
+// NaN candidate (a diverged state) makes the force NaN whatever the order,
+// which is applied at the end.
 template <int MODE, bool ENERGY, bool HIST, int BS>
-__global__ __launch_bounds__(BS) void k_force(
+__global__ __launch_bounds__(BS, 5) void k_force(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
@@ -418,6 +542,13 @@ __global__ __launch_bounds__(BS) void k_force(
         HistoryIO hist, int n) {
     const Params& P = *Pp;
     __shared__ uint32_t s_list[kContactList][BS];
+    __shared__ float s_cpair[kMaxMat * kMaxMat];
+    // A lone simulation's pair damping table, in LDS: one lookup per contact.
+    const bool lds_tab = P.mtab == nullptr;
+    if (lds_tab) {
+        for (int k = threadIdx.x; k < kMaxMat * kMaxMat; k += BS) s_cpair[k] = P.c_pair[k];
+        __syncthreads();
+    }
     const int lb = logical_block();
     const int p = lb * blockDim.x + threadIdx.x;
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
@@ -435,12 +566,16 @@ __global__ __launch_bounds__(BS) void k_force(
         const int member = member_of(P, vi4.w);
         const float mu = P.members == 1 ? P.mu : P.mmu[member];
         const float m = mat_mass(P, member, mat);
+        auto cpair = [&](int mj) __attribute__((always_inline)) {
+            return lds_tab ? s_cpair[mat * kMaxMat + mj] : mat_cpair(P, member, mat, mj);
+        };
         float f[3] = {0.f, 0.f, 0.f};
         if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
 
         // Contact history: the old list was written in the same contact order,
         // so its next entry is prefetched one contact ahead and normally
-        // matches; a miss falls back to a search of the whole list.
+        // matches; a miss falls back to a search of the whole list.  Slot 0 is
+        // loaded with the list's length (it exists whether or not in use).
         const int par = HIST ? ctl->hist_parity : 0;
         const float4* __restrict__ h_in = hist.buf[par];
         float4* __restrict__ h_out = hist.buf[par ^ 1];
@@ -448,7 +583,7 @@ __global__ __launch_bounds__(BS) void k_force(
         const int h_old = HIST ? hist.cnt[par][hs] : 0;
         int h_cur = 0, h_new = 0;
         float4 h_next = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (HIST && h_old > 0) h_next = h_in[hs];
+        if (HIST) h_next = h_in[hs];
         auto h_take = [&](int partner, float xi[3]) __attribute__((always_inline)) {
             if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
                 xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
@@ -467,7 +602,8 @@ __global__ __launch_bounds__(BS) void k_force(
                 h_cur = found;
             }
             ++h_cur;
-            if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + hs];
+            // the next slot, loaded unconditionally (slots below kHist exist)
+            h_next = h_in[static_cast<size_t>(min(h_cur, kHist - 1)) * n + hs];
         };
         auto h_put = [&](int partner, const float xi[3]) __attribute__((always_inline)) {
             if (MODE != kModeRHS && h_new < kHist) {
@@ -531,41 +667,37 @@ __global__ __launch_bounds__(BS) void k_force(
             __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
             return q4;
         };
-        // Candidates of segment z (0..2) of a column: cell z's first kcap grains.
-        auto seg_begin = [](const uint4& q4, int z) __attribute__((always_inline)) {
-            return z == 0 ? q4.x : (z == 1 ? q4.y : q4.z);
-        };
-        auto seg_end = [kcap](const uint4& q4, int z) __attribute__((always_inline)) {
-            const uint32_t b = z == 0 ? q4.x : (z == 1 ? q4.y : q4.z);
-            const uint32_t e = z == 0 ? q4.y : (z == 1 ? q4.z : q4.w);
-            return min(e, b + kcap);
-        };
-        // Phase 2: the full contact law for the listed candidates, in the
+        // Phase 2: the full contact law for the listed contacts, in the
         // order phase 1 found them (the canonical stencil order).
         int listed = 0;
         auto flush = [&]() __attribute__((always_inline)) {
+            uint32_t qn = 0;
+            float4 pjn = make_float4(0.f, 0.f, 0.f, 0.f), vjn = pjn;
+            if (listed > 0) {
+                qn = s_list[0][threadIdx.x];
+                pjn = Sp[qn];
+                vjn = Sv[qn];
+            }
             for (int k = 0; k < listed; ++k) {
-                const uint32_t q = s_list[k][threadIdx.x];
-                const float4 pj = Sp[q];
+                const float4 pj = pjn, vj = vjn;
+                // the next contact's loads (the last one's again after it:
+                // unconditional, so that they are in flight during this one)
+                qn = s_list[min(k + 1, listed - 1)][threadIdx.x];
+                pjn = Sp[qn];
+                vjn = Sv[qn];
                 float d0, d1, d2;
                 const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
                 const int jd = __float_as_int(pj.w);
                 if (ENERGY && jd > id) e_grain += contact_energy(P, 2.f * P.radius - norm);
-                if (!HIST) {
-                    ncont += (norm - P.radius - P.radius < 0.f) ? 1u : 0u;
-                    pair_contact(P, mu, member, v, mat, d0, d1, d2, norm, Sv, q, f);
-                    continue;
-                }
                 const float dist = norm - P.radius - P.radius;
                 const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
                 const float fp = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
                 f[0] += fp * nrm[0];
                 f[1] += fp * nrm[1];
                 f[2] += fp * nrm[2];
-                if (dist < 0.f) {
+                if (dist < 0.f) {  // every listed candidate
                     ++ncont;
-                    const float4 vj = Sv[q];
-                    const float c = mat_cpair(P, member, mat, grain_mat(vj.w));
+                    const float c = cpair(grain_mat(vj.w));
                     const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
                     float vn = 0.f;
                     vn += vr[0] * nrm[0];
@@ -573,72 +705,64 @@ __global__ __launch_bounds__(BS) void k_force(
                     vn += vr[2] * nrm[2];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) f[a] -= c * vn * nrm[a];
-                    float xi[3];
-                    h_take(jd, xi);
-                    if (ENERGY && jd > id) {
-                        float s2 = 0.f;
-                        s2 += xi[0] * xi[0];
-                        s2 += xi[1] * xi[1];
-                        s2 += xi[2] * xi[2];
-                        e_grain += 0.5f * P.kt * s2;
+                    if (HIST) {
+                        float xi[3];
+                        h_take(jd, xi);
+                        if (ENERGY && jd > id) {
+                            float s2 = 0.f;
+                            s2 += xi[0] * xi[0];
+                            s2 += xi[1] * xi[1];
+                            s2 += xi[2] * xi[2];
+                            e_grain += 0.5f * P.kt * s2;
+                        }
+                        spring_friction(P, mu, f, nrm, vr, vn, fp, c, xi);
+                        h_put(jd, xi);
+                    } else {
+                        float vt[3];
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * nrm[a];
+                        viscous_friction(f, vt, mu * fp, c);
                     }
-                    spring_friction(P, mu, f, nrm, vr, vn, fp, c, xi);
-                    h_put(jd, xi);
                 }
             }
             listed = 0;
         };
-        // Phase 1: squared distances only.  A candidate beyond contact adds
-        // exact zeros in the reference's arithmetic, so only touching (or
-        // NaN) candidates are listed, in stencil order, up to kContactList
-        // of them; phase 2 drains the list.  A grain with more touching
-        // candidates scans again and lists the ones after those drained.
+        // Phase 1: squared distances only.
         int consumed = 0, seen = 0;
-        auto probe = [&](uint32_t q, float4 pj) __attribute__((always_inline)) {
-            const float d0 = x[0] - pj.x, d1 = x[1] - pj.y, d2 = x[2] - pj.z;
+        bool nan_f = false, nan_e = false;
+        auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
+            const float d0 = x[0] - px, d1 = x[1] - py, d2 = x[2] - pz;
             float s2 = 0.f;
             s2 += d0 * d0;
             s2 += d1 * d1;
             s2 += d2 * d2;
-            if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
-                if (seen >= consumed && listed < kContactList) {
-                    s_list[listed][threadIdx.x] = q;
-                    ++listed;
+            if (!(s2 >= P.touch2) && q != static_cast<uint32_t>(p)) {
+                if (s2 < P.touch2) {
+                    if (seen >= consumed && listed < kContactList) {
+                        s_list[listed][threadIdx.x] = q;
+                        ++listed;
+                    }
+                    ++seen;
+                } else {
+                    nan_f = true;
+                    if (ENERGY && __float_as_int(Sp[q].w) > id) nan_e = true;
                 }
-                ++seen;
             }
-        };
-        // Candidates [b, e) in order, four loads in flight.
-        auto scan = [&](uint32_t b, uint32_t e) __attribute__((always_inline)) {
-            uint32_t q = b;
-            for (; q + 4 <= e; q += 4) {
-                const float4 a0 = Sp[q], a1 = Sp[q + 1], a2 = Sp[q + 2], a3 = Sp[q + 3];
-                probe(q, a0);
-                probe(q + 1, a1);
-                probe(q + 2, a2);
-                probe(q + 3, a3);
-            }
-            for (; q < e; ++q) probe(q, Sp[q]);
         };
         for (;;) {
             seen = 0;
-            uint4 next = column(0);
-            for (int c = 0; c < 9; ++c) {
-                const uint4 quad = next;
-                if (c < 8) next = column(c + 1);
-                // The column's three cells are contiguous in cell order; only
-                // a cell above particles_per_cell splits the range.
-                if (quad.y - quad.x <= kcap && quad.z - quad.y <= kcap && quad.w - quad.z <= kcap) {
-                    scan(quad.x, quad.w);
-                } else {
-#pragma unroll
-                    for (int z = 0; z < 3; ++z) scan(seg_begin(quad, z), seg_end(quad, z));
-                }
-            }
+            walk_stencil<4, false>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe);
             consumed += listed;
             flush();
             if (seen <= consumed) break;
         }
+        if (nan_f) {
+            const float qnan = __builtin_nanf("");
+            f[0] += qnan;
+            f[1] += qnan;
+            f[2] += qnan;
+        }
+        if (ENERGY && nan_e) e_grain += __builtin_nanf("");
         if (HIST && MODE != kModeRHS) hist.cnt[par ^ 1][p] = min(h_new, kHist);
 
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
@@ -682,12 +806,380 @@ __global__ __launch_bounds__(BS) void k_force(
         }
     }
     if (ENERGY) {
-        // One deterministic double sum per 64 grains, whatever the block size.
-        const double ew = wave_sum(static_cast<double>(e_grain));
-        const int wbase = p & ~63;
-        if ((threadIdx.x & 63) == 0 && wbase < n) ewave[wbase >> 6] = ew;
+        const double ew = group_energy<1>(static_cast<double>(e_grain));
+        if ((threadIdx.x & (kEnergyGroup - 1)) == 0 && p < n) ewave[p / kEnergyGroup] = ew;
     }
     if (MODE != kModeRHS) store_block_part(parts, lb, lane_member, lo, hi, accel_bits, 0.0, false, ncont);
+}
+
+// ---------------------------------------------------------------------------
+// Pass 5, queued: the forces, springs and integration of k_force with the
+// work of one wave (64 / L grains, L adjacent lanes per grain) spread over
+// its lanes by contact rather than by grain.
+//   Phase 1: the L lanes of a grain scan equal slices of the grain's stencil
+//            candidates (the nine columns concatenated) and list the touching
+//            ones: squared distance below touch2, which is exactly the
+//            contact law's `dist < 0`.  A grain's lists, lane after lane, are
+//            its contacts in stencil order, so every contact's spring slot
+//            (walls first) is known before any is evaluated.
+//   Phase 2: the wave's listed contacts form one queue, grain after grain,
+//            evaluated 64 at a time, one per lane.  Each lane leaves its
+//            force and energy addends in LDS, and the grain's lanes add them
+//            in queue order, which is the order in which k_force adds them:
+//            the sums are the same bits.
+// A lane with more than kQueueCap contacts in its slice drains them over
+// further passes; a grain's later lanes queue nothing until its earlier ones
+// are drained, so the order holds.  A NaN candidate (a diverged state) is
+// not queued: it makes the force (and, from a higher id, the energy) NaN
+// whatever the order, which is applied at the end.  Blocks are one wave.
+template <int MODE, bool ENERGY, bool HIST, int L>
+__global__ __launch_bounds__(64) void k_force_q(
+        const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
+        const float4* __restrict__ Sv, const float4* __restrict__ Np,
+        const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
+        float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
+        BlockPart* __restrict__ parts, double* __restrict__ ewave, HistoryIO hist, int n) {
+    static_assert(MODE != kModeRHS && (L == 1 || L == 2 || L == 4), "queued force pass variants");
+    constexpr int G = 64 / L;  // grains per wave
+    const Params& P = *Pp;
+    __shared__ uint32_t s_list[kQueueCap][64];  // per lane: cell-sorted slots of listed contacts
+    __shared__ int s_q[64];                     // per lane: queue index of its first listed contact
+    __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
+    __shared__ float4 s_rec[3][64];             // per queue entry: force addends, energy addends
+    __shared__ int2 s_gh[G];                    // per grain: old spring list (index, count)
+    __shared__ float s_cpair[kMaxMat * kMaxMat];
+    // A lone simulation's pair damping table, in LDS: one lookup per contact.
+    const bool lds_tab = P.mtab == nullptr;
+    if (lds_tab) {
+        for (int k = threadIdx.x; k < kMaxMat * kMaxMat; k += 64) s_cpair[k] = P.c_pair[k];
+        __syncthreads();
+    }
+    const int lane = static_cast<int>(threadIdx.x);
+    const int g = lane / L, sub = lane % L;
+    const int lb = logical_block();
+    const int p = lb * G + g;
+    const bool live = p < n;
+
+    const int par = HIST ? ctl->hist_parity : 0;
+    const float4* __restrict__ h_in = hist.buf[par];
+    float4* __restrict__ h_out = hist.buf[par ^ 1];
+    // Spring of `partner` in the old list (index hs, h_old entries), zero for
+    // a new contact.  The old list was written in contact order, so slot
+    // `guess` (the contact's new slot) normally holds it.
+    auto h_find = [&](int hs, int h_old, int partner, int guess, float xi[3]) __attribute__((always_inline)) {
+        xi[0] = xi[1] = xi[2] = 0.f;
+        const float4 hg = h_in[static_cast<size_t>(min(guess, kHist - 1)) * n + hs];  // always in bounds
+        if (guess < h_old && __float_as_int(hg.w) == partner) {
+            xi[0] = hg.x; xi[1] = hg.y; xi[2] = hg.z;
+            return;
+        }
+        for (int t = 0; t < h_old; ++t) {
+            if (t == guess) continue;
+            const float4 h = h_in[static_cast<size_t>(t) * n + hs];
+            if (__float_as_int(h.w) == partner) {
+                xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
+                return;
+            }
+        }
+    };
+
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi4 = pi;
+    float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
+    int id = 0, mat = 0, member = 0;
+    float mu = 0.f, m = 0.f;
+    float f[3] = {0.f, 0.f, 0.f};
+    float e_grain = 0.f;
+    int hs = 0, h_old = 0, wc = 0;  // old spring list; wall contacts
+    unsigned ncont = 0;
+    if (live) {
+        pi = Sp[p];
+        vi4 = Sv[p];
+        x[0] = pi.x; x[1] = pi.y; x[2] = pi.z;
+        v[0] = vi4.x; v[1] = vi4.y; v[2] = vi4.z;
+        id = __float_as_int(pi.w);
+        mat = grain_mat(vi4.w);
+        member = member_of(P, vi4.w);
+        mu = P.members == 1 ? P.mu : P.mmu[member];
+        m = mat_mass(P, member, mat);
+        if (HIST) {
+            hs = hist.hidx[p];
+            h_old = hist.cnt[par][hs];
+        }
+        if (ENERGY) e_grain = grain_energy_self(P, x, v, m);
+        // Gravity and the six walls, as k_force (every lane of the grain).
+        const float c_wall = mat_cwall(P, member, mat);
+        f[2] -= m * P.gravity;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const float dist = side == 0 ? x[a] - P.wlo[a] - P.radius : P.whi[a] - x[a] - P.radius;
+                const float fw = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
+                if (side == 0) f[a] += fw;
+                else f[a] -= fw;
+                if (dist < 0.f) {
+                    f[a] -= c_wall * v[a];
+                    if (HIST) {
+                        float nrm[3] = {0.f, 0.f, 0.f};
+                        nrm[a] = side == 0 ? 1.f : -1.f;
+                        float vn = 0.f;
+                        vn += v[0] * nrm[0];
+                        vn += v[1] * nrm[1];
+                        vn += v[2] * nrm[2];
+                        float xi[3];
+                        h_find(hs, h_old, wall_partner(a, side), wc, xi);
+                        if (ENERGY) {
+                            float s2 = 0.f;
+                            s2 += xi[0] * xi[0];
+                            s2 += xi[1] * xi[1];
+                            s2 += xi[2] * xi[2];
+                            e_grain += 0.5f * P.kt * s2;
+                        }
+                        spring_friction(P, mu, f, nrm, v, vn, fw, c_wall, xi);
+                        if (sub == 0 && wc < kHist) {
+                            h_out[static_cast<size_t>(wc) * n + p] =
+                                    make_float4(xi[0], xi[1], xi[2], __int_as_float(wall_partner(a, side)));
+                        }
+                    } else {
+                        float vt[3];
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) vt[b] = (b == a) ? 0.f : v[b];
+                        viscous_friction(f, vt, mu * fw, c_wall);
+                    }
+                    ++wc;
+                }
+            }
+        }
+    }
+    if (sub == 0) s_gh[g] = make_int2(hs, h_old);
+
+    const uint32_t kcap = static_cast<uint32_t>(P.kcap);
+    int cx = 0, cy = 0, cz = 0;
+    MemberCells mc{};
+    if (live) {
+        mc = member_cells(P, member);
+        grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
+    }
+    auto column = [&](int c) __attribute__((always_inline)) {
+        uint4 q4;
+        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+        return q4;
+    };
+    // This lane's slice [s_lo, s_hi) of the grain's candidate sequence.
+    uint32_t s_lo = 0u, s_hi = 0xFFFFFFFFu;
+    if (L > 1 && live) {
+        uint32_t total = 0;
+        for (int c = 0; c < 9; ++c) total += column_length(column(c), kcap);
+        s_lo = total * static_cast<uint32_t>(sub) / L;
+        s_hi = total * static_cast<uint32_t>(sub + 1) / L;
+    }
+    int cons = 0;     // this lane's contacts already queued
+    int seen = 0;     // touching candidates met by the current scan
+    bool nan_f = false, nan_e = false;
+    auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
+        const float d0 = x[0] - px, d1 = x[1] - py, d2 = x[2] - pz;
+        float s2 = 0.f;
+        s2 += d0 * d0;
+        s2 += d1 * d1;
+        s2 += d2 * d2;
+        if (!(s2 >= P.touch2) && q != static_cast<uint32_t>(p)) {
+            if (s2 < P.touch2) {
+                if (seen >= cons && seen < cons + kQueueCap) s_list[seen - cons][lane] = q;
+                ++seen;
+            } else {
+                nan_f = true;
+                if (__float_as_int(Sp[q].w) > id) nan_e = true;
+            }
+        }
+    };
+
+    int total = 0;      // touching candidates in this lane's slice
+    int lane_slot = 0;  // spring slot of this lane's first contact
+    int grain_slots = 0;
+    for (int pass = 0;; ++pass) {
+        seen = 0;
+        if (live && (pass == 0 || cons < total)) walk_stencil<8, true>(Sp, kcap, s_lo, s_hi, column, probe);
+        if (pass == 0) {
+            total = seen;
+            int b = wc;
+#pragma unroll
+            for (int k = 1; k < L; ++k) {
+                const int o = __shfl(total, lane - k, 64);
+                if (sub >= k) b += o;
+            }
+            lane_slot = b;
+            grain_slots = __shfl(lane_slot + total, g * L + L - 1, 64);
+        }
+        const int listed = min(total - cons, kQueueCap);
+        const unsigned long long drained = __ballot(cons + listed == total);
+        const unsigned long long before = ((1ull << sub) - 1ull) << (g * L);
+        const int cnt = (drained & before) == before ? listed : 0;
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        const int q0 = incl - cnt;
+        const int T = __shfl(incl, 63, 64);
+        const int gq0 = __shfl(q0, g * L, 64);
+        const int gq1n = __shfl(q0, ((g + 1) * L) & 63, 64);
+        const int gq1 = (g + 1) * L < 64 ? gq1n : T;
+        s_q[lane] = q0;
+        s_slot[lane] = lane_slot + cons;
+        __syncthreads();
+        for (int base = 0; base < T; base += 64) {
+            const int e = base + lane;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+            if (e < T) {
+                int src = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    if (s_q[src + step] <= e) src += step;
+                }
+                const int j = e - s_q[src];
+                const uint32_t q = s_list[j][src];
+                const int slot = s_slot[src] + j;
+                const int og = src / L;
+                const int po = lb * G + og;
+                const float4 opi = Sp[po], ovi = Sv[po];
+                const float ox[3] = {opi.x, opi.y, opi.z}, ov[3] = {ovi.x, ovi.y, ovi.z};
+                const int oid = __float_as_int(opi.w);
+                const int omat = grain_mat(ovi.w);
+                const int omember = member_of(P, ovi.w);
+                const float omu = P.members == 1 ? P.mu : P.mmu[omember];
+                const float4 pj = Sp[q];
+                float d0, d1, d2;
+                const float norm = pair_norm(ox, pj.x, pj.y, pj.z, d0, d1, d2);
+                const int jd = __float_as_int(pj.w);
+                if (ENERGY && jd > oid) r0.w = contact_energy(P, 2.f * P.radius - norm);
+                const float dist = norm - P.radius - P.radius;
+                const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
+                const float fp = fmaxf(0.f, fminf(P.max_force, P.kneg * dist));
+                r0.x = fp * nrm[0];
+                r0.y = fp * nrm[1];
+                r0.z = fp * nrm[2];
+                ++ncont;
+                const float4 vj = Sv[q];
+                const float c = lds_tab ? s_cpair[omat * kMaxMat + grain_mat(vj.w)]
+                                        : mat_cpair(P, omember, omat, grain_mat(vj.w));
+                const float vr[3] = {ov[0] - vj.x, ov[1] - vj.y, ov[2] - vj.z};
+                float vn = 0.f;
+                vn += vr[0] * nrm[0];
+                vn += vr[1] * nrm[1];
+                vn += vr[2] * nrm[2];
+                r1.x = -(c * vn * nrm[0]);
+                r1.y = -(c * vn * nrm[1]);
+                r1.z = -(c * vn * nrm[2]);
+                if (HIST) {
+                    const int2 gh = s_gh[og];
+                    float xi[3];
+                    h_find(gh.x, gh.y, jd, slot, xi);
+                    if (ENERGY && jd > oid) {
+                        float s2 = 0.f;
+                        s2 += xi[0] * xi[0];
+                        s2 += xi[1] * xi[1];
+                        s2 += xi[2] * xi[2];
+                        r1.w = 0.5f * P.kt * s2;
+                    }
+                    float ft[3] = {0.f, 0.f, 0.f};
+                    spring_friction(P, omu, ft, nrm, vr, vn, fp, c, xi);
+                    r2.x = ft[0];
+                    r2.y = ft[1];
+                    r2.z = ft[2];
+                    if (slot < kHist) {
+                        h_out[static_cast<size_t>(slot) * n + po] =
+                                make_float4(xi[0], xi[1], xi[2], __int_as_float(jd));
+                    }
+                } else {
+                    float ft[3] = {0.f, 0.f, 0.f};
+                    float vt[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) vt[a] = vr[a] - vn * nrm[a];
+                    viscous_friction(ft, vt, omu * fp, c);
+                    r2.x = ft[0];
+                    r2.y = ft[1];
+                    r2.z = ft[2];
+                }
+            }
+            s_rec[0][lane] = r0;
+            s_rec[1][lane] = r1;
+            s_rec[2][lane] = r2;
+            __syncthreads();
+            const int r_lo = max(gq0, base), r_hi = min(gq1, base + 64);
+            for (int r = r_lo; r < r_hi; ++r) {
+                const float4 a0 = s_rec[0][r - base], a1 = s_rec[1][r - base], a2 = s_rec[2][r - base];
+                f[0] += a0.x;
+                f[1] += a0.y;
+                f[2] += a0.z;
+                f[0] += a1.x;
+                f[1] += a1.y;
+                f[2] += a1.z;
+                f[0] += a2.x;
+                f[1] += a2.y;
+                f[2] += a2.z;
+                if (ENERGY) {
+                    e_grain += a0.w;
+                    e_grain += a1.w;
+                }
+            }
+            __syncthreads();
+        }
+        cons += cnt;
+        if (__ballot(cons < total) == 0ull) break;
+    }
+    const unsigned long long gmask = ((L == 1) ? 1ull : ((1ull << L) - 1ull)) << (g * L);
+    if (__ballot(nan_f) & gmask) {
+        const float qnan = __builtin_nanf("");
+        f[0] += qnan;
+        f[1] += qnan;
+        f[2] += qnan;
+    }
+    if (ENERGY && (__ballot(nan_e) & gmask)) e_grain += __builtin_nanf("");
+
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    unsigned accel_bits = 0;
+    int lane_member = -1;
+    if (live && sub == 0) {
+        if (HIST) hist.cnt[par ^ 1][p] = min(grain_slots, kHist);
+        const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
+        float vn[3], xn[3];
+        if (MODE == kModeSIE) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) vn[k] = v[k] + P.dt * a[k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = x[k] + P.dt * vn[k];
+        } else {
+            const float4 bp = Np[p], bv = Nv[p];
+            const float x0[3] = {bp.x, bp.y, bp.z}, v0[3] = {bv.x, bv.y, bv.z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                vn[k] = v0[k] + P.dt * a[k];
+                xn[k] = x0[k] + P.dt * vn[k];
+            }
+        }
+        Tp[p] = make_float4(xn[0], xn[1], xn[2], pi.w);
+        Tv[p] = make_float4(vn[0], vn[1], vn[2], vi4.w);
+        lane_member = member;
+        int nx, ny, nz;
+        grain_cell(P, xn[0], xn[1], xn[2], nx, ny, nz);
+        key_out[p] = pack_cell(nx, ny, nz);
+        lo[0] = hi[0] = nx;
+        lo[1] = hi[1] = ny;
+        lo[2] = hi[2] = nz;
+        float s2 = 0.f;
+        s2 += a[0] * a[0];
+        s2 += a[1] * a[1];
+        s2 += a[2] * a[2];
+        const float acc = sqrtf(s2);
+        if (acc == acc) accel_bits = __float_as_uint(acc);
+        ncont += static_cast<unsigned>(wc);
+    }
+    if (ENERGY) {
+        const double ew = group_energy<L>(live ? static_cast<double>(e_grain) : 0.0);
+        if (sub == 0 && (g & (kEnergyGroup - 1)) == 0 && live) ewave[p / kEnergyGroup] = ew;
+    }
+    store_block_part(parts, lb, lane_member, lo, hi, accel_bits, 0.0, false, ncont);
 }
 
 // Largest CIC deposit magnitude of one grain as float bits (0 for a
@@ -738,6 +1230,82 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
     }
 }
 
+// k_finalize for a lone simulation's force pass: one pass over the partials
+// and the energy groups, one block-wide combine.  The sums run in the order
+// of the general path (per thread strided, then waves in order), so the bits
+// are the same.
+__device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
+                                const double* __restrict__ ewave, double* __restrict__ energy_log,
+                                int flags) {
+    __shared__ int s_iv[16][8];
+    __shared__ double s_dv[16][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nw = blockDim.x >> 6;
+    unsigned am = 0u;
+    unsigned long long csum = 0;
+    int bx[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
+    for (int b = tid; b < nparts; b += blockDim.x) {
+        const BlockPart& bp = parts[b];
+        am = max(am, bp.accel);
+        csum += bp.contacts;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bx[k] = min(bx[k], bp.box0[k]);
+            bx[3 + k] = max(bx[3 + k], bp.box0[3 + k]);
+        }
+    }
+    double e = 0.0;
+    if (flags & (kFinEnergyLog | kFinEnergyLast)) {
+        const int ng = (P.n + kEnergyGroup - 1) / kEnergyGroup;
+        for (int w = tid; w < ng; w += blockDim.x) e += ewave[w];
+    }
+    e = wave_sum(e);
+    const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
+    am = wave_max_u(am);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bx[k] = wave_min(bx[k]);
+        bx[3 + k] = wave_max(bx[3 + k]);
+    }
+    if (lane == 0) {
+        s_dv[wave][0] = e;
+        s_dv[wave][1] = cw;
+        s_iv[wave][0] = static_cast<int>(am);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s_iv[wave][1 + k] = bx[k];
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double total = 0.0, contacts = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        total += s_dv[w][0];
+        contacts += s_dv[w][1];
+        am = max(am, static_cast<unsigned>(s_iv[w][0]));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bx[k] = min(bx[k], s_iv[w][1 + k]);
+            bx[3 + k] = max(bx[3 + k], s_iv[w][4 + k]);
+        }
+    }
+    if (flags & kFinBox) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.mbox[k] = bx[k];
+        P.mcell[0] = 0u;
+        P.mcell[1] = static_cast<uint32_t>((bx[3] - bx[0] + 3) * (bx[4] - bx[1] + 3) * (bx[5] - bx[2] + 3));
+    }
+    if (flags & kFinAccel) {
+        ctl->accel_last = am;
+        ctl->force_passes = ctl->force_passes + 1;
+        ctl->contact_sum = ctl->contact_sum + static_cast<unsigned long long>(contacts);
+    }
+    if (flags & kFinEnergyLast) ctl->energy_last = total;
+    if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
+    if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
+        energy_log[ctl->energy_count] = total;
+        ctl->energy_count = ctl->energy_count + 1;
+    }
+}
+
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __restrict__ Pp, Ctl* ctl,
                                                      const BlockPart* __restrict__ parts,
                                                      int nparts, int gpp,
@@ -747,6 +1315,10 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
                                                      const double* __restrict__ ewave,
                                                      double* __restrict__ energy_log, int flags) {
     const Params& P = *Pp;
+    if (P.members == 1 && !(flags & kFinCicScale)) {
+        finalize_single(P, ctl, parts, nparts, ewave, energy_log, flags);
+        return;
+    }
     __shared__ int s_box[16][1];
     unsigned am = 0u;
     double e = 0.0;
@@ -756,7 +1328,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
         csum += parts[b].contacts;
     }
     if (flags & (kFinEnergyLog | kFinEnergyLast)) {
-        const int nw = (P.n + 63) >> 6;
+        const int nw = (P.n + kEnergyGroup - 1) / kEnergyGroup;
         for (int w = threadIdx.x; w < nw; w += blockDim.x) e += ewave[w];
     }
     const double total = block_sum(e);
@@ -892,6 +1464,22 @@ WD_INST_FORCE(kModeRHS, false, false, 256)
 WD_INST_FORCE(kModeRHS, false, true, 256)
 WD_INST_FORCE(kModeRHS, true, false, 256)
 WD_INST_FORCE(kModeRHS, true, true, 256)
+#define WD_INST_FORCE_Q(M, E, H, L)                                                              \
+    template __global__ void k_force_q<M, E, H, L>(const Params*, Ctl*, const float4*, const float4*, \
+                                                   const float4*, const float4*, const uint32_t*,     \
+                                                   float4*, float4*, uint32_t*, BlockPart*, double*,  \
+                                                   HistoryIO, int);
+#define WD_INST_FORCE_QL(M, E, H) WD_INST_FORCE_Q(M, E, H, 1) WD_INST_FORCE_Q(M, E, H, 2) WD_INST_FORCE_Q(M, E, H, 4)
+WD_INST_FORCE_QL(kModeSIE, false, false)
+WD_INST_FORCE_QL(kModeSIE, false, true)
+WD_INST_FORCE_QL(kModeSIE, true, false)
+WD_INST_FORCE_QL(kModeSIE, true, true)
+WD_INST_FORCE_QL(kModeBEP, false, false)
+WD_INST_FORCE_QL(kModeBEP, false, true)
+WD_INST_FORCE_QL(kModeBEP, true, false)
+WD_INST_FORCE_QL(kModeBEP, true, true)
+#undef WD_INST_FORCE_QL
+#undef WD_INST_FORCE_Q
 #undef WD_INST_FORCE_H
 #undef WD_INST_FORCE_B
 #undef WD_INST_FORCE

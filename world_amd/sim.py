@@ -234,13 +234,24 @@ class ParticleDynamics:
 
     # -- launch configuration (extension) ---------------------------------
     def set_force_block(self, block: int) -> None:
-        """0: autotune on the next long take_step; 64 / 128 / 256: fixed."""
+        """0: autotune on the next long take_step; 64 / 128 / 256: fixed,
+        one lane per grain."""
         check(lib().wd_set_force_block(self._h, int(block)))
 
     def force_block(self) -> int:
         v = C.c_int(0)
         check(lib().wd_get_force_block(self._h, C.byref(v)))
         return v.value
+
+    def set_force_variant(self, block: int, lanes: int) -> None:
+        """(0, 0): autotune; (64|128|256, 0): one lane per grain; (64, 1|2|4):
+        one-wave blocks with contacts spread over the lanes, `lanes` per grain."""
+        check(lib().wd_set_force_variant(self._h, int(block), int(lanes)))
+
+    def force_variant(self) -> tuple[int, int]:
+        b, l = C.c_int(0), C.c_int(0)
+        check(lib().wd_get_force_variant(self._h, C.byref(b), C.byref(l)))
+        return b.value, l.value
 
     def contact_stats(self) -> tuple[int, int]:
         """(force passes, contacts summed over them) since construction."""
