@@ -1,13 +1,14 @@
-"""The batched driver counterparts (world_amd/bin/dataset, chaos,
-model_problem) write the reference drivers' file formats, and the batched
-runs write exactly the bytes of the sequential ones (one simulation at a
-time, as the reference drivers loop).
+"""The driver counterparts (world_amd/bin/dataset, chaos, model_problem,
+stability_and_accuracy) write the reference drivers' file formats, and the
+batched runs write exactly the bytes of the sequential ones (one simulation
+at a time, as the reference drivers loop).
 
 Layouts checked here are the reference's own: grids.bin (magic RTPGRD01,
 int32[7] header, checkpoint times, [R][K][4][nx][ny][nz] float32; documented
 in surrogate/dataset_io.py:8-21 and written by src/dataset.cpp:161-214),
 manifest.csv (src/dataset.cpp:176-226), divergence.csv and final_fields.bin
-(src/chaos.cpp:134-218), history.dat (src/model_problem.cpp:108-142)."""
+(src/chaos.cpp:134-218), history.dat (src/model_problem.cpp:108-142),
+stability_results.dat (src/stability_and_accuracy.cpp:251-267)."""
 import csv
 import subprocess
 from pathlib import Path
@@ -183,3 +184,57 @@ def test_model_problem_history_layout(tmp_path):
         np.testing.assert_array_equal(data[:, 0], np.arange(201))
         np.testing.assert_allclose(data[0, 2:], [-0.5, 0.0, -0.98999, 1.0, 0.0, 0.0], rtol=1e-6)
         assert data[-1, 5] < 1.0  # friction has slowed the grain
+
+
+def sweep_rows(path):
+    rows = []
+    for line in Path(path).read_text().splitlines():
+        if line.startswith("#"):
+            assert line == "# dt max_force status(0=unstable,1=steady,2=timed_out) energy"
+            continue
+        dt, mf, st, e = line.split()
+        rows.append((float(dt), float(mf), int(st), float(e)))
+    return rows
+
+
+def test_stability_sweep_classifies_each_pair_in_its_own_process(tmp_path):
+    """The stability sweep (src/stability_and_accuracy.cpp) runs every
+    (dt, max_force) pair in a child process and writes
+    stability_results.dat.  A 0.2 s step carries the falling cube out of the
+    domain (U), a 0.2 ms step keeps it in bounds (S); in steady_state mode a
+    run still falling at the end is timed out (T), and a run whose grid
+    overflows ends its process and is counted unstable (U)."""
+    base = BLOB.split("dataset:")[0]
+    fixed = base + """stability:
+  mode: fixed_time
+  sim_time: 0.4
+sweep:
+  dt: [0.0002, 0.2]
+  max_force: [2000.0, 4000.0]
+"""
+    (tmp_path / "fixed.yaml").write_text(fixed)
+    r = run([str(BIN / "stability_and_accuracy"), "fixed.yaml", "fixed"], tmp_path)
+    assert "Legend: U = unstable, S = steady, T = timed out" in r.stdout
+    rows = sweep_rows(tmp_path / "fixed" / "stability_results.dat")
+    assert [(round(d, 6), m) for d, m, _, _ in rows] == [(0.0002, 2000.0), (0.0002, 4000.0), (0.2, 2000.0), (0.2, 4000.0)]
+    assert [s for _, _, s, _ in rows] == [1, 1, 0, 0]
+    assert (tmp_path / "fixed" / "stability_plot.gp").read_text().count("set object") == 4
+
+    # dt 0.2 ms: omega dt = 0.32 for k = max_force / r, well inside the damped
+    # semi-implicit Euler limit (at 1 ms the jittered cube's contacts blow up)
+    steady = base.replace("  particles_per_cell: 8\n", "  particles_per_cell: 1\n") + """stability:
+  mode: steady_state
+  sim_time: 0.004
+  acceleration_cutoff: 0.01
+  steps_per_steadiness_check: 10
+sweep:
+  dt: [0.0002]
+  max_force: [2000.0]
+"""
+    packed = steady.replace("  jitter: 0.25\n", "  jitter: 0.25\n  lattice_spacing: 0.5\n")
+    (tmp_path / "falling.yaml").write_text(steady.replace("  particles_per_cell: 1\n", "  particles_per_cell: 8\n"))
+    (tmp_path / "packed.yaml").write_text(packed)
+    run([str(BIN / "stability_and_accuracy"), "falling.yaml", "falling"], tmp_path)
+    assert [s for _, _, s, _ in sweep_rows(tmp_path / "falling" / "stability_results.dat")] == [2]
+    run([str(BIN / "stability_and_accuracy"), "packed.yaml", "packed"], tmp_path)
+    assert [s for _, _, s, _ in sweep_rows(tmp_path / "packed" / "stability_results.dat")] == [0]

@@ -67,7 +67,7 @@ def build_cpp_tests() -> Path:
     return out
 
 
-DRIVERS = ("dataset", "chaos", "model_problem")
+DRIVERS = ("dataset", "chaos", "model_problem", "stability_and_accuracy")
 
 
 def build_drivers() -> list[Path]:
