@@ -1618,49 +1618,121 @@ __device__ __forceinline__ void axis_weights(float p, float lo, float hi, int si
     w1 = frac;
 }
 
+// u64 LDS accumulators of one k_cic_deposit block (nodes of its box x 4 channels).
+constexpr int kCicWindow = 2048;
+
+// Adds each grain's mass and momentum, CIC-weighted, to the 8 surrounding
+// nodes as fixed-point integers (so the sums do not depend on order).  Grains
+// are in cell order, so a block's grains cover a small box of nodes: the block
+// accumulates there in LDS and adds each touched node to the grid once.  A
+// block whose box does not fit, or that spans two ensemble members, adds to
+// the grid directly.
 __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict__ Pp,
                                                         const float4* __restrict__ Tp,
                                                         const float4* __restrict__ Tv,
                                                         unsigned long long* __restrict__ acc,
                                                         int n) {
     const Params& P = *Pp;
+    __shared__ unsigned long long s_acc[kCicWindow];
+    __shared__ int s_red[kBlock / 64][8];
     const int i = logical_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = Tp[i], v = Tv[i];
-    const int member = member_of(P, v.w);
-    const unsigned sb = P.mscale[member];
-    if (sb >= 0x7F800000u) return;  // non-finite input: the member's grid is reported as NaN
-    const float m = mat_mass(P, member, grain_mat(v.w));
-    if (m == 0.f) return;
-    const double scale =
-            ldexp(1.0, cic_shift(__uint_as_float(sb), P.mgoff[member + 1] - P.mgoff[member]));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t nodes = static_cast<size_t>(P.dgx) * P.dgy * P.dgz;
-    acc += static_cast<size_t>(member) * 4 * nodes;
-    int ix[2], iy[2], iz[2];
-    float wx[2], wy[2], wz[2];
-    axis_weights(p.x, P.lo[0], P.hi[0], P.dgx, ix[0], ix[1], wx[0], wx[1]);
-    axis_weights(p.y, P.lo[1], P.hi[1], P.dgy, iy[0], iy[1], wy[0], wy[1]);
-    axis_weights(p.z, P.lo[2], P.hi[2], P.dgz, iz[0], iz[1], wz[0], wz[1]);
-    const float mom[3] = {m * v.x, m * v.y, m * v.z};
-    #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const float w = wx[a] * wy[b] * wz[c];
-                if (w == 0.f) continue;
-                const size_t node = (static_cast<size_t>(ix[a]) * P.dgy + iy[b]) * P.dgz + iz[c];
-                atomicAdd(&acc[node], static_cast<unsigned long long>(
-                                              llrint(static_cast<double>(m * w) * scale)));
-                #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    atomicAdd(&acc[(k + 1) * nodes + node],
-                              static_cast<unsigned long long>(
-                                      llrint(static_cast<double>(mom[k] * w) * scale)));
+    bool active = false;
+    int member = 0;
+    double scale = 0.0;
+    float m = 0.f, mom[3] = {0.f, 0.f, 0.f};
+    int ix[2] = {0, 0}, iy[2] = {0, 0}, iz[2] = {0, 0};
+    float wx[2] = {0.f, 0.f}, wy[2] = {0.f, 0.f}, wz[2] = {0.f, 0.f};
+    if (i < n) {
+        const float4 p = Tp[i], v = Tv[i];
+        member = member_of(P, v.w);
+        const unsigned sb = P.mscale[member];
+        m = mat_mass(P, member, grain_mat(v.w));
+        // a non-finite member grid is reported as NaN; massless grains add nothing
+        if (sb < 0x7F800000u && m != 0.f) {
+            active = true;
+            scale = ldexp(1.0, cic_shift(__uint_as_float(sb), P.mgoff[member + 1] - P.mgoff[member]));
+            axis_weights(p.x, P.lo[0], P.hi[0], P.dgx, ix[0], ix[1], wx[0], wx[1]);
+            axis_weights(p.y, P.lo[1], P.hi[1], P.dgy, iy[0], iy[1], wy[0], wy[1]);
+            axis_weights(p.z, P.lo[2], P.hi[2], P.dgz, iz[0], iz[1], wz[0], wz[1]);
+            mom[0] = m * v.x;
+            mom[1] = m * v.y;
+            mom[2] = m * v.z;
+        }
+    }
+    // The block's node box and member range (active grains only).
+    int r[8] = {active ? ix[0] : kEmptyMin, active ? iy[0] : kEmptyMin, active ? iz[0] : kEmptyMin,
+                active ? ix[1] : kEmptyMax, active ? iy[1] : kEmptyMax, active ? iz[1] : kEmptyMax,
+                active ? member : kEmptyMin, active ? member : kEmptyMax};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = (k < 3 || k == 6) ? wave_min(r[k]) : wave_max(r[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s_red[wave][k] = r[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+            r[k] = (k < 3 || k == 6) ? min(r[k], s_red[w][k]) : max(r[k], s_red[w][k]);
+        }
+    }
+    if (r[6] > r[7]) return;  // no active grain in the block
+    const int bx = r[3] - r[0] + 1, by = r[4] - r[1] + 1, bz = r[5] - r[2] + 1;
+    const long long vol = static_cast<long long>(bx) * by * bz;
+    auto value = [&](float q, float w) __attribute__((always_inline)) {
+        return static_cast<unsigned long long>(llrint(static_cast<double>(q * w) * scale));
+    };
+    if (r[6] != r[7] || 4 * vol > kCicWindow) {
+        if (!active) return;
+        unsigned long long* g = acc + static_cast<size_t>(member) * 4 * nodes;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const float w = wx[a] * wy[b] * wz[c];
+                    if (w == 0.f) continue;
+                    const size_t node = (static_cast<size_t>(ix[a]) * P.dgy + iy[b]) * P.dgz + iz[c];
+                    atomicAdd(&g[node], value(m, w));
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) atomicAdd(&g[(k + 1) * nodes + node], value(mom[k], w));
                 }
             }
         }
+        return;
+    }
+    const int nv = static_cast<int>(vol);
+    for (int k = threadIdx.x; k < 4 * nv; k += blockDim.x) s_acc[k] = 0ull;
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const float w = wx[a] * wy[b] * wz[c];
+                    if (w == 0.f) continue;
+                    const int ln = ((ix[a] - r[0]) * by + (iy[b] - r[1])) * bz + (iz[c] - r[2]);
+                    atomicAdd(&s_acc[ln], value(m, w));
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) atomicAdd(&s_acc[(k + 1) * nv + ln], value(mom[k], w));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long* g = acc + static_cast<size_t>(r[6]) * 4 * nodes;
+    for (int k = threadIdx.x; k < 4 * nv; k += blockDim.x) {
+        const unsigned long long v = s_acc[k];
+        if (v == 0ull) continue;
+        const int ch = k / nv, ln = k - ch * nv;
+        const int lx = ln / (by * bz), rem = ln - lx * by * bz, ly = rem / bz, lz = rem - ly * bz;
+        const size_t node = (static_cast<size_t>(r[0] + lx) * P.dgy + (r[1] + ly)) * P.dgz + (r[2] + lz);
+        atomicAdd(&g[static_cast<size_t>(ch) * nodes + node], v);
     }
 }
 
