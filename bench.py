@@ -18,6 +18,9 @@ Also measured, as secondary entries of the same JSON object:
             handle, against looping the same members one at a time;
   adjoint   the 8,192-grain window (mu 0.3, checkpoints every 20 steps): cost
             of a backward step against a forward step;
+  batched_adjoint  16 uneven members (4,096..16,384 grains) of that adjoint
+            in one batch handle, 100-step windows, checkpoints every 10
+            steps, against the same windows member by member;
   sliding   one grain launched at 1 m/s on the floor at mu 0.5: fitted
             deceleration against -mu g = -4.905 m/s^2;
   cpu_baseline  the CPU oracle on the host's cores.
@@ -196,6 +199,57 @@ def adjoint_leg(window: int = 200, every: int = 20) -> dict:
             "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd}
 
 
+def batched_adjoint_leg(members: int = 16, window: int = 100, every: int = 10) -> dict:
+    """The density-loss adjoint over 16 uneven columns (4,096 to 16,384 grains,
+    mu 0.3, aspect 2 as the 8,192-grain window) in one batch handle: 100-step
+    windows with a checkpoint every sqrt(100) steps, against the same windows
+    recorded and differentiated member by member."""
+    import numpy as np
+
+    from world_amd import workloads
+    from world_amd.sim import Ensemble, ParticleDynamics
+
+    cs = workloads.uneven_ensemble(members, seed=77, n_min=4096, n_max=16384, aspect=(2.0, 2.0),
+                                   mu=(0.3, 0.3))
+
+    def window_cost(h, cot):
+        h.take_step(20)
+        h.sync()
+        t = time.perf_counter()
+        h.take_step(window)
+        h.sync()
+        fwd = time.perf_counter() - t
+        rec = bwd = 0.0
+        for _ in range(2):  # the first round allocates the checkpoints
+            t = time.perf_counter()
+            h.adjoint_record(window, every)
+            h.sync()
+            rec = time.perf_counter() - t
+            t = time.perf_counter()
+            h.adjoint_backward(cot)
+            bwd = time.perf_counter() - t
+        return fwd, rec, bwd
+
+    rng = np.random.default_rng(0)
+    with Ensemble(cs) as ens:
+        n = ens.n
+        cot = rng.normal(size=members * 4 * ens.density_grid_nodes()).astype(np.float32)
+        fwd, rec, bwd = window_cost(ens, cot)
+    lrec = lbwd = 0.0
+    for c in cs:
+        with ParticleDynamics(c) as sim:
+            cot = rng.normal(size=4 * sim.density_grid_nodes()).astype(np.float32)
+            _, r, b = window_cost(sim, cot)
+            lrec += r
+            lbwd += b
+    return {"workload": "uneven_adjoint_16", "members": members, "grains": n, "window_steps": window,
+            "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd / window,
+            "record_ms_per_step": 1e3 * rec / window, "backward_ms_per_step": 1e3 * bwd / window,
+            "backward_over_forward": bwd / fwd,
+            "looped_record_plus_backward_ms_per_step": 1e3 * (lrec + lbwd) / window,
+            "speedup_over_looped": (lrec + lbwd) / (rec + bwd)}
+
+
 def sliding_leg() -> dict:
     import numpy as np
 
@@ -276,7 +330,7 @@ def main() -> int:
                     help="steps replayed with HIP events around every kernel for the "
                          "roofline (default: the warmup + timed steps of each workload)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the variants, ensemble, adjoint and sliding legs")
+                    help="skip the variants, ensemble, adjoint, batched adjoint and sliding legs")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 256 MiB device copy (counter calibration)")
@@ -351,6 +405,7 @@ def main() -> int:
         out["variants"] = variants
         out["ensemble"] = ensemble_leg(args.ensemble_steps)
         out["adjoint"] = adjoint_leg()
+        out["batched_adjoint"] = batched_adjoint_leg()
         out["sliding"] = sliding_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)

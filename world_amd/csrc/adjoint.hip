@@ -349,8 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     // The stored list is in contact order, so its next entry (prefetched one
     // contact ahead) normally matches; a miss searches the whole list.
     int h_cur = 0;
-    float4 h_next = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (h_old > 0) h_next = h_in[hs];
+    float4 h_next = h_in[hs];  // slot 0, whether or not in use
     auto h_find = [&](int partner, float xi[3]) __attribute__((always_inline)) {
         int found = -1;
         if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
@@ -369,7 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             if (found < 0) return -1;
         }
         h_cur = found + 1;
-        if (h_cur < h_old) h_next = h_in[static_cast<size_t>(h_cur) * n + hs];
+        // the next slot, loaded unconditionally (slots below kHist exist)
+        h_next = h_in[static_cast<size_t>(min(h_cur, kHist - 1)) * n + hs];
         return found;
     };
     auto g_next = [&](float g[3]) __attribute__((always_inline)) {
@@ -419,9 +419,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
     int listed = 0;
     auto flush = [&]() __attribute__((always_inline)) {
+        uint32_t qn = 0;
+        float4 pjn = make_float4(0.f, 0.f, 0.f, 0.f), vjn = pjn;
+        if (listed > 0) {
+            qn = s_list[0][threadIdx.x];
+            pjn = Sp[qn];
+            vjn = Sv[qn];
+        }
         for (int k = 0; k < listed; ++k) {
-            const uint32_t q = s_list[k][threadIdx.x];
-            const float4 pj = Sp[q];
+            const float4 pj = pjn, vj = vjn;
+            // the next contact's loads, unconditional (the last one's again)
+            qn = s_list[min(k + 1, listed - 1)][threadIdx.x];
+            pjn = Sp[qn];
+            vjn = Sv[qn];
             float d0, d1, d2;
             const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
             const float dist = norm - P.radius - P.radius;
@@ -430,7 +440,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
             const float raw = P.kneg * dist;
             const float fp = fmaxf(0.f, fminf(P.max_force, raw));
-            const float4 vj = Sv[q];
             const float c = mat_cpair(P, member, mat, grain_mat(vj.w));
             const float vr[3] = {v[0] - vj.x, v[1] - vj.y, v[2] - vj.z};
             float vn = 0.f;
@@ -465,17 +474,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         }
         listed = 0;
     };
-    // Candidate scan of the forward pass (touching candidates listed in LDS
-    // in stencil order, drained by flush; a grain with more than
-    // kContactList of them scans again past the ones drained).
+    // The forward pass's candidate walk: touching candidates (squared
+    // distance below touch2, exactly the contact law's dist < 0) listed in
+    // LDS in stencil order and drained by flush; a grain with more than
+    // kContactList of them walks again past the ones drained.
     int consumed = 0, seen = 0;
-    auto probe = [&](uint32_t q, float4 pj) __attribute__((always_inline)) {
-        const float e0 = x[0] - pj.x, e1 = x[1] - pj.y, e2 = x[2] - pj.z;
+    auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
+        const float e0 = x[0] - px, e1 = x[1] - py, e2 = x[2] - pz;
         float s2 = 0.f;
         s2 += e0 * e0;
         s2 += e1 * e1;
         s2 += e2 * e2;
-        if (!(s2 > P.cut2) && q != static_cast<uint32_t>(p)) {
+        if (s2 < P.touch2 && q != static_cast<uint32_t>(p)) {
             if (seen >= consumed && listed < kContactList) {
                 s_list[listed][threadIdx.x] = q;
                 ++listed;
@@ -483,29 +493,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             ++seen;
         }
     };
-    auto scan = [&](uint32_t b, uint32_t e) __attribute__((always_inline)) {
-        uint32_t q = b;
-        for (; q + 4 <= e; q += 4) {
-            const float4 a0 = Sp[q], a1 = Sp[q + 1], a2 = Sp[q + 2], a3 = Sp[q + 3];
-            probe(q, a0);
-            probe(q + 1, a1);
-            probe(q + 2, a2);
-            probe(q + 3, a3);
-        }
-        for (; q < e; ++q) probe(q, Sp[q]);
+    auto column = [&](int c) __attribute__((always_inline)) {
+        uint4 q4;
+        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+        return q4;
     };
     for (;;) {
         seen = 0;
-        for (int c = 0; c < 9; ++c) {
-            uint4 q4;
-            __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
-            if (q4.y - q4.x <= kcap && q4.z - q4.y <= kcap && q4.w - q4.z <= kcap) {
-                scan(q4.x, q4.w);
-            } else {
-                const uint32_t b3[4] = {q4.x, q4.y, q4.z, q4.w};
-                for (int z = 0; z < 3; ++z) scan(b3[z], min(b3[z + 1], b3[z] + kcap));
-            }
-        }
+        walk_stencil<4, false>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe);
         consumed += listed;
         flush();
         if (seen <= consumed) break;

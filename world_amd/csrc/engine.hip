@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/world_amd.h"
@@ -93,6 +94,8 @@ struct wd_sim {
     int force_var = -1;           // index into kForceVariants; -1 until the autotuner has chosen
     bool force_fixed = false;     // set by wd_set_force_block / wd_set_force_variant
     long long steps_run = 0;      // steps taken by this handle
+    int* bin_box_out = nullptr;   // where k_count copies the boxes it bins with (adjoint recompute)
+    uint32_t* bin_cell_out = nullptr;
     long long next_tune = 0;      // step count at which the autotuner runs (again)
     double tune_ms[sizeof(kForceVariants) / sizeof(kForceVariants[0])] = {};  // last tuning's time per variant (force pass + its reduction)
     int energy_cap = 0;
@@ -146,11 +149,15 @@ struct wd_sim {
         double2* gparam = nullptr;
         float *grid_cot = nullptr, *state_cot = nullptr;
         std::vector<void*> owned;
+        // Captured segments (recompute + contact adjoints), keyed by
+        // {steps, spring parity, lxi buffer, force variant}.
+        std::vector<std::pair<std::array<int, 4>, hipGraphExec_t>> seg_graphs;
     };
     AdjointState* adj = nullptr;
 
     void drop_adjoint() {
         if (!adj) return;
+        for (auto& g : adj->seg_graphs) (void)hipGraphExecDestroy(g.second);
         for (void* b : adj->owned) (void)hipFree(b);
         delete adj;
         adj = nullptr;
@@ -253,7 +260,7 @@ struct wd_sim {
         const int gb = grain_blocks(n);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
         go("k_count", k_count, dim3(gb), dim3(kBlock), (const Params*)dp, ctl, (const float4*)Tv, key,
-           slot, count, n);
+           slot, count, n, bin_box_out, bin_cell_out);
         go("k_scan", k_scan, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
            status);
         go("k_scatter", k_scatter, dim3(gb), dim3(kBlock), (const Params*)dp, (const uint32_t*)key,
@@ -1386,29 +1393,21 @@ int adj_run_record(wd_sim* s) {
     return s->sync();
 }
 
-// Recomputes `steps` steps from checkpoint `ck`, with the engine's bin and
-// spring buffers redirected into the slots: step k bins into slot k and
-// reads its springs from slot k, writing the next ones into slot k + 1.
-int adj_recompute(wd_sim* s, const wd_sim::Snap& ck, int steps) {
+// One segment of the backward sweep: recomputes its `steps` steps from
+// checkpoint `ck` with the engine's bin and spring buffers redirected into the
+// slots (step k bins into slot k and reads its springs from slot k, writing
+// the next ones into slot k + 1), then applies the contact adjoints of those
+// steps in reverse.  `par` is the device's spring parity at the segment start
+// and `cur` the lxi buffer the last step's adjoint reads.  Everything after
+// the checkpoint load reads only the slots and fixed buffers, so it is
+// captured once per {steps, par, cur, variant} and replayed for the other
+// segments (eagerly with kernel timing on).
+void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
     auto* A = s->adj;
-    int par = 0;
-    WD_HIP(hipStreamSynchronize(s->stream));
-    WD_HIP(hipMemcpy(&par, &s->ctl->hist_parity, sizeof(int), hipMemcpyDeviceToHost));
-    float4* const Sp0 = s->Sp;
-    float4* const Sv0 = s->Sv;
-    uint32_t* const start0 = s->start;
-    int* const hidx0 = s->hidx[0];
-    float4* const H0[2] = {s->H[0], s->H[1]};
-    int* const Hc0[2] = {s->Hc[0], s->Hc[1]};
-    s->H[par] = A->slot[0].H;
-    s->Hc[par] = A->slot[0].Hc;
-    snap_load(s, ck);
     for (int k = 0; k < steps; ++k) {
         wd_sim::BinSlot& b = A->slot[k];
-        WD_HIP(hipMemcpyAsync(b.mbox, s->d_mbox, sizeof(int) * 6 * s->members, hipMemcpyDeviceToDevice,
-                              s->stream));
-        WD_HIP(hipMemcpyAsync(b.mcell, s->d_mcell, sizeof(uint32_t) * (s->members + 1),
-                              hipMemcpyDeviceToDevice, s->stream));
+        s->bin_box_out = b.mbox;  // k_count keeps the step's boxes in the slot
+        s->bin_cell_out = b.mcell;
         s->Sp = b.Sp;
         s->Sv = b.Sv;
         s->start = b.start;
@@ -1420,16 +1419,75 @@ int adj_recompute(wd_sim* s, const wd_sim::Snap& ck, int steps) {
         s->enqueue_step();
         par ^= 1;
     }
-    s->Sp = Sp0;
-    s->Sv = Sv0;
-    s->start = start0;
-    s->hidx[0] = hidx0;
-    s->H[0] = H0[0];
-    s->H[1] = H0[1];
-    s->Hc[0] = Hc0[0];
-    s->Hc[1] = Hc0[1];
+    const int blocks = static_cast<int>((s->n + kBlock - 1) / kBlock);
+    const int gb = grain_blocks(s->n);
+    for (int k = steps - 1; k >= 0; --k) {
+        const wd_sim::BinSlot& b = A->slot[k];
+        hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
+                           (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
+                           (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
+                           (const int*)b.mbox, (const uint32_t*)b.mcell, (const float4*)A->lx,
+                           (const float4*)A->lv, (const float4*)A->lxi[cur], A->lxi[cur ^ 1],
+                           A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->gparam,
+                           s->n);
+        hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
+                           A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->lx, A->lv, s->n);
+        cur ^= 1;
+    }
+}
+
+int adj_segment(wd_sim* s, const wd_sim::Snap& ck, int steps, int par, int cur) {
+    auto* A = s->adj;
+    float4* const Sp0 = s->Sp;
+    float4* const Sv0 = s->Sv;
+    uint32_t* const start0 = s->start;
+    int* const hidx0 = s->hidx[0];
+    float4* const H0[2] = {s->H[0], s->H[1]};
+    int* const Hc0[2] = {s->Hc[0], s->Hc[1]};
+    auto restore = [&]() {
+        s->bin_box_out = nullptr;
+        s->bin_cell_out = nullptr;
+        s->Sp = Sp0;
+        s->Sv = Sv0;
+        s->start = start0;
+        s->hidx[0] = hidx0;
+        s->H[0] = H0[0];
+        s->H[1] = H0[1];
+        s->Hc[0] = Hc0[0];
+        s->Hc[1] = Hc0[1];
+    };
+    s->H[par] = A->slot[0].H;  // the checkpoint's springs go to slot 0
+    s->Hc[par] = A->slot[0].Hc;
+    snap_load(s, ck);
+    int rc = WD_OK;
+    if (s->timing) {
+        adj_segment_body(s, steps, par, cur);
+    } else {
+        const std::array<int, 4> key{steps, par, cur, s->force_var};
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : A->seg_graphs) {
+            if (g.first == key) exec = g.second;
+        }
+        if (!exec) {
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                adj_segment_body(s, steps, par, cur);
+                e = hipStreamEndCapture(s->stream, &g);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                restore();
+                return fail(WD_ERR_HIP, std::string("adjoint segment capture: ") + hipGetErrorString(e));
+            }
+            A->seg_graphs.push_back({key, exec});
+        }
+        if (hipGraphLaunch(exec, s->stream) != hipSuccess) rc = fail(WD_ERR_HIP, "adjoint segment launch");
+    }
+    restore();
     WD_HIP(hipGetLastError());
-    return WD_OK;
+    return rc;
 }
 
 }  // namespace
@@ -1508,32 +1566,25 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
     WD_HIP(hipMemsetAsync(A->lxi[1], 0, sizeof(float4) * kHist * n, s->stream));
     WD_HIP(hipMemsetAsync(A->gparam, 0, sizeof(double2) * n, s->stream));
     const int blocks = static_cast<int>((n + kBlock - 1) / kBlock);
-    const int gb = grain_blocks(s->n);
     snap_load(s, A->ckpt[A->nseg]);
     hipLaunchKernelGGL(k_adj_seed, dim3(blocks), dim3(kBlock), 0, s->stream, s->dp, s->Tp, s->Tv,
                        grid_cot ? A->grid_cot : nullptr, state_cot ? A->state_cot : nullptr, A->lx,
                        A->lv, s->n);
     const bool energy = s->energy_on;
     s->energy_on = false;  // recomputed steps append nothing to the energy log
+    // The device's spring parity, read once: a recomputed step flips it, a
+    // checkpoint load leaves it.
+    int par = 0;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(&par, &s->ctl->hist_parity, sizeof(int), hipMemcpyDeviceToHost));
     int cur = 0;
     for (int seg = A->nseg - 1; seg >= 0; --seg) {
         const int s0 = seg * A->every;
         const int K = std::min(A->nsteps, s0 + A->every) - s0;
-        rc = adj_recompute(s, A->ckpt[seg], K);
+        rc = adj_segment(s, A->ckpt[seg], K, par, cur);
         if (rc != WD_OK) break;
-        for (int k = K - 1; k >= 0; --k) {
-            const wd_sim::BinSlot& b = A->slot[k];
-            hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
-                               (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
-                               (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
-                               (const int*)b.mbox, (const uint32_t*)b.mcell, (const float4*)A->lx,
-                               (const float4*)A->lv, (const float4*)A->lxi[cur], A->lxi[cur ^ 1],
-                               A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->gparam,
-                               s->n);
-            hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
-                               A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->lx, A->lv, s->n);
-            cur ^= 1;
-        }
+        par ^= K & 1;
+        cur ^= K & 1;
     }
     s->energy_on = energy;
     if (rc != WD_OK) return rc;

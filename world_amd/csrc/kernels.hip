@@ -302,15 +302,23 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4* __restrict__ Tp
 
 // ---------------------------------------------------------------------------
 // Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
+// box_out / cell_out (null unless a caller keeps the bin of this pass): a
+// copy of the member boxes and cell offsets the pass bins with.
 __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp, Ctl* ctl,
                                                   const float4* __restrict__ Tv,
                                                   const uint32_t* __restrict__ key,
                                                   uint32_t* __restrict__ slot,
-                                                  uint32_t* __restrict__ count, int n) {
+                                                  uint32_t* __restrict__ count, int n,
+                                                  int* __restrict__ box_out,
+                                                  uint32_t* __restrict__ cell_out) {
     const Params& P = *Pp;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->tile_counter = 0;
         ctl->epoch = ctl->epoch + 1;
+    }
+    if (blockIdx.x == 0 && box_out) {
+        for (int k = threadIdx.x; k < 6 * P.members; k += blockDim.x) box_out[k] = P.mbox[k];
+        for (int k = threadIdx.x; k <= P.members; k += blockDim.x) cell_out[k] = P.mcell[k];
     }
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -1306,6 +1314,112 @@ __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __re
     }
 }
 
+// Members k_finalize reduces through LDS (more take the general path below).
+constexpr int kFinMembers = 512;
+
+// k_finalize for up to kFinMembers members: one pass over the partials, each
+// adding its first and last member's box (force pass: occupied cells; CIC
+// scale pass: the largest deposit magnitude, in [3]) to that member's entry in
+// LDS with integer min / max atomics (order-free, so the result is the general
+// path's), and the grains of a member lying wholly inside one block folded
+// one by one.  The energy and contact sums run in the order of the general
+// path, so their bits are the same.
+__device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
+                                 int gpp, const uint32_t* __restrict__ key, const float4* __restrict__ Tp,
+                                 const float4* __restrict__ Tv, const double* __restrict__ ewave,
+                                 double* __restrict__ energy_log, int flags) {
+    __shared__ int s_mb[kFinMembers][6];
+    __shared__ int s_iv[16];
+    __shared__ double s_dv[16][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nw = blockDim.x >> 6;
+    const int E = P.members;
+    const bool scale = (flags & kFinCicScale) != 0;
+    for (int k = tid; k < 6 * E; k += blockDim.x) s_mb[k / 6][k % 6] = (k % 6) < 3 ? kEmptyMin : kEmptyMax;
+    __syncthreads();
+    auto merge = [&](int m, const int* b) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (b[k] != kEmptyMin) atomicMin(&s_mb[m][k], b[k]);
+            if (b[3 + k] != kEmptyMax) atomicMax(&s_mb[m][3 + k], b[3 + k]);
+        }
+    };
+    unsigned am = 0u;
+    unsigned long long csum = 0;
+    for (int b = tid; b < nparts; b += blockDim.x) {
+        const BlockPart& bp = parts[b];
+        am = max(am, bp.accel);
+        csum += bp.contacts;
+        if (bp.m0 < 0 || !(flags & (kFinBox | kFinCicScale))) continue;
+        merge(bp.m0, bp.box0);
+        if (bp.m1 > bp.m0) merge(bp.m1, bp.box1);
+        for (int mm = bp.m0 + 1; mm < bp.m1; ++mm) {  // members wholly inside this block
+            int g[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
+            for (int i = P.mgoff[mm]; i < P.mgoff[mm + 1]; ++i) {
+                if (scale) {
+                    g[3] = max(g[3], static_cast<int>(cic_grain_bits(P, Tp[i], Tv[i])));
+                } else {
+                    int c3[3];
+                    unpack_cell(key[i], c3[0], c3[1], c3[2]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        g[k] = min(g[k], c3[k]);
+                        g[3 + k] = max(g[3 + k], c3[k]);
+                    }
+                }
+            }
+            merge(mm, g);
+        }
+    }
+    double e = 0.0;
+    if (flags & (kFinEnergyLog | kFinEnergyLast)) {
+        const int ng = (P.n + kEnergyGroup - 1) / kEnergyGroup;
+        for (int w = tid; w < ng; w += blockDim.x) e += ewave[w];
+    }
+    e = wave_sum(e);
+    const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
+    am = wave_max_u(am);
+    if (lane == 0) {
+        s_dv[wave][0] = e;
+        s_dv[wave][1] = cw;
+        s_iv[wave] = static_cast<int>(am);
+    }
+    __syncthreads();  // also completes the LDS atomics
+    if (flags & kFinBox) {
+        for (int k = tid; k < 6 * E; k += blockDim.x) P.mbox[k] = s_mb[k / 6][k % 6];
+    }
+    if (scale) {
+        for (int m = tid; m < E; m += blockDim.x) P.mscale[m] = s_mb[m][3] < 0 ? 0u : static_cast<unsigned>(s_mb[m][3]);
+    }
+    if (tid != 0) return;
+    if (flags & kFinBox) {
+        uint32_t off = 0;
+        for (int m = 0; m < E; ++m) {
+            P.mcell[m] = off;
+            const int* bx = s_mb[m];
+            off += static_cast<uint32_t>((bx[3] - bx[0] + 3) * (bx[4] - bx[1] + 3) * (bx[5] - bx[2] + 3));
+        }
+        P.mcell[E] = off;
+    }
+    double total = 0.0, contacts = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        total += s_dv[w][0];
+        contacts += s_dv[w][1];
+        am = max(am, static_cast<unsigned>(s_iv[w]));
+    }
+    if (flags & kFinAccel) {
+        ctl->accel_last = am;
+        ctl->force_passes = ctl->force_passes + 1;
+        ctl->contact_sum = ctl->contact_sum + static_cast<unsigned long long>(contacts);
+    }
+    if (flags & kFinEnergyLast) ctl->energy_last = total;
+    if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
+    if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
+        energy_log[ctl->energy_count] = total;
+        ctl->energy_count = ctl->energy_count + 1;
+    }
+}
+
 __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __restrict__ Pp, Ctl* ctl,
                                                      const BlockPart* __restrict__ parts,
                                                      int nparts, int gpp,
@@ -1317,6 +1431,10 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     const Params& P = *Pp;
     if (P.members == 1 && !(flags & kFinCicScale)) {
         finalize_single(P, ctl, parts, nparts, ewave, energy_log, flags);
+        return;
+    }
+    if (P.members <= kFinMembers) {
+        finalize_members(P, ctl, parts, nparts, gpp, key, Tp, Tv, ewave, energy_log, flags);
         return;
     }
     __shared__ int s_box[16][1];
