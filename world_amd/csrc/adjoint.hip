@@ -22,6 +22,16 @@
 //                   grain's own contact order: no atomics, deterministic.
 // CIC cotangents enter through k_adj_seed.  Everything is accumulated in FP32
 // except the parameter gradients (per-grain double sums over the window).
+//
+// Layout: within a step the adjoint arrays follow that step's cell order (a
+// lane's own entries are contiguous with its neighbours'): dL/d(state s+1) at
+// the step's cell-sorted positions, the cotangents of the springs the step
+// wrote at those positions and of the springs it read at the recorded spring
+// index, pair contacts by partner position.  k_adj_gather hands the result
+// to the previous step's order through the recorded spring index (nearly the
+// identity between steps); a checkpoint is in original order, and
+// k_adj_to_cells maps into a segment's last cell order.  The parameter
+// gradients stay in original order.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -332,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     const float m = mat_mass(P, member, mat);
 
     // SIE: v' = v + dt F/m, x' = x + dt v'.
-    const float4 lx4 = lx[id], lv4 = lv[id];
+    const float4 lx4 = lx[p], lv4 = lv[p];
     const float lvt[3] = {lv4.x + P.dt * lx4.x, lv4.y + P.dt * lx4.y, lv4.z + P.dt * lx4.z};
     float gF[3];
 #pragma unroll
@@ -343,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
 
     const int hs = hidx[p];  // the grain's index in the springs the step read
     const int h_old = h_cnt[hs];
-    for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(0.f, 0.f, 0.f, 0.f);
     int h_new = 0, kc = 0;
     // Stored spring of `partner` and its slot (-1: a new contact, zero spring).
     // The stored list is in contact order, so its next entry (prefetched one
@@ -374,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
     };
     auto g_next = [&](float g[3]) __attribute__((always_inline)) {
         if (h_new < kHist) {
-            const float4 q = lxi_next[static_cast<size_t>(h_new) * n + id];
+            const float4 q = lxi_next[static_cast<size_t>(h_new) * n + p];
             g[0] = q.x; g[1] = q.y; g[2] = q.z;
         } else {
             g[0] = g[1] = g[2] = 0.f;
@@ -402,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
                 g_next(gX);
                 contact_vjp(P, mu, nrm, v, vn, fw, fw == raw, dist, c_wall, xi, gF, gX, g_n, g_dist,
                             g_vr, g_xi, gk, gmu);
-                if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+                if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
                 gx[a] += side == 0 ? g_dist : -g_dist;
 #pragma unroll
                 for (int b = 0; b < 3; ++b) gv[b] += g_vr[b];
@@ -428,6 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         }
         for (int k = 0; k < listed; ++k) {
             const float4 pj = pjn, vj = vjn;
+            const uint32_t qk = qn;
             // the next contact's loads, unconditional (the last one's again)
             qn = s_list[min(k + 1, listed - 1)][threadIdx.x];
             pjn = Sp[qn];
@@ -451,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             g_next(gX);
             contact_vjp(P, mu, nrm, vr, vn, fp, fp == raw, dist, c, xi, gF, gX, g_n, g_dist, g_vr,
                         g_xi, gk, gmu);
-            if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + id] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+            if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
             // n = d / |d|, dist = |d| - 2r
             const float gnn = dot3(g_n, nrm);
             float gd[3];
@@ -463,9 +474,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
                 gv[a] += g_vr[a];
             }
             if (kc < kAdjContacts) {
-                cg_d[static_cast<size_t>(kc) * n + id] = make_float4(gd[0], gd[1], gd[2], __int_as_float(jd));
-                cg_v[static_cast<size_t>(kc) * n + id] = make_float4(g_vr[0], g_vr[1], g_vr[2], 0.f);
-                cg_p[static_cast<size_t>(id) * kAdjContacts + kc] = jd;
+                cg_d[static_cast<size_t>(kc) * n + p] = make_float4(gd[0], gd[1], gd[2], __int_as_float(jd));
+                cg_v[static_cast<size_t>(kc) * n + p] = make_float4(g_vr[0], g_vr[1], g_vr[2], 0.f);
+                cg_p[static_cast<size_t>(p) * kAdjContacts + kc] = static_cast<int>(qk);
             } else {
                 ctl->error = kAdjErrContacts;
             }
@@ -505,25 +516,27 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         flush();
         if (seen <= consumed) break;
     }
-    cg_cnt[id] = min(kc, kAdjContacts);
-    own_x[id] = make_float4(gx[0], gx[1], gx[2], 0.f);
-    own_v[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
+    cg_cnt[p] = min(kc, kAdjContacts);
+    own_x[p] = make_float4(gx[0], gx[1], gx[2], 0.f);
+    own_v[p] = make_float4(gv[0], gv[1], gv[2], 0.f);
     double2 gp = gparam[id];
     gp.x += static_cast<double>(gmu);
     gp.y += static_cast<double>(gk);
     gparam[id] = gp;
 }
 
-// Pass B: dL/d(state s) of grain j = its own share + minus the stored share
-// of every pair contact (i, j) that grain i evaluated, in j's contact order.
-// Partner lists are grain-major (cg_p[i * kAdjContacts + t]) so that a
-// search reads four partners per load.
+// Pass B: dL/d(state s) of the grain at cell-sorted position j = its own
+// share + minus the stored share of every pair contact (i, j) that grain i
+// evaluated, in j's contact order, written at j's position in the previous
+// step's order (hidx[j]).  Partner lists are grain-major
+// (cg_p[i * kAdjContacts + t]) so that a search reads four partners per load.
 __global__ __launch_bounds__(kBlock) void k_adj_gather(const float4* __restrict__ own_x,
                                                        const float4* __restrict__ own_v,
                                                        const float4* __restrict__ cg_d,
                                                        const float4* __restrict__ cg_v,
                                                        const int* __restrict__ cg_cnt,
                                                        const int* __restrict__ cg_p,
+                                                       const int* __restrict__ hidx,
                                                        float4* __restrict__ lx, float4* __restrict__ lv,
                                                        int n) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -555,8 +568,26 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(const float4* __restrict_
             }
         }
     }
-    lx[j] = gx;
-    lv[j] = gv;
+    const int dst = hidx[j];
+    lx[dst] = gx;
+    lv[dst] = gv;
+}
+
+// dL/d(state) and the spring cotangents from original order (a checkpoint's
+// order) into the cell order of a segment's last recomputed step (Sp: that
+// step's cell-sorted positions, the original index in .w).
+__global__ __launch_bounds__(kBlock) void k_adj_to_cells(const float4* __restrict__ Sp,
+                                                         const float4* __restrict__ lx_id,
+                                                         const float4* __restrict__ lv_id,
+                                                         const float4* __restrict__ lxi_id,
+                                                         float4* __restrict__ lx, float4* __restrict__ lv,
+                                                         float4* __restrict__ lxi, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int id = __float_as_int(Sp[p].w);
+    lx[p] = lx_id[id];
+    lv[p] = lv_id[id];
+    for (int t = 0; t < kHist; ++t) lxi[static_cast<size_t>(t) * n + p] = lxi_id[static_cast<size_t>(t) * n + id];
 }
 
 }  // namespace wd

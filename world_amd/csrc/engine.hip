@@ -143,6 +143,7 @@ struct wd_sim {
         std::vector<Snap> ckpt;
         std::vector<BinSlot> slot;  // every + 1 (the last one receives the final springs)
         float4 *lx = nullptr, *lv = nullptr, *own_x = nullptr, *own_v = nullptr;
+        float4 *lx2 = nullptr, *lv2 = nullptr;  // dL/d(state) of a segment's last step, in its cell order
         float4 *lxi[2] = {nullptr, nullptr}, *cg_d = nullptr, *cg_v = nullptr;
         int* cg_cnt = nullptr;
         int* cg_p = nullptr;  // partners of the stored pair contacts, grain-major
@@ -1398,7 +1399,8 @@ int adj_run_record(wd_sim* s) {
 // slots (step k bins into slot k and reads its springs from slot k, writing
 // the next ones into slot k + 1), then applies the contact adjoints of those
 // steps in reverse.  `par` is the device's spring parity at the segment start
-// and `cur` the lxi buffer the last step's adjoint reads.  Everything after
+// and `cur` the lxi buffer that holds the spring cotangents (in original
+// order) at the segment start.  Everything after
 // the checkpoint load reads only the slots and fixed buffers, so it is
 // captured once per {steps, par, cur, variant} and replayed for the other
 // segments (eagerly with kernel timing on).
@@ -1421,17 +1423,26 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
     }
     const int blocks = static_cast<int>((s->n + kBlock - 1) / kBlock);
     const int gb = grain_blocks(s->n);
+    // The cotangents arrive in original order (the checkpoint's); the last
+    // step's contact adjoint reads them in that step's cell order.
+    hipLaunchKernelGGL(k_adj_to_cells, dim3(blocks), dim3(kBlock), 0, s->stream,
+                       (const float4*)A->slot[steps - 1].Sp, (const float4*)A->lx, (const float4*)A->lv,
+                       (const float4*)A->lxi[cur], A->lx2, A->lv2, A->lxi[cur ^ 1], s->n);
+    cur ^= 1;
     for (int k = steps - 1; k >= 0; --k) {
         const wd_sim::BinSlot& b = A->slot[k];
+        const bool last = k == steps - 1;
         hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
                            (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
                            (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
-                           (const int*)b.mbox, (const uint32_t*)b.mcell, (const float4*)A->lx,
-                           (const float4*)A->lv, (const float4*)A->lxi[cur], A->lxi[cur ^ 1],
-                           A->own_x, A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->gparam,
-                           s->n);
+                           (const int*)b.mbox, (const uint32_t*)b.mcell,
+                           (const float4*)(last ? A->lx2 : A->lx), (const float4*)(last ? A->lv2 : A->lv),
+                           (const float4*)A->lxi[cur], A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d,
+                           A->cg_v, A->cg_cnt, A->cg_p, A->gparam, s->n);
+        // dL/d(state before step k), at the previous step's positions
         hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
-                           A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, A->lx, A->lv, s->n);
+                           A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, (const int*)b.hidx, A->lx,
+                           A->lv, s->n);
         cur ^= 1;
     }
 }
@@ -1529,6 +1540,8 @@ extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
     if (rc == WD_OK) rc = adj_alloc(s, &A->cg_p, static_cast<size_t>(kAdjContacts) * n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->lx, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->lv, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lx2, n);
+    if (rc == WD_OK) rc = adj_alloc(s, &A->lv2, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->own_x, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->own_v, n);
     if (rc == WD_OK) rc = adj_alloc(s, &A->lxi[0], static_cast<size_t>(kHist) * n);
@@ -1584,7 +1597,7 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
         rc = adj_segment(s, A->ckpt[seg], K, par, cur);
         if (rc != WD_OK) break;
         par ^= K & 1;
-        cur ^= K & 1;
+        cur ^= (K + 1) & 1;  // K steps and the move into cell order
     }
     s->energy_on = energy;
     if (rc != WD_OK) return rc;

@@ -1346,14 +1346,36 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
     };
     unsigned am = 0u;
     unsigned long long csum = 0;
-    for (int b = tid; b < nparts; b += blockDim.x) {
-        const BlockPart& bp = parts[b];
-        am = max(am, bp.accel);
-        csum += bp.contacts;
-        if (bp.m0 < 0 || !(flags & (kFinBox | kFinCicScale))) continue;
-        merge(bp.m0, bp.box0);
-        if (bp.m1 > bp.m0) merge(bp.m1, bp.box1);
-        for (int mm = bp.m0 + 1; mm < bp.m1; ++mm) {  // members wholly inside this block
+    const int rounds = (nparts + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
+    for (int r = 0; r < rounds; ++r) {  // the same trip count in every lane
+        const int b = r * static_cast<int>(blockDim.x) + tid;
+        const bool valid = b < nparts;
+        const BlockPart* bp = parts + (valid ? b : 0);
+        const int m0 = valid ? bp->m0 : -1, m1 = valid ? bp->m1 : -1;
+        if (valid) {
+            am = max(am, bp->accel);
+            csum += bp->contacts;
+        }
+        if (!(flags & (kFinBox | kFinCicScale))) continue;
+        // Consecutive partials almost always belong to one member: a wave
+        // whose partials all hold one member (and no second one) merges
+        // its reduced box with one set of atomics.
+        const int mw = __shfl(m0, 0, 64);
+        const bool one = __all(!valid || (m0 == mw && m1 == m0)) && mw >= 0;
+        if (one) {
+            int w6[6];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                w6[k] = wave_min(valid ? bp->box0[k] : kEmptyMin);
+                w6[3 + k] = wave_max(valid ? bp->box0[3 + k] : kEmptyMax);
+            }
+            if (lane == 0) merge(mw, w6);
+            continue;
+        }
+        if (m0 < 0) continue;
+        merge(m0, bp->box0);
+        if (m1 > m0) merge(m1, bp->box1);
+        for (int mm = m0 + 1; mm < m1; ++mm) {  // members wholly inside this block
             int g[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
             for (int i = P.mgoff[mm]; i < P.mgoff[mm + 1]; ++i) {
                 if (scale) {
