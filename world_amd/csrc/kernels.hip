@@ -199,8 +199,15 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
     for (int k = 0; k < 3; ++k) {
         v[k] = wave_min(in0 ? lo[k] : kEmptyMin);
         v[3 + k] = wave_max(in0 ? hi[k] : kEmptyMax);
-        v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
-        v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+        v[6 + k] = kEmptyMin;
+        v[9 + k] = kEmptyMax;
+    }
+    if (mhi != mlo && mhi >= 0) {  // block-uniform: a second member only where one starts
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
+            v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+        }
     }
     v[12] = static_cast<int>(wave_max_u(accel));
     const unsigned cw = wave_sum_u(contacts);
