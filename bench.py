@@ -337,8 +337,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; --gpus must be 1 "
-              "(multi-GPU slab decomposition is a design, not part of this build)",
+        print("bench.py: measures one GPU only; --gpus must be 1 "
+              "(the multi-GPU slab decomposition is designed, not implemented)",
               file=sys.stderr)
         return 2
 

@@ -331,3 +331,21 @@ def test_contact_count_matches_oracle(oracle_lib):
         passes, contacts = sim.contact_stats()
         np.testing.assert_array_equal(sim.state(), R.final_state())
     assert passes == 30 and contacts == want
+
+
+def test_retuning_keeps_results():
+    """The online tuner cycles every force-pass variant on the first long
+    call and again after 256 steps; 300 autotuned steps equal 300 steps with
+    the variant pinned, bit for bit."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, _ = jittered(c, seed=31)
+    with ParticleDynamics(c) as tuned, ParticleDynamics(c) as fixed:
+        tuned.set_state(st)
+        fixed.set_state(st)
+        fixed.set_force_variant(128, 0)
+        for _ in range(10):  # 30-step calls: tuning at 0 and again past 256 steps
+            tuned.take_step(30)
+            fixed.take_step(30)
+        assert tuned.force_variant() in FORCE_VARIANTS
+        np.testing.assert_array_equal(tuned.state(), fixed.state())
+        assert_springs_equal(tuned.history(), fixed.history())
