@@ -166,61 +166,25 @@ def ensemble_leg(steps: int) -> dict:
             "speedup": tl / tb, "stable": stable}
 
 
-def adjoint_leg(window: int = 200, every: int = 20) -> dict:
-    """Backward-step cost of the 8,192-grain density-loss window against a
-    forward step of the same handle."""
-    import numpy as np
-
-    from world_amd import workloads
-    from world_amd.sim import ParticleDynamics
-
-    c = workloads.column_8k(friction=0.3)
-    sim = ParticleDynamics(c)
-    sim.take_step(20)
-    sim.sync()
-    t = time.perf_counter()
-    sim.take_step(window)
-    sim.sync()
-    fwd = (time.perf_counter() - t) / window
-    G = np.random.default_rng(0).normal(size=4 * sim.density_grid_nodes()).astype(np.float32)
-    rec = bwd = 0.0
-    for _ in range(2):  # the first round allocates the checkpoints
+def adjoint_window(make, cot_of, window: int, every: int):
+    """Forward, record and backward cost of one window, all over the same
+    steps: a handle advanced 20 + window steps times a forward step over the
+    next window; a second handle records and differentiates the window
+    before it (its allocations and graph captures) and then the same window.
+    Returns seconds (forward, record, backward) for the window."""
+    with make() as a:
+        a.take_step(20 + window)
+        a.sync()
         t = time.perf_counter()
-        sim.adjoint_record(window, every)
-        sim.sync()
-        rec = (time.perf_counter() - t) / window
-        t = time.perf_counter()
-        sim.adjoint_backward(G)
-        bwd = (time.perf_counter() - t) / window
-    sim.close()
-    return {"workload": "column_8k_window", "grains": 8192, "window_steps": window,
-            "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd,
-            "record_ms_per_step": 1e3 * rec, "backward_ms_per_step": 1e3 * bwd,
-            "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd}
-
-
-def batched_adjoint_leg(members: int = 16, window: int = 100, every: int = 10) -> dict:
-    """The density-loss adjoint over 16 uneven columns (4,096 to 16,384 grains,
-    mu 0.3, aspect 2 as the 8,192-grain window) in one batch handle: 100-step
-    windows with a checkpoint every sqrt(100) steps, against the same windows
-    recorded and differentiated member by member."""
-    import numpy as np
-
-    from world_amd import workloads
-    from world_amd.sim import Ensemble, ParticleDynamics
-
-    cs = workloads.uneven_ensemble(members, seed=77, n_min=4096, n_max=16384, aspect=(2.0, 2.0),
-                                   mu=(0.3, 0.3))
-
-    def window_cost(h, cot):
+        a.take_step(window)
+        a.sync()
+        fwd = time.perf_counter() - t
+    with make() as h:
+        cot = cot_of(h)
         h.take_step(20)
         h.sync()
-        t = time.perf_counter()
-        h.take_step(window)
-        h.sync()
-        fwd = time.perf_counter() - t
         rec = bwd = 0.0
-        for _ in range(2):  # the first round allocates the checkpoints
+        for _ in range(2):
             t = time.perf_counter()
             h.adjoint_record(window, every)
             h.sync()
@@ -228,20 +192,56 @@ def batched_adjoint_leg(members: int = 16, window: int = 100, every: int = 10) -
             t = time.perf_counter()
             h.adjoint_backward(cot)
             bwd = time.perf_counter() - t
-        return fwd, rec, bwd
+    return fwd, rec, bwd
 
+
+def adjoint_leg(window: int = 200, every: int = 20) -> dict:
+    """Backward-step cost of the 8,192-grain density-loss window against a
+    forward step over the same window."""
+    import numpy as np
+
+    from world_amd import workloads
+    from world_amd.sim import ParticleDynamics
+
+    c = workloads.column_8k(friction=0.3)
+    fwd, rec, bwd = adjoint_window(
+        lambda: ParticleDynamics(c),
+        lambda h: np.random.default_rng(0).normal(size=4 * h.density_grid_nodes()).astype(np.float32),
+        window, every)
+    return {"workload": "column_8k_window", "grains": 8192, "window_steps": window,
+            "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd / window,
+            "record_ms_per_step": 1e3 * rec / window, "backward_ms_per_step": 1e3 * bwd / window,
+            "backward_over_forward": bwd / fwd, "record_plus_backward_over_forward": (rec + bwd) / fwd}
+
+
+def batched_adjoint_leg(members: int = 16, window: int = 100, every: int = 10) -> dict:
+    """The density-loss adjoint over 16 uneven columns (4,096 to 16,384 grains,
+    mu 0.3, aspect 2 as the 8,192-grain window) in one batch handle: 100-step
+    windows with a checkpoint every sqrt(100) steps, against the same windows
+    recorded and differentiated member by member.  Forward and backward are
+    timed over the same steps."""
+    import numpy as np
+
+    from world_amd import workloads
+    from world_amd.sim import Ensemble, ParticleDynamics
+
+    cs = workloads.uneven_ensemble(members, seed=77, n_min=4096, n_max=16384, aspect=(2.0, 2.0),
+                                   mu=(0.3, 0.3))
     rng = np.random.default_rng(0)
-    with Ensemble(cs) as ens:
-        n = ens.n
-        cot = rng.normal(size=members * 4 * ens.density_grid_nodes()).astype(np.float32)
-        fwd, rec, bwd = window_cost(ens, cot)
+    with Ensemble(cs) as probe:
+        n = probe.n
+    fwd, rec, bwd = adjoint_window(
+        lambda: Ensemble(cs),
+        lambda h: rng.normal(size=members * 4 * h.density_grid_nodes()).astype(np.float32),
+        window, every)
     lrec = lbwd = 0.0
     for c in cs:
-        with ParticleDynamics(c) as sim:
-            cot = rng.normal(size=4 * sim.density_grid_nodes()).astype(np.float32)
-            _, r, b = window_cost(sim, cot)
-            lrec += r
-            lbwd += b
+        _, r, b = adjoint_window(
+            lambda: ParticleDynamics(c),
+            lambda h: rng.normal(size=4 * h.density_grid_nodes()).astype(np.float32),
+            window, every)
+        lrec += r
+        lbwd += b
     return {"workload": "uneven_adjoint_16", "members": members, "grains": n, "window_steps": window,
             "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd / window,
             "record_ms_per_step": 1e3 * rec / window, "backward_ms_per_step": 1e3 * bwd / window,
