@@ -137,7 +137,8 @@ int wd_config_grain_count(const wd_config* cfg, int* n_out);
 int wd_initial_state(const wd_config* cfg, float* state_out, int* material_out, int capacity);
 
 /* Constructs a simulation: initializes on the host, uploads, sizes every
- * buffer (ParticleDynamics::ParticleDynamics, src/particle_dynamics.cu:10-94). */
+ * buffer (ParticleDynamics::ParticleDynamics, src/particle_dynamics.cu:10-94).
+ * A handle holds at most 268,435,455 (2^28 - 1) grains; more is WD_ERR_ARG. */
 int wd_create(const wd_config* cfg, wd_sim** out);
 
 /* Constructs a simulation from caller-supplied grains instead of the

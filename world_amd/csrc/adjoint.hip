@@ -367,15 +367,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
             found = h_cur;
         } else {
             xi[0] = xi[1] = xi[2] = 0.f;
-            for (int t = 0; t < h_old; ++t) {
-                const float4 h = h_in[static_cast<size_t>(t) * n + hs];
-                if (__float_as_int(h.w) == partner) {
-                    xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
-                    found = t;
-                    break;
-                }
-            }
+            found = spring_slot(h_in, n, hs, h_old, partner);
             if (found < 0) return -1;
+            const float4 h = h_in[static_cast<size_t>(found) * n + hs];
+            xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
         }
         h_cur = found + 1;
         // the next slot, loaded unconditionally (slots below kHist exist)

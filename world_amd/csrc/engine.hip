@@ -817,6 +817,8 @@ int create_with(const wd_config* cfgs, int members, const std::vector<int>& goff
     }
     const int n = goff[members];
     if (members > (1 << 26)) return fail(WD_ERR_ARG, "too many ensemble members");
+    // Kernels address grain arrays with 32-bit byte offsets.
+    if (n >= (1 << 28)) return fail(WD_ERR_ARG, "at most 268,435,455 grains per handle");
     for (int m = 0; m < members; ++m) {
         for (int i = goff[m]; i < goff[m + 1]; ++i) {
             if (mat[i] < 0 || mat[i] >= cfgs[m].num_masses) return fail(WD_ERR_ARG, "material id out of range");

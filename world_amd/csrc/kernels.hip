@@ -78,6 +78,35 @@ __device__ __forceinline__ uint32_t column_length(const uint4& q4, uint32_t kcap
                                    : q4.w - q4.x;
 }
 
+// Position of cell-sorted slot q: a 32-bit byte offset from the array base
+// (grain counts stay below 2^28), so the load takes the scalar-base form and
+// no 64-bit address is formed per candidate.
+__device__ __forceinline__ const float* slot_pos(const float4* __restrict__ Sp, uint32_t q) {
+    return reinterpret_cast<const float*>(reinterpret_cast<const char*>(Sp) + (q << 4));
+}
+
+// Slot of `partner` in an old spring list (slot-major with stride n, grain
+// index hs, h_old entries), or -1 when it holds no such partner.  The
+// partner ids are loaded eight at a time, all in flight together, rather than
+// one dependent load per slot.
+__device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int n, int hs, int h_old,
+                                           int partner) {
+    const int* __restrict__ ids = reinterpret_cast<const int*>(h_in) + 3;  // .w of each entry
+    for (int t0 = 0; t0 < h_old; t0 += 8) {
+        int w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = min(t0 + k, h_old - 1);  // slots past the list reload its last entry
+            w[k] = ids[(static_cast<size_t>(t) * n + hs) * 4];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (t0 + k < h_old && w[k] == partner) return t0 + k;
+        }
+    }
+    return -1;
+}
+
 // Walks one grain's stencil candidates in the reference's neighbor-row order
 // (columns dx, then dy; the three cells dz = -1, 0, +1 of a column; at most
 // kcap grains of a cell) and calls probe(q, x, y, z) with the cell-sorted slot
@@ -89,6 +118,28 @@ template <int BATCH, bool GROUPED, class Column, class Probe>
 __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint32_t kcap, uint32_t s_lo,
                                              uint32_t s_hi, Column column, Probe probe) {
     uint32_t off = 0;  // sequence position of the run's first candidate
+    // Candidates b0 + [0, len): a column with no cell over capacity.
+    auto run1 = [&](uint32_t b0, uint32_t len) __attribute__((always_inline)) {
+        const uint32_t i0 = min(s_lo > off ? s_lo - off : 0u, len);
+        const uint32_t i1 = s_hi > off ? min(s_hi - off, len) : 0u;
+        for (uint32_t i = i0; i < i1; i += BATCH) {
+            float px[BATCH], py[BATCH], pz[BATCH];
+            // Every slot loads (one past the run reloads the run's last
+            // candidate), so that all loads are in flight together.
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                const float* s = slot_pos(Sp, b0 + min(i + k, i1 - 1));
+                px[k] = s[0];
+                py[k] = s[1];
+                pz[k] = s[2];
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                if (i + k < i1) probe(b0 + i + k, px[k], py[k], pz[k]);
+            }
+        }
+        off += len;
+    };
     // Candidates b0 + [0, l0), b1 + [0, l1), b2 + [0, l2) in that order.
     auto run3 = [&](uint32_t b0, uint32_t l0, uint32_t b1, uint32_t l1, uint32_t b2, uint32_t l2)
             __attribute__((always_inline)) {
@@ -105,8 +156,7 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             for (int k = 0; k < BATCH; ++k) {
                 const uint32_t t = i + k;
                 q[k] = t < l0 ? b0 + t : (t < l0 + l1 ? b1 + (t - l0) : b2 + (t - l0 - l1));
-                const uint32_t ql = t < i1 ? q[k] : q[0];
-                const float* s = reinterpret_cast<const float*>(Sp + ql);
+                const float* s = slot_pos(Sp, t < i1 ? q[k] : q[0]);
                 px[k] = s[0];
                 py[k] = s[1];
                 pz[k] = s[2];
@@ -125,7 +175,7 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             const uint4 a = next;
             if (c < 8) next = column(c + 1);
             if (!column_capped(a, kcap)) {
-                run3(a.x, a.w - a.x, a.w, 0u, a.w, 0u);
+                run1(a.x, a.w - a.x);
             } else {
                 run3(a.x, min(a.y - a.x, kcap), a.y, min(a.z - a.y, kcap), a.z, min(a.w - a.z, kcap));
             }
@@ -337,10 +387,13 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
 }
 
 // Pass 2: start[c] = sum of count[0..c) over the concatenated member
-// sub-grids, written up to and including index `cells`; counts are zeroed for the next pass and
-// over-capacity cells are added to the lifetime overflow total.  Tiles are
-// handed out by an atomic counter, so a tile only waits on tiles already
-// owned by running blocks (forward progress without co-residency limits).
+// sub-grids, written up to and including index `cells`; counts are zeroed
+// for the next pass and over-capacity cells are added to the lifetime
+// overflow total.  A tile only waits on lower tiles: block b's first tile is
+// b, and a lower-indexed block is dispatched first on its XCD; a tile past
+// the grid is claimed by a running block, which reaches it once its current
+// tile (waiting only on lower ones) is done.  So every wait ends, without
+// co-residency limits.
 __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict__ Pp, Ctl* ctl,
                                                        uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ start,
@@ -355,11 +408,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     __shared__ uint32_t s_prefix;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (static_cast<int>(blockIdx.x) >= ntiles) return;  // never takes a tile
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(&ctl->tile_counter, 1u));
-        __syncthreads();
-        const int tile = s_tile;
-        if (tile >= ntiles) break;
+    // A block scans tile blockIdx.x first, then (when the grid has fewer
+    // blocks than tiles) tiles handed out by the counter past the grid, each
+    // claimed once the block is free.
+    const bool more = ntiles > static_cast<int>(gridDim.x);
+    for (int tile = static_cast<int>(blockIdx.x); tile < ntiles;) {
         const size_t base = static_cast<size_t>(tile) * kScanTile + threadIdx.x * kScanItems;
         uint32_t v[kScanItems];
         const uint4* src = reinterpret_cast<const uint4*>(count + base);
@@ -468,7 +521,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
             const uint4 q = src[k];
             if (q.x | q.y | q.z | q.w) cz[k] = make_uint4(0u, 0u, 0u, 0u);
         }
+        if (!more) break;
+        if (threadIdx.x == 0) s_tile = static_cast<int>(gridDim.x + atomicAdd(&ctl->tile_counter, 1u));
         __syncthreads();
+        tile = s_tile;
+        __syncthreads();  // every thread has read the claim before the next one is written
     }
 }
 
@@ -604,16 +661,10 @@ __global__ __launch_bounds__(BS, 5) void k_force(
                 xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
             } else {
                 xi[0] = xi[1] = xi[2] = 0.f;
-                int found = -1;
-                for (int t = 0; t < h_old; ++t) {
-                    const float4 h = h_in[static_cast<size_t>(t) * n + hs];
-                    if (__float_as_int(h.w) == partner) {
-                        xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
-                        found = t;
-                        break;
-                    }
-                }
+                const int found = spring_slot(h_in, n, hs, h_old, partner);
                 if (found < 0) return;  // a new contact: the spring starts at zero
+                const float4 h = h_in[static_cast<size_t>(found) * n + hs];
+                xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
                 h_cur = found;
             }
             ++h_cur;
@@ -842,7 +893,7 @@ __global__ __launch_bounds__(BS, 5) void k_force(
 //            force and energy addends in LDS, and the grain's lanes add them
 //            in queue order, which is the order in which k_force adds them:
 //            the sums are the same bits.
-// A lane with more than kQueueCap contacts in its slice drains them over
+// A lane with more than QCAP contacts in its slice drains them over
 // further passes; a grain's later lanes queue nothing until its earlier ones
 // are drained, so the order holds.  A NaN candidate (a diverged state) is
 // not queued: it makes the force (and, from a higher id, the energy) NaN
@@ -856,8 +907,9 @@ __global__ __launch_bounds__(64) void k_force_q(
         BlockPart* __restrict__ parts, double* __restrict__ ewave, HistoryIO hist, int n) {
     static_assert(MODE != kModeRHS && (L == 1 || L == 2 || L == 4), "queued force pass variants");
     constexpr int G = 64 / L;  // grains per wave
+    constexpr int QCAP = kQueueCap;
     const Params& P = *Pp;
-    __shared__ uint32_t s_list[kQueueCap][64];  // per lane: cell-sorted slots of listed contacts
+    __shared__ uint32_t s_list[QCAP][64];       // per lane: cell-sorted slots of listed contacts
     __shared__ int s_q[64];                     // per lane: queue index of its first listed contact
     __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
     __shared__ float4 s_rec[3][64];             // per queue entry: force addends, energy addends
@@ -888,13 +940,10 @@ __global__ __launch_bounds__(64) void k_force_q(
             xi[0] = hg.x; xi[1] = hg.y; xi[2] = hg.z;
             return;
         }
-        for (int t = 0; t < h_old; ++t) {
-            if (t == guess) continue;
+        const int t = spring_slot(h_in, n, hs, h_old, partner);
+        if (t >= 0) {
             const float4 h = h_in[static_cast<size_t>(t) * n + hs];
-            if (__float_as_int(h.w) == partner) {
-                xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
-                return;
-            }
+            xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
         }
     };
 
@@ -999,7 +1048,7 @@ __global__ __launch_bounds__(64) void k_force_q(
         s2 += d2 * d2;
         if (!(s2 >= P.touch2) && q != static_cast<uint32_t>(p)) {
             if (s2 < P.touch2) {
-                if (seen >= cons && seen < cons + kQueueCap) s_list[seen - cons][lane] = q;
+                if (seen >= cons && seen < cons + QCAP) s_list[seen - cons][lane] = q;
                 ++seen;
             } else {
                 nan_f = true;
@@ -1013,7 +1062,7 @@ __global__ __launch_bounds__(64) void k_force_q(
     int grain_slots = 0;
     for (int pass = 0;; ++pass) {
         seen = 0;
-        if (live && (pass == 0 || cons < total)) walk_stencil<8, true>(Sp, kcap, s_lo, s_hi, column, probe);
+        if (live && (pass == 0 || cons < total)) walk_stencil<(L == 1 ? 4 : 8), (L != 1)>(Sp, kcap, s_lo, s_hi, column, probe);
         if (pass == 0) {
             total = seen;
             int b = wc;
@@ -1025,7 +1074,7 @@ __global__ __launch_bounds__(64) void k_force_q(
             lane_slot = b;
             grain_slots = __shfl(lane_slot + total, g * L + L - 1, 64);
         }
-        const int listed = min(total - cons, kQueueCap);
+        const int listed = min(total - cons, QCAP);
         const unsigned long long drained = __ballot(cons + listed == total);
         const unsigned long long before = ((1ull << sub) - 1ull) << (g * L);
         const int cnt = (drained & before) == before ? listed : 0;
