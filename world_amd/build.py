@@ -67,7 +67,7 @@ def build_cpp_tests() -> Path:
     return out
 
 
-DRIVERS = ("dataset", "chaos", "model_problem", "stability_and_accuracy")
+DRIVERS = ("dataset", "chaos", "model_problem", "stability_and_accuracy", "broadcast")
 
 
 def build_drivers() -> list[Path]:
