@@ -1308,20 +1308,41 @@ __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __re
     unsigned am = 0u;
     unsigned long long csum = 0;
     int bx[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
-    for (int b = tid; b < nparts; b += blockDim.x) {
-        const BlockPart& bp = parts[b];
-        am = max(am, bp.accel);
-        csum += bp.contacts;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            bx[k] = min(bx[k], bp.box0[k]);
-            bx[3 + k] = max(bx[3 + k], bp.box0[3 + k]);
-        }
-    }
+    // Two partials and two energy groups per round, their loads issued
+    // together (the second index clamped, so neither load is predicated).
+    const int stride = static_cast<int>(blockDim.x);
+    const bool with_e = (flags & (kFinEnergyLog | kFinEnergyLast)) != 0;
+    const int ng = with_e ? (P.n + kEnergyGroup - 1) / kEnergyGroup : 0;
     double e = 0.0;
-    if (flags & (kFinEnergyLog | kFinEnergyLast)) {
-        const int ng = (P.n + kEnergyGroup - 1) / kEnergyGroup;
-        for (int w = tid; w < ng; w += blockDim.x) e += ewave[w];
+    for (int b = tid; b < max(nparts, ng); b += 2 * stride) {
+        const int b2 = b + stride;
+        const BlockPart& p1 = parts[min(b, nparts - 1)];
+        const BlockPart& p2 = parts[min(b2, nparts - 1)];
+        const double e1 = with_e ? ewave[min(b, ng - 1)] : 0.0;
+        const double e2 = with_e ? ewave[min(b2, ng - 1)] : 0.0;
+        int q1[8], q2[8];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            q1[k] = p1.box0[k];
+            q2[k] = p2.box0[k];
+        }
+        q1[6] = static_cast<int>(p1.accel);
+        q1[7] = static_cast<int>(p1.contacts);
+        q2[6] = static_cast<int>(p2.accel);
+        q2[7] = static_cast<int>(p2.contacts);
+        auto fold = [&](const int q[8]) __attribute__((always_inline)) {
+            am = max(am, static_cast<unsigned>(q[6]));
+            csum += static_cast<unsigned>(q[7]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                bx[k] = min(bx[k], q[k]);
+                bx[3 + k] = max(bx[3 + k], q[3 + k]);
+            }
+        };
+        if (b < nparts) fold(q1);
+        if (b2 < nparts) fold(q2);
+        if (b < ng) e += e1;   // groups in the order tid, tid + stride, ...
+        if (b2 < ng) e += e2;
     }
     e = wave_sum(e);
     const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
