@@ -201,6 +201,23 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
     }
 }
 
+// Thread tid's share of the energy groups, ewave[tid], ewave[tid + stride],
+// ... added in that order, four loads per round issued together (clamped
+// indices, so none is predicated).
+__device__ __forceinline__ double ewave_share(const double* __restrict__ ewave, int ng, int tid, int stride) {
+    double e = 0.0;
+    for (int w = tid; w < ng; w += 4 * stride) {
+        double x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = ewave[min(w + k * stride, ng - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (w + k * stride < ng) e += x[k];
+        }
+    }
+    return e;
+}
+
 // Deterministic block sum of one double per thread (fixed butterfly and wave
 // order); the result is valid in thread 0.
 __device__ double block_sum(double v) {
@@ -1472,8 +1489,7 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
     }
     double e = 0.0;
     if (flags & (kFinEnergyLog | kFinEnergyLast)) {
-        const int ng = (P.n + kEnergyGroup - 1) / kEnergyGroup;
-        for (int w = tid; w < ng; w += blockDim.x) e += ewave[w];
+        e = ewave_share(ewave, (P.n + kEnergyGroup - 1) / kEnergyGroup, tid, static_cast<int>(blockDim.x));
     }
     e = wave_sum(e);
     const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
@@ -1545,8 +1561,8 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
         csum += parts[b].contacts;
     }
     if (flags & (kFinEnergyLog | kFinEnergyLast)) {
-        const int nw = (P.n + kEnergyGroup - 1) / kEnergyGroup;
-        for (int w = threadIdx.x; w < nw; w += blockDim.x) e += ewave[w];
+        e = ewave_share(ewave, (P.n + kEnergyGroup - 1) / kEnergyGroup, static_cast<int>(threadIdx.x),
+                        static_cast<int>(blockDim.x));
     }
     const double total = block_sum(e);
     const double contacts = block_sum(static_cast<double>(csum));  // exact below 2^53
