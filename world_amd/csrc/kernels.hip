@@ -755,19 +755,25 @@ __global__ __launch_bounds__(BS, 5) void k_force(
         int listed = 0;
         auto flush = [&]() __attribute__((always_inline)) {
             uint32_t qn = 0;
-            float4 pjn = make_float4(0.f, 0.f, 0.f, 0.f), vjn = pjn;
+            float4 pjn = make_float4(0.f, 0.f, 0.f, 0.f), vjn = pjn, pjn2 = pjn, vjn2 = pjn;
             if (listed > 0) {
                 qn = s_list[0][threadIdx.x];
                 pjn = Sp[qn];
                 vjn = Sv[qn];
+                qn = s_list[min(1, listed - 1)][threadIdx.x];
+                pjn2 = Sp[qn];
+                vjn2 = Sv[qn];
             }
             for (int k = 0; k < listed; ++k) {
                 const float4 pj = pjn, vj = vjn;
-                // the next contact's loads (the last one's again after it:
-                // unconditional, so that they are in flight during this one)
-                qn = s_list[min(k + 1, listed - 1)][threadIdx.x];
-                pjn = Sp[qn];
-                vjn = Sv[qn];
+                pjn = pjn2;
+                vjn = vjn2;
+                // the loads of the contact after the next (the last one's
+                // again past the end: unconditional, so that two contacts'
+                // loads are in flight during this one)
+                qn = s_list[min(k + 2, listed - 1)][threadIdx.x];
+                pjn2 = Sp[qn];
+                vjn2 = Sv[qn];
                 float d0, d1, d2;
                 const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
                 const int jd = __float_as_int(pj.w);
