@@ -63,10 +63,12 @@ def cic_bytes_per_grain_step(n: int, nodes: int, every: int) -> float:
     return (28.0 + 16.0 * nodes / n) / every
 
 
-def run_gpu(cfg, steps: int, warmup: int, cic_every: int, energy: bool, profile_steps: int):
+def run_gpu(cfg, steps: int, warmup: int, cic_every: int, energy: bool, profile_steps: int, pin=None):
     from world_amd.sim import ParticleDynamics
 
     sim = ParticleDynamics(cfg)
+    if pin:
+        sim.set_force_variant(*pin)
     n = sim.n
     if energy:
         sim.enable_energy_log(steps + warmup + profile_steps + 8)
@@ -125,7 +127,7 @@ def roofline_of(run: dict) -> dict:
     achieved = bpg * n / (ms * 1e-3) if ms > 0 else 0.0
     return {"bound": "hbm", "kernel": "k_force", "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-            "traffic": traffic_of("k_force", n), "bytes_per_grain": bpg,
+            "traffic": traffic_of("k_force", n, run["force_variant"]), "bytes_per_grain": bpg,
             "contacts_per_grain": run["contacts_per_grain"], "avg_launch_ms": ms}
 
 
@@ -282,9 +284,10 @@ def copy_probe(mib: int = 256) -> None:
     torch.cuda.synchronize()
 
 
-def traffic_of(kernel: str, n: int):
-    """HBM bytes per launch of `kernel` at n grains, from the committed
-    per-grain counter summary (profiles/hbm_traffic.json), or None."""
+def traffic_of(kernel: str, n: int, variant: dict):
+    """HBM bytes per launch of `kernel` at n grains with the force-pass
+    variant the run used, from the committed per-grain counter summary
+    (profiles/hbm_traffic.json), or None when it has no such entry."""
     p = ROOT / "profiles" / "hbm_traffic.json"
     if not p.exists():
         return None
@@ -292,8 +295,11 @@ def traffic_of(kernel: str, n: int):
         per = json.loads(p.read_text())["bytes_per_grain"][kernel]
     except (KeyError, ValueError):
         return None
-    best = min(per, key=lambda e: abs(e["grains"] - n))
-    return best["bytes_per_grain"] * n if abs(best["grains"] - n) <= 0.05 * n else None
+    tag = f"{variant['block']}/{variant['lanes_per_grain']}"
+    for e in per:
+        if e.get("variant") == tag and abs(e["grains"] - n) <= 0.05 * n:
+            return e["bytes_per_grain"] * n
+    return None
 
 
 def cpu_baseline(cfg, target_s: float = 10.0) -> dict:
@@ -334,6 +340,11 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 256 MiB device copy (counter calibration)")
+    ap.add_argument("--pin-32k", default="", metavar="BLOCK/LANES",
+                    help="fix the 32,768-grain force-pass variant instead of tuning it online "
+                         "(counter runs: the counters perturb the timings the tuner reads)")
+    ap.add_argument("--pin-1m", default="", metavar="BLOCK/LANES",
+                    help="the same for the 1,048,576-grain column")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -348,7 +359,10 @@ def main() -> int:
         copy_probe()
     cfg = workloads.column_collapse_32k(friction=0.5)
     prof_main = args.warmup + args.steps if args.profile_steps < 0 else args.profile_steps
-    main_run = run_gpu(cfg, args.steps, args.warmup, 40, True, prof_main)
+    def pin_of(text):
+        return tuple(int(v) for v in text.split("/")) if text else None
+
+    main_run = run_gpu(cfg, args.steps, args.warmup, 40, True, prof_main, pin_of(args.pin_32k))
     n = main_run["n"]
     value = n * args.steps / main_run["elapsed"]
     nodes = 64 ** 3
@@ -382,7 +396,7 @@ def main() -> int:
     if args.large_steps > 0:
         big = workloads.column_collapse_1m()
         prof_big = 10 + args.large_steps if args.profile_steps < 0 else args.profile_steps
-        lr = run_gpu(big, args.large_steps, 10, 100, False, prof_big)
+        lr = run_gpu(big, args.large_steps, 10, 100, False, prof_big, pin_of(args.pin_1m))
         lv = lr["n"] * args.large_steps / lr["elapsed"]
         lb = step_bytes(lr, 128 ** 3, 100)
         out["large"] = {
