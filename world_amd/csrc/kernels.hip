@@ -956,9 +956,11 @@ __global__ __launch_bounds__(64) void k_force_q(
     // Spring of `partner` in the old list (index hs, h_old entries), zero for
     // a new contact.  The old list was written in contact order, so slot
     // `guess` (the contact's new slot) normally holds it.
-    auto h_find = [&](int hs, int h_old, int partner, int guess, float xi[3]) __attribute__((always_inline)) {
+    // hg: slot `guess` of the old list, loaded by the caller (any slot below
+    // kHist exists).
+    auto h_find_at = [&](int hs, int h_old, int partner, int guess, const float4& hg,
+                         float xi[3]) __attribute__((always_inline)) {
         xi[0] = xi[1] = xi[2] = 0.f;
-        const float4 hg = h_in[static_cast<size_t>(min(guess, kHist - 1)) * n + hs];  // always in bounds
         if (guess < h_old && __float_as_int(hg.w) == partner) {
             xi[0] = hg.x; xi[1] = hg.y; xi[2] = hg.z;
             return;
@@ -968,6 +970,9 @@ __global__ __launch_bounds__(64) void k_force_q(
             const float4 h = h_in[static_cast<size_t>(t) * n + hs];
             xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
         }
+    };
+    auto h_find = [&](int hs, int h_old, int partner, int guess, float xi[3]) __attribute__((always_inline)) {
+        h_find_at(hs, h_old, partner, guess, h_in[static_cast<size_t>(min(guess, kHist - 1)) * n + hs], xi);
     };
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi4 = pi;
@@ -1115,27 +1120,56 @@ __global__ __launch_bounds__(64) void k_force_q(
         s_q[lane] = q0;
         s_slot[lane] = lane_slot + cons;
         __syncthreads();
+        // One queue entry's indices and loads.  With 2 or 4 lanes per grain
+        // the next round's entries are fetched while the current round is
+        // evaluated; one lane per grain fetches each round's at its start,
+        // keeping the registers (and occupancy) it needs at 1M grains.  An
+        // index past the queue fetches its last entry: every load is
+        // unconditional.
+        struct Entry {
+            int slot, og;
+            uint32_t q;
+            float4 opi, ovi, pj, vj, hg;
+        };
+        auto fetch = [&](int e, Entry& t) __attribute__((always_inline)) {
+            int src = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                if (s_q[src + step] <= e) src += step;
+            }
+            const int j = e - s_q[src];
+            t.q = s_list[j][src];
+            t.slot = s_slot[src] + j;
+            t.og = src / L;
+            const int po = lb * G + t.og;
+            t.opi = Sp[po];
+            t.ovi = Sv[po];
+            t.pj = Sp[t.q];
+            t.vj = Sv[t.q];
+            if (HIST) t.hg = h_in[static_cast<size_t>(min(t.slot, kHist - 1)) * n + s_gh[t.og].x];
+        };
+        constexpr bool kAhead = L > 1;
+        Entry cur, nxt;
+        if (kAhead && T > 0) fetch(min(lane, T - 1), cur);
         for (int base = 0; base < T; base += 64) {
+            if (kAhead) {
+                if (base + 64 < T) fetch(min(base + 64 + lane, T - 1), nxt);
+            } else {
+                fetch(min(base + lane, T - 1), cur);
+            }
             const int e = base + lane;
             float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
             if (e < T) {
-                int src = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    if (s_q[src + step] <= e) src += step;
-                }
-                const int j = e - s_q[src];
-                const uint32_t q = s_list[j][src];
-                const int slot = s_slot[src] + j;
-                const int og = src / L;
+                const int slot = cur.slot;
+                const int og = cur.og;
                 const int po = lb * G + og;
-                const float4 opi = Sp[po], ovi = Sv[po];
+                const float4 opi = cur.opi, ovi = cur.ovi;
                 const float ox[3] = {opi.x, opi.y, opi.z}, ov[3] = {ovi.x, ovi.y, ovi.z};
                 const int oid = __float_as_int(opi.w);
                 const int omat = grain_mat(ovi.w);
                 const int omember = member_of(P, ovi.w);
                 const float omu = P.members == 1 ? P.mu : P.mmu[omember];
-                const float4 pj = Sp[q];
+                const float4 pj = cur.pj;
                 float d0, d1, d2;
                 const float norm = pair_norm(ox, pj.x, pj.y, pj.z, d0, d1, d2);
                 const int jd = __float_as_int(pj.w);
@@ -1147,7 +1181,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                 r0.y = fp * nrm[1];
                 r0.z = fp * nrm[2];
                 ++ncont;
-                const float4 vj = Sv[q];
+                const float4 vj = cur.vj;
                 const float c = lds_tab ? s_cpair[omat * kMaxMat + grain_mat(vj.w)]
                                         : mat_cpair(P, omember, omat, grain_mat(vj.w));
                 const float vr[3] = {ov[0] - vj.x, ov[1] - vj.y, ov[2] - vj.z};
@@ -1161,7 +1195,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                 if (HIST) {
                     const int2 gh = s_gh[og];
                     float xi[3];
-                    h_find(gh.x, gh.y, jd, slot, xi);
+                    h_find_at(gh.x, gh.y, jd, slot, cur.hg, xi);
                     if (ENERGY && jd > oid) {
                         float s2 = 0.f;
                         s2 += xi[0] * xi[0];
@@ -1211,6 +1245,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                 }
             }
             __syncthreads();
+            if (kAhead) cur = nxt;
         }
         cons += cnt;
         if (__ballot(cons < total) == 0ull) break;
