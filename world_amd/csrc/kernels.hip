@@ -314,6 +314,48 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
     }
 }
 
+// store_block_part for a block of one wave: the same partial, reduced with
+// wave operations only (no LDS, so the pass's LDS bounds its occupancy less).
+__device__ void store_wave_part(BlockPart* parts, int slot, int member, const int lo[3], const int hi[3],
+                                unsigned accel, unsigned contacts) {
+    const int lane = threadIdx.x & 63;
+    const int mlo = wave_min(member < 0 ? kEmptyMin : member);
+    const int mhi = wave_max(member);
+    const bool in0 = member >= 0 && member == mlo;
+    const bool in1 = member >= 0 && member == mhi && mhi != mlo;
+    int v[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = wave_min(in0 ? lo[k] : kEmptyMin);
+        v[3 + k] = wave_max(in0 ? hi[k] : kEmptyMax);
+        v[6 + k] = kEmptyMin;
+        v[9 + k] = kEmptyMax;
+    }
+    if (mhi != mlo && mhi >= 0) {  // wave-uniform: a second member only where one starts
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
+            v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+        }
+    }
+    const unsigned am = wave_max_u(accel);
+    const unsigned ct = wave_sum_u(contacts);
+    if (lane == 0) {
+        BlockPart bp;
+        bp.m0 = mhi < 0 ? -1 : mlo;
+        bp.m1 = mhi;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            bp.box0[k] = v[k];
+            bp.box1[k] = v[6 + k];
+        }
+        bp.accel = am;
+        bp.contacts = ct;
+        bp.energy = 0.0;
+        parts[slot] = bp;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1301,7 +1343,7 @@ __global__ __launch_bounds__(64) void k_force_q(
         const double ew = group_energy<L>(live ? static_cast<double>(e_grain) : 0.0);
         if (sub == 0 && (g & (kEnergyGroup - 1)) == 0 && live) ewave[p / kEnergyGroup] = ew;
     }
-    store_block_part(parts, lb, lane_member, lo, hi, accel_bits, 0.0, false, ncont);
+    store_wave_part(parts, lb, lane_member, lo, hi, accel_bits, ncont);
 }
 
 // Largest CIC deposit magnitude of one grain as float bits (0 for a
