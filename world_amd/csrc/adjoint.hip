@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_snap_load(const Params* __restrict__
         lo[1] = hi[1] = cy;
         lo[2] = hi[2] = cz;
     }
-    store_block_part(parts, logical_block(), member, lo, hi, 0u, 0.0, false);
+    store_block_part(parts, P.members > 1 ? P.pmem : nullptr, logical_block(), member, lo, hi, 0u);
 }
 
 // dL/dx, dL/dv of the final state from the cotangent of its CIC grids

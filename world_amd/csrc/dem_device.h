@@ -26,6 +26,8 @@ constexpr int kMatTab = 2 * kMaxMat + kMaxMat * kMaxMat;  // one member's materi
 
 // Device-resident solver constants (one copy per handle, read through a
 // pointer so that dt can change without re-capturing the step graph).
+struct BlockMembers;
+
 struct Params {
     int n;
     int gx, gy, gz;        // collision grid
@@ -58,6 +60,7 @@ struct Params {
     const float* mtab;     // [members][kMatTab] mass, c_wall, c_pair per member (null: the
                            // tables below serve every grain)
     unsigned long long* movf;  // [members] over-capacity cells, summed over passes
+    BlockMembers* pmem;    // member part of the block partials (written when members > 1)
     float mass[kMaxMat];
     float c_wall[kMaxMat];
     float c_pair[kMaxMat * kMaxMat];

@@ -89,6 +89,7 @@ struct wd_sim {
     size_t nodes = 0;                  // CIC nodes per member channel
     int nblocks = 0;
     BlockPart* parts = nullptr;
+    BlockMembers* pmem = nullptr;   // member part of the partials (ensembles only)
     double* energy_log = nullptr;
     double* ewave = nullptr;      // per-kEnergyGroup-grain energy sums of the last energy pass
     int force_var = -1;           // index into kForceVariants; -1 until the autotuner has chosen
@@ -182,7 +183,7 @@ struct wd_sim {
                         Hc[1], key, slot, count,
                         start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
-                        hidx[0], hidx[1], d_mtab, d_movf};
+                        hidx[0], hidx[1], d_mtab, d_movf, pmem};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -192,7 +193,7 @@ struct wd_sim {
         Hc[0] = Hc[1] = nullptr;
         hidx[0] = hidx[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
-        st6 = nullptr; mat_d = nullptr; parts = nullptr; energy_log = nullptr; ewave = nullptr;
+        st6 = nullptr; mat_d = nullptr; parts = nullptr; pmem = nullptr; energy_log = nullptr; ewave = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
         d_mscale = nullptr; d_mtab = nullptr; d_movf = nullptr;
@@ -706,6 +707,7 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->mat_d, n));
     s->nblocks = grain_blocks(n, 16);  // the most partials any pass writes (k_force_q, 4 lanes)
     WD_TRY(dalloc(&s->parts, s->nblocks));
+    if (s->members > 1) WD_TRY(dalloc(&s->pmem, s->nblocks));
     WD_TRY(dalloc(&s->ewave, static_cast<size_t>((n + kEnergyGroup - 1) / kEnergyGroup)));
     s->nodes = static_cast<size_t>(c.density_grid_size_x) * c.density_grid_size_y * c.density_grid_size_z;
     s->cic_count = static_cast<size_t>(4) * s->nodes * static_cast<size_t>(s->members);
@@ -731,6 +733,7 @@ int allocate(wd_sim* s) {
     s->hp.mmu = s->d_mmu;
     s->hp.mscale = s->d_mscale;
     s->hp.movf = s->d_movf;
+    s->hp.pmem = s->pmem;
     s->hp.mtab = s->d_mtab;
     WD_TRY(upload_member_tables(s));
     WD_TRY(dalloc(&s->cic_acc, s->cic_count));  // int64 fixed point, all members

@@ -35,19 +35,22 @@ struct Ctl {
     unsigned long long contact_sum;   // contacts (walls + pairs) summed over those passes
 };
 
-// Per-block partial results of a grain pass, reduced by k_finalize.  A block
-// covers consecutive grains, so it holds a run of members m0..m1; the boxes
-// of its first and last member are reduced here (members strictly between
-// them lie wholly inside the block and are reduced by k_finalize from the
-// cell keys).
+// Per-block partial results of a grain pass, reduced by k_finalize, in two
+// arrays indexed by block.  A block covers consecutive grains, so it holds a
+// run of members m0..m1; the boxes of its first and last member are reduced
+// here (members strictly between them lie wholly inside the block and are
+// reduced by k_finalize from the cell keys).  A lone simulation's pass
+// writes and reads only the first array (32 B per block).
 struct BlockPart {
-    int m0, m1;       // first and last member of the block's grains (-1: none)
     int box0[6];      // min cx, cy, cz / max cx, cy, cz of member m0's grains
-    int box1[6];      // the same for member m1 (when m1 != m0)
     unsigned accel;   // float bits of the block's max |a| (or deposit magnitude)
     unsigned contacts;  // contacts (walls + pairs) of the block's grains
-    double energy;    // unused (energies are summed per kEnergyGroup grains into ewave)
 };
+struct BlockMembers {
+    int m0, m1;       // first and last member of the block's grains (-1: none)
+    int box1[6];      // the box of member m1 (when m1 != m0)
+};
+static_assert(sizeof(BlockPart) == 32 && sizeof(BlockMembers) == 32, "block partials are 32 B");
 
 // What k_finalize publishes.
 enum FinalizeFlags {

@@ -237,10 +237,10 @@ __device__ double block_sum(double v) {
 
 // Reduces, over the block, the boxes of its first and last member (lanes
 // with member < 0 contribute nothing), the max of `accel` and the sum of
-// `energy`, and stores them with plain stores in parts[slot] (thread 0).
-__device__ void store_block_part(BlockPart* parts, int slot, int member, const int lo[3],
-                                 const int hi[3], unsigned accel, double energy, bool with_energy,
-                                 unsigned contacts = 0u) {
+// `contacts`, and stores them with plain stores at parts[slot] and, when pm
+// is not null (an ensemble), pm[slot] (thread 0).
+__device__ void store_block_part(BlockPart* parts, BlockMembers* pm, int slot, int member, const int lo[3],
+                                 const int hi[3], unsigned accel, unsigned contacts = 0u) {
     __shared__ int s_m[16][2];
     __shared__ int s_box[16][14];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -283,8 +283,7 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
         for (int k = 0; k < 13; ++k) s_box[wave][k] = v[k];
         s_box[wave][13] = static_cast<int>(cw);
     }
-    const double e = with_energy ? block_sum(energy) : 0.0;  // contains __syncthreads
-    if (!with_energy) __syncthreads();
+    __syncthreads();
     if (threadIdx.x == 0) {
         unsigned am = static_cast<unsigned>(s_box[0][12]);
         unsigned ct = static_cast<unsigned>(s_box[0][13]);
@@ -300,24 +299,26 @@ __device__ void store_block_part(BlockPart* parts, int slot, int member, const i
             am = max(am, static_cast<unsigned>(s_box[w][12]));
         }
         BlockPart bp;
-        bp.m0 = mhi < 0 ? -1 : mlo;
-        bp.m1 = mhi;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            bp.box0[k] = v[k];
-            bp.box1[k] = v[6 + k];
-        }
+        for (int k = 0; k < 6; ++k) bp.box0[k] = v[k];
         bp.accel = am;
         bp.contacts = ct;
-        bp.energy = e;
         parts[slot] = bp;
+        if (pm) {
+            BlockMembers bm;
+            bm.m0 = mhi < 0 ? -1 : mlo;
+            bm.m1 = mhi;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) bm.box1[k] = v[6 + k];
+            pm[slot] = bm;
+        }
     }
 }
 
 // store_block_part for a block of one wave: the same partial, reduced with
 // wave operations only (no LDS, so the pass's LDS bounds its occupancy less).
-__device__ void store_wave_part(BlockPart* parts, int slot, int member, const int lo[3], const int hi[3],
-                                unsigned accel, unsigned contacts) {
+__device__ void store_wave_part(BlockPart* parts, BlockMembers* pm, int slot, int member, const int lo[3],
+                                const int hi[3], unsigned accel, unsigned contacts) {
     const int lane = threadIdx.x & 63;
     const int mlo = wave_min(member < 0 ? kEmptyMin : member);
     const int mhi = wave_max(member);
@@ -342,17 +343,19 @@ __device__ void store_wave_part(BlockPart* parts, int slot, int member, const in
     const unsigned ct = wave_sum_u(contacts);
     if (lane == 0) {
         BlockPart bp;
-        bp.m0 = mhi < 0 ? -1 : mlo;
-        bp.m1 = mhi;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            bp.box0[k] = v[k];
-            bp.box1[k] = v[6 + k];
-        }
+        for (int k = 0; k < 6; ++k) bp.box0[k] = v[k];
         bp.accel = am;
         bp.contacts = ct;
-        bp.energy = 0.0;
         parts[slot] = bp;
+        if (pm) {
+            BlockMembers bm;
+            bm.m0 = mhi < 0 ? -1 : mlo;
+            bm.m1 = mhi;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) bm.box1[k] = v[6 + k];
+            pm[slot] = bm;
+        }
     }
 }
 
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const Params* __restrict__ Pp
         lo[1] = hi[1] = cy;
         lo[2] = hi[2] = cz;
     }
-    store_block_part(parts, logical_block(), member, lo, hi, 0u, 0.0, false);
+    store_block_part(parts, P.members > 1 ? P.pmem : nullptr, logical_block(), member, lo, hi, 0u);
 }
 
 // Original-order material ids pushed into the working layout.
@@ -940,7 +943,9 @@ __global__ __launch_bounds__(BS, 5) void k_force(
         const double ew = group_energy<1>(static_cast<double>(e_grain));
         if ((threadIdx.x & (kEnergyGroup - 1)) == 0 && p < n) ewave[p / kEnergyGroup] = ew;
     }
-    if (MODE != kModeRHS) store_block_part(parts, lb, lane_member, lo, hi, accel_bits, 0.0, false, ncont);
+    if (MODE != kModeRHS) {
+        store_block_part(parts, P.members > 1 ? P.pmem : nullptr, lb, lane_member, lo, hi, accel_bits, ncont);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,7 +1348,7 @@ __global__ __launch_bounds__(64) void k_force_q(
         const double ew = group_energy<L>(live ? static_cast<double>(e_grain) : 0.0);
         if (sub == 0 && (g & (kEnergyGroup - 1)) == 0 && live) ewave[p / kEnergyGroup] = ew;
     }
-    store_wave_part(parts, lb, lane_member, lo, hi, accel_bits, ncont);
+    store_wave_part(parts, P.members > 1 ? P.pmem : nullptr, lb, lane_member, lo, hi, accel_bits, ncont);
 }
 
 // Largest CIC deposit magnitude of one grain as float bits (0 for a
@@ -1375,8 +1380,9 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
     if (g1 > g0) {
         for (int b = g0 / bs + lane; b <= (g1 - 1) / bs; b += stride) {
             const BlockPart& bp = parts[b];
-            if (bp.m0 == m || bp.m1 == m) {
-                const int* bx = bp.m0 == m ? bp.box0 : bp.box1;
+            const BlockMembers& bm = P.pmem[b];
+            if (bm.m0 == m || bm.m1 == m) {
+                const int* bx = bm.m0 == m ? bp.box0 : bm.box1;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     lo[k] = min(lo[k], bx[k]);
@@ -1528,7 +1534,10 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
         const int b = r * static_cast<int>(blockDim.x) + tid;
         const bool valid = b < nparts;
         const BlockPart* bp = parts + (valid ? b : 0);
-        const int m0 = valid ? bp->m0 : -1, m1 = valid ? bp->m1 : -1;
+        // A lone simulation writes no member part: every partial is member 0's
+        // (an empty block's box is the identity, which merges as a no-op).
+        const BlockMembers* bm = P.members > 1 ? P.pmem + (valid ? b : 0) : nullptr;
+        const int m0 = valid ? (bm ? bm->m0 : 0) : -1, m1 = valid ? (bm ? bm->m1 : 0) : -1;
         if (valid) {
             am = max(am, bp->accel);
             csum += bp->contacts;
@@ -1551,7 +1560,7 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
         }
         if (m0 < 0) continue;
         merge(m0, bp->box0);
-        if (m1 > m0) merge(m1, bp->box1);
+        if (m1 > m0) merge(m1, bm->box1);
         for (int mm = m0 + 1; mm < m1; ++mm) {  // members wholly inside this block
             int g[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
             for (int i = P.mgoff[mm]; i < P.mgoff[mm + 1]; ++i) {
@@ -1907,7 +1916,7 @@ __global__ __launch_bounds__(kBlock) void k_cic_scale(const Params* __restrict__
         member = member_of(P, v.w);
         hi[0] = static_cast<int>(cic_grain_bits(P, Tp[i], v));
     }
-    store_block_part(parts, blockIdx.x, member, lo, hi, 0u, 0.0, false);
+    store_block_part(parts, P.members > 1 ? P.pmem : nullptr, blockIdx.x, member, lo, hi, 0u);
 }
 
 // Fixed-point exponent S for n grains of largest magnitude M: n*M*2^S < 2^62.
