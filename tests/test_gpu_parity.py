@@ -349,3 +349,32 @@ def test_retuning_keeps_results():
         assert tuned.force_variant() in FORCE_VARIANTS
         np.testing.assert_array_equal(tuned.state(), fixed.state())
         assert_springs_equal(tuned.history(), fixed.history())
+
+
+def test_wide_box_scan_matches_oracle(oracle_lib):
+    """2,048 grains scattered over a 4 x 4 m box: the occupied box spans
+    ~12 M cells, so the cell scan has more tiles (4,096 cells each) than its
+    grid has blocks (256), and blocks take further tiles from its counter.
+    Cell order and five steps equal the oracle's."""
+    c = workloads.column(16, 16, 8, half_width=2.0, ceiling=1.0)
+    st, mat = _capi.initial_state(c)
+    st = st.reshape(-1, 6).copy()
+    rng = np.random.default_rng(5)
+    st[:, 0:2] = rng.uniform(-1.9, 1.9, (len(st), 2))
+    st[:, 2] = rng.uniform(0.05, 0.9, len(st))
+    st[:, 3:] = rng.normal(0, 0.3, (len(st), 3))
+    st = st.astype(np.float32)
+    d = (c.x_max - c.x_min) / c.collision_grid_size_x
+    span = [int(np.floor((st[:, a].max() - st[:, a].min()) / d)) + 3 for a in range(3)]
+    assert span[0] * span[1] * span[2] > 256 * 4096
+    with ParticleDynamics(c) as sim:
+        sim.set_state(st)
+        ids, keys = sim.cell_order()
+        sim.take_step(5)
+        got = sim.state()
+    oid, okeys, _ = oracle_lib.cell_order(c, st)
+    np.testing.assert_array_equal(ids, oid)
+    np.testing.assert_array_equal(keys.astype(np.int64), okeys)
+    want, over, _ = oracle_lib.steps(c, st, mat, 5, threads=8)
+    assert over == 0
+    np.testing.assert_array_equal(got, want)
