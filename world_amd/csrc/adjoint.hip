@@ -312,12 +312,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ 
     lv[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
 }
 
+// Grains per block of the per-grain adjoint passes: one wave, so that a few
+// thousand grains spread over as many CUs as there are waves (the passes use
+// no block-wide step).
+constexpr int kAdjBlock = 64;
+
 // Pass A of the adjoint of one SIE step (see the file comment).  Grains in
 // cell order (Sp/Sv, start: the bin of state s); h_in = the springs the
 // forward step read; lxi_next[k][id] = dL/d(spring slot k written by the
 // step); lxi_prev[t][id] receives dL/d(spring slot t read by the step).  The
 // cotangent lists are indexed by original id.
-__global__ __launch_bounds__(kBlock) void k_adj_contacts(
+__global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const uint32_t* __restrict__ start,
         const int* __restrict__ hidx, const float4* __restrict__ h_in,
@@ -328,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
         float4* __restrict__ cg_d, float4* __restrict__ cg_v, int* __restrict__ cg_cnt,
         int* __restrict__ cg_p, double2* __restrict__ gparam, int n) {
     const Params& P = *Pp;
-    __shared__ uint32_t s_list[kContactList][kBlock];
+    __shared__ uint32_t s_list[kContactList][kAdjBlock];
     const int p = logical_block() * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const float4 pi = Sp[p];
@@ -525,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_contacts(
 // evaluated, in j's contact order, written at j's position in the previous
 // step's order (hidx[j]).  Partner lists are grain-major
 // (cg_p[i * kAdjContacts + t]) so that a search reads four partners per load.
-__global__ __launch_bounds__(kBlock) void k_adj_gather(const float4* __restrict__ own_x,
+__global__ __launch_bounds__(kAdjBlock) void k_adj_gather(const float4* __restrict__ own_x,
                                                        const float4* __restrict__ own_v,
                                                        const float4* __restrict__ cg_d,
                                                        const float4* __restrict__ cg_v,

@@ -1427,7 +1427,7 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
         par ^= 1;
     }
     const int blocks = static_cast<int>((s->n + kBlock - 1) / kBlock);
-    const int gb = grain_blocks(s->n);
+    const int agb = grain_blocks(s->n, kAdjBlock);
     // The cotangents arrive in original order (the checkpoint's); the last
     // step's contact adjoint reads them in that step's cell order.
     hipLaunchKernelGGL(k_adj_to_cells, dim3(blocks), dim3(kBlock), 0, s->stream,
@@ -1437,7 +1437,7 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
     for (int k = steps - 1; k >= 0; --k) {
         const wd_sim::BinSlot& b = A->slot[k];
         const bool last = k == steps - 1;
-        hipLaunchKernelGGL(k_adj_contacts, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl,
+        hipLaunchKernelGGL(k_adj_contacts, dim3(agb), dim3(kAdjBlock), 0, s->stream, s->dp, s->ctl,
                            (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
                            (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
                            (const int*)b.mbox, (const uint32_t*)b.mcell,
@@ -1445,7 +1445,7 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
                            (const float4*)A->lxi[cur], A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d,
                            A->cg_v, A->cg_cnt, A->cg_p, A->gparam, s->n);
         // dL/d(state before step k), at the previous step's positions
-        hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(kBlock), 0, s->stream, A->own_x,
+        hipLaunchKernelGGL(k_adj_gather, dim3((s->n + kAdjBlock - 1) / kAdjBlock), dim3(kAdjBlock), 0, s->stream, A->own_x,
                            A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, (const int*)b.hidx, A->lx,
                            A->lv, s->n);
         cur ^= 1;
