@@ -259,13 +259,16 @@ struct wd_sim {
     // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
     void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
                      float4* nout_v) {
-        const int gb = grain_blocks(n);
+        // Up to 256K grains the per-grain bin passes run one-wave blocks, so
+        // that they spread over every CU; beyond, 256-thread blocks.
+        const int bbs = n <= (1 << 18) ? 64 : kBlock;
+        const int gb = grain_blocks(n, bbs);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
-        go("k_count", k_count, dim3(gb), dim3(kBlock), (const Params*)dp, ctl, (const float4*)Tv, key,
+        go("k_count", k_count, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tv, key,
            slot, count, n, bin_box_out, bin_cell_out);
         go("k_scan", k_scan, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
            status);
-        go("k_scatter", k_scatter, dim3(gb), dim3(kBlock), (const Params*)dp, (const uint32_t*)key,
+        go("k_scatter", k_scatter, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
            (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
         // Picard iterations after the first carry the spring index of the
         // step's first bin through every re-sort (nmode 2).
@@ -273,7 +276,7 @@ struct wd_sim {
         const Carry carry{nin_p, nin_v, nout_p, nout_v, hidx[hidx_cur], history() ? hidx[hout] : nullptr};
         hidx_cur = hout;
         auto reorder = [&](auto kern) {
-            go("k_reorder", kern, dim3(gb), dim3(kBlock), (const Params*)dp, (const uint32_t*)key,
+            go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
                (const uint32_t*)start, (const int*)tmp_id, (const float4*)Tp, (const float4*)Tv, Sp, Sv,
                carry, n);
         };
