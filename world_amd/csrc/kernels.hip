@@ -1167,16 +1167,15 @@ __global__ __launch_bounds__(64) void k_force_q(
         s_q[lane] = q0;
         s_slot[lane] = lane_slot + cons;
         __syncthreads();
-        // One queue entry's indices and loads.  With 2 or 4 lanes per grain
-        // the next round's entries are fetched while the current round is
-        // evaluated; one lane per grain fetches each round's at its start,
-        // keeping the registers (and occupancy) it needs at 1M grains.  An
+        // One queue entry's indices and loads; the owner grain's own state
+        // comes from its lanes' registers by shuffles.  The next round's
+        // entries are fetched while the current round is evaluated.  An
         // index past the queue fetches its last entry: every load is
         // unconditional.
         struct Entry {
             int slot, og;
             uint32_t q;
-            float4 opi, ovi, pj, vj, hg;
+            float4 pj, vj, hg;
         };
         auto fetch = [&](int e, Entry& t) __attribute__((always_inline)) {
             int src = 0;
@@ -1188,14 +1187,11 @@ __global__ __launch_bounds__(64) void k_force_q(
             t.q = s_list[j][src];
             t.slot = s_slot[src] + j;
             t.og = src / L;
-            const int po = lb * G + t.og;
-            t.opi = Sp[po];
-            t.ovi = Sv[po];
             t.pj = Sp[t.q];
             t.vj = Sv[t.q];
             if (HIST) t.hg = h_in[static_cast<size_t>(min(t.slot, kHist - 1)) * n + s_gh[t.og].x];
         };
-        constexpr bool kAhead = L > 1;
+        constexpr bool kAhead = true;
         Entry cur, nxt;
         if (kAhead && T > 0) fetch(min(lane, T - 1), cur);
         for (int base = 0; base < T; base += 64) {
@@ -1205,16 +1201,19 @@ __global__ __launch_bounds__(64) void k_force_q(
                 fetch(min(base + lane, T - 1), cur);
             }
             const int e = base + lane;
+            // the owner's state, from a lane of its grain (every lane takes part)
+            const int ol = cur.og * L;
+            const float ox[3] = {__shfl(x[0], ol, 64), __shfl(x[1], ol, 64), __shfl(x[2], ol, 64)};
+            const float ov[3] = {__shfl(v[0], ol, 64), __shfl(v[1], ol, 64), __shfl(v[2], ol, 64)};
+            const int oid = __shfl(id, ol, 64);
+            const float ovw = __shfl(vi4.w, ol, 64);
             float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
             if (e < T) {
                 const int slot = cur.slot;
                 const int og = cur.og;
                 const int po = lb * G + og;
-                const float4 opi = cur.opi, ovi = cur.ovi;
-                const float ox[3] = {opi.x, opi.y, opi.z}, ov[3] = {ovi.x, ovi.y, ovi.z};
-                const int oid = __float_as_int(opi.w);
-                const int omat = grain_mat(ovi.w);
-                const int omember = member_of(P, ovi.w);
+                const int omat = grain_mat(ovw);
+                const int omember = member_of(P, ovw);
                 const float omu = P.members == 1 ? P.mu : P.mmu[omember];
                 const float4 pj = cur.pj;
                 float d0, d1, d2;
