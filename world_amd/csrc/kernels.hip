@@ -67,6 +67,12 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 
 // True when a column's cell-start quad (starts of cells dz = -1, 0, +1 and of
 // the next cell) has a cell holding more than kcap grains.
+// Set bits of a wave mask below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+    return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+}
+
 __device__ __forceinline__ bool column_capped(const uint4& q4, uint32_t kcap) {
     return q4.y - q4.x > kcap || q4.z - q4.y > kcap || q4.w - q4.z > kcap;
 }
@@ -980,7 +986,8 @@ __global__ __launch_bounds__(64) void k_force_q(
     constexpr int QCAP = kQueueCap;
     const Params& P = *Pp;
     __shared__ uint32_t s_list[QCAP][64];       // per lane: cell-sorted slots of listed contacts
-    __shared__ int s_q[64];                     // per lane: queue index of its first listed contact
+    __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
+    __shared__ uint8_t s_src[L == 1 ? QCAP * 64 : 1];  // one lane per grain: owner lane of each queue entry
     __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
     __shared__ float4 s_rec[3][64];             // per queue entry: force addends, energy addends
     __shared__ int2 s_gh[G];                    // per grain: old spring list (index, count)
@@ -1153,18 +1160,42 @@ __global__ __launch_bounds__(64) void k_force_q(
         const unsigned long long drained = __ballot(cons + listed == total);
         const unsigned long long before = ((1ull << sub) - 1ull) << (g * L);
         const int cnt = (drained & before) == before ? listed : 0;
-        int incl = cnt;
+        int T = 0, gq0 = 0, gq1 = 0;
+        uint32_t* const s_flat = &s_list[0][0];
+        if constexpr (L == 1) {
+            // Column-major queue: every grain's first listed contact, then
+            // every grain's second, and so on.  A grain's contacts keep their
+            // order, and a round of 64 holds few of any one grain's, so the
+            // per-grain sums after a round are short.  The list is compacted
+            // in place: row j's entries land below index 64 (j + 1), where
+            // no unread row lies.  An entry is the partner slot (below 2^28)
+            // with its row in the top four bits.
+            for (int j = 0; j < QCAP; ++j) {
+                const unsigned long long mask = __ballot(j < cnt);
+                if (mask == 0ull) break;
+                const uint32_t q = s_list[j][lane];
+                __syncthreads();  // the row is read before any of it is overwritten
+                if (j < cnt) {
+                    const int pos = T + lanes_below(mask);
+                    s_flat[pos] = q | (static_cast<uint32_t>(j) << 28);
+                    s_src[pos] = static_cast<uint8_t>(lane);
+                }
+                T += __popcll(mask);
+            }
+        } else {
+            int incl = cnt;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t;
+            }
+            const int q0 = incl - cnt;
+            T = __shfl(incl, 63, 64);
+            gq0 = __shfl(q0, g * L, 64);
+            const int gq1n = __shfl(q0, ((g + 1) * L) & 63, 64);
+            gq1 = (g + 1) * L < 64 ? gq1n : T;
+            s_q[lane] = q0;
         }
-        const int q0 = incl - cnt;
-        const int T = __shfl(incl, 63, 64);
-        const int gq0 = __shfl(q0, g * L, 64);
-        const int gq1n = __shfl(q0, ((g + 1) * L) & 63, 64);
-        const int gq1 = (g + 1) * L < 64 ? gq1n : T;
-        s_q[lane] = q0;
         s_slot[lane] = lane_slot + cons;
         __syncthreads();
         // One queue entry's indices and loads; the owner grain's own state
@@ -1178,21 +1209,30 @@ __global__ __launch_bounds__(64) void k_force_q(
             float4 pj, vj, hg;
         };
         auto fetch = [&](int e, Entry& t) __attribute__((always_inline)) {
-            int src = 0;
+            if constexpr (L == 1) {
+                const uint32_t w = s_flat[e];
+                const int src = s_src[e];
+                t.q = w & 0x0FFFFFFFu;
+                t.slot = s_slot[src] + static_cast<int>(w >> 28);
+                t.og = src;
+            } else {
+                int src = 0;
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                if (s_q[src + step] <= e) src += step;
+                for (int step = 32; step > 0; step >>= 1) {
+                    if (s_q[src + step] <= e) src += step;
+                }
+                const int j = e - s_q[src];
+                t.q = s_list[j][src];
+                t.slot = s_slot[src] + j;
+                t.og = src / L;
             }
-            const int j = e - s_q[src];
-            t.q = s_list[j][src];
-            t.slot = s_slot[src] + j;
-            t.og = src / L;
             t.pj = Sp[t.q];
             t.vj = Sv[t.q];
             if (HIST) t.hg = h_in[static_cast<size_t>(min(t.slot, kHist - 1)) * n + s_gh[t.og].x];
         };
         constexpr bool kAhead = true;
         Entry cur, nxt;
+        int jc = 0, cjc = 0;  // one lane per grain: first row in the round, its first queue index
         if (kAhead && T > 0) fetch(min(lane, T - 1), cur);
         for (int base = 0; base < T; base += 64) {
             if (kAhead) {
@@ -1273,9 +1313,8 @@ __global__ __launch_bounds__(64) void k_force_q(
             s_rec[1][lane] = r1;
             s_rec[2][lane] = r2;
             __syncthreads();
-            const int r_lo = max(gq0, base), r_hi = min(gq1, base + 64);
-            for (int r = r_lo; r < r_hi; ++r) {
-                const float4 a0 = s_rec[0][r - base], a1 = s_rec[1][r - base], a2 = s_rec[2][r - base];
+            auto add = [&](int r) __attribute__((always_inline)) {
+                const float4 a0 = s_rec[0][r], a1 = s_rec[1][r], a2 = s_rec[2][r];
                 f[0] += a0.x;
                 f[1] += a0.y;
                 f[2] += a0.z;
@@ -1289,6 +1328,25 @@ __global__ __launch_bounds__(64) void k_force_q(
                     e_grain += a0.w;
                     e_grain += a1.w;
                 }
+            };
+            if constexpr (L == 1) {
+                // Rows jc, jc + 1, ... overlap the round; a grain adds its
+                // entry of each, in row order.
+                for (int j = jc, cj = cjc; j < QCAP && cj < base + 64; ++j) {
+                    const unsigned long long mask = __ballot(j < cnt);
+                    if (mask == 0ull) break;
+                    const int r = cj + lanes_below(mask) - base;
+                    if (j < cnt && r >= 0 && r < 64) add(r);
+                    const int cn = cj + __popcll(mask);
+                    if (cn <= base + 64) {
+                        jc = j + 1;
+                        cjc = cn;
+                    }
+                    cj = cn;
+                }
+            } else {
+                const int r_lo = max(gq0, base), r_hi = min(gq1, base + 64);
+                for (int r = r_lo; r < r_hi; ++r) add(r - base);
             }
             __syncthreads();
             if (kAhead) cur = nxt;
