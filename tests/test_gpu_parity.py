@@ -318,6 +318,33 @@ def test_dense_cluster_matches_oracle_for_every_variant(oracle_lib, kcap, model)
             assert sim.compute_total_energy() == pytest.approx(want_e, rel=1e-6), v
 
 
+@pytest.mark.parametrize("model", ["history", "viscous"])
+def test_nan_grain_spreads_as_in_oracle_for_every_variant(oracle_lib, model):
+    """A diverged state: grain 37's height is NaN.  Its cell on that axis is
+    0 (the conversion saturates NaN to 0), which here lies within the stencil
+    of the bottom layer, so its neighbours meet a NaN candidate (squared
+    distance NaN): their forces turn NaN, and the NaN spreads on the next
+    step.  Every variant gives the oracle's states, NaN for NaN."""
+    c = workloads.column(4, 4, 5, friction=0.5)
+    c.z_min = -0.005  # bottom layer in cell 1, next to the NaN grain's cell 0
+    if model == "viscous":
+        c = viscous_cfg(c)
+    st, mat = jittered(c, seed=5)
+    st[37, 2] = np.nan
+    ora = oracle_lib.Sim(c, st, mat)
+    ora.steps(2, threads=1)
+    want = ora.state()
+    ora.close()
+    assert 5 < np.count_nonzero(np.isnan(want).any(axis=1)) < len(want)
+    for v in FORCE_VARIANTS:
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.set_force_variant(*v)
+            sim.take_step(2)
+            np.testing.assert_array_equal(sim.state(), want, err_msg=str(v))
+            assert np.isnan(sim.compute_total_energy()), v
+
+
 def test_contact_count_matches_oracle(oracle_lib):
     """The force pass's contact counter (walls and pairs, each side) equals the
     contacts the oracle's FP32 forward takes, step by step."""

@@ -65,14 +65,14 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 }
 
 
-// True when a column's cell-start quad (starts of cells dz = -1, 0, +1 and of
-// the next cell) has a cell holding more than kcap grains.
 // Set bits of a wave mask below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long mask) {
     return static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
 }
 
+// True when a column's cell-start quad (starts of cells dz = -1, 0, +1 and of
+// the next cell) has a cell holding more than kcap grains.
 __device__ __forceinline__ bool column_capped(const uint4& q4, uint32_t kcap) {
     return q4.y - q4.x > kcap || q4.z - q4.y > kcap || q4.w - q4.z > kcap;
 }
@@ -119,8 +119,10 @@ __device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int 
 // and position of those at positions [s_lo, s_hi) of that sequence.  A run
 // (one column, or with GROUPED the three columns of one dx) is walked with
 // BATCH position loads in flight, while the next run's cell starts load.
-// column(c) returns column c's cell-start quad.
-template <int BATCH, bool GROUPED, class Column, class Probe>
+// column(c) returns column c's cell-start quad.  With FLAT, probe is called
+// for every slot of a batch, as probe(q, x, y, z, valid), and must ignore the
+// slots past the run (valid false) itself: it can then run without branches.
+template <int BATCH, bool GROUPED, bool FLAT = false, class Column, class Probe>
 __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint32_t kcap, uint32_t s_lo,
                                              uint32_t s_hi, Column column, Probe probe) {
     uint32_t off = 0;  // sequence position of the run's first candidate
@@ -141,7 +143,8 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             }
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
-                if (i + k < i1) probe(b0 + i + k, px[k], py[k], pz[k]);
+                if constexpr (FLAT) probe(b0 + i + k, px[k], py[k], pz[k], i + k < i1);
+                else if (i + k < i1) probe(b0 + i + k, px[k], py[k], pz[k]);
             }
         }
         off += len;
@@ -169,7 +172,8 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             }
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
-                if (i + k < i1) probe(q[k], px[k], py[k], pz[k]);
+                if constexpr (FLAT) probe(q[k], px[k], py[k], pz[k], i + k < i1);
+                else if (i + k < i1) probe(q[k], px[k], py[k], pz[k]);
             }
         }
         off += tot;
@@ -203,6 +207,39 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             run3(a.x, min(a.y - a.x, kcap), a.y, min(a.z - a.y, kcap), a.z, min(a.w - a.z, kcap));
             run3(b.x, min(b.y - b.x, kcap), b.y, min(b.z - b.y, kcap), b.z, min(b.w - b.z, kcap));
             run3(c.x, min(c.y - c.x, kcap), c.y, min(c.z - c.y, kcap), c.z, min(c.w - c.z, kcap));
+        }
+    }
+}
+
+// A lane's slice [s_lo, s_hi) of a grain's candidate sequence when no cell
+// of its stencil is over capacity: column c is the run cb[c] + [0, len), at
+// sequence positions cpre[c] + [0, len).  The whole slice is loaded B
+// positions at a time, across column boundaries, so a slice of up to B
+// candidates costs one load latency however many columns it spans.  probe
+// is called in sequence order, as by walk_stencil.
+template <int B, class Probe>
+__device__ __forceinline__ void walk_slice(const float4* __restrict__ Sp, const uint32_t cb[9],
+                                           const uint32_t cpre[9], uint32_t s_lo, uint32_t s_hi, Probe probe) {
+    for (uint32_t t0 = s_lo; t0 < s_hi; t0 += B) {
+        uint32_t q[B];
+        float px[B], py[B], pz[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const uint32_t t = min(t0 + k, s_hi - 1);  // past the slice: reload its last candidate
+            uint32_t s = cb[0] + t;
+#pragma unroll
+            for (int c = 1; c < 9; ++c) {
+                if (t >= cpre[c]) s = cb[c] + (t - cpre[c]);
+            }
+            q[k] = s;
+            const float* r = slot_pos(Sp, s);
+            px[k] = r[0];
+            py[k] = r[1];
+            pz[k] = r[2];
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            if (t0 + k < s_hi) probe(q[k], px[k], py[k], pz[k]);
         }
     }
 }
@@ -985,7 +1022,9 @@ __global__ __launch_bounds__(64) void k_force_q(
     constexpr int G = 64 / L;  // grains per wave
     constexpr int QCAP = kQueueCap;
     const Params& P = *Pp;
-    __shared__ uint32_t s_list[QCAP][64];       // per lane: cell-sorted slots of listed contacts
+    // Per lane: cell-sorted slots of listed contacts; with one lane per
+    // grain, row QCAP takes the stores of candidates that are not listed.
+    __shared__ uint32_t s_list[L == 1 ? QCAP + 1 : QCAP][64];
     __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
     __shared__ uint8_t s_src[L == 1 ? QCAP * 64 : 1];  // one lane per grain: owner lane of each queue entry
     __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
@@ -1111,25 +1150,34 @@ __global__ __launch_bounds__(64) void k_force_q(
         __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
         return q4;
     };
-    // This lane's slice [s_lo, s_hi) of the grain's candidate sequence.
+    // This lane's slice [s_lo, s_hi) of the grain's candidate sequence (set
+    // by the first walk when L > 1).
     uint32_t s_lo = 0u, s_hi = 0xFFFFFFFFu;
-    if (L > 1 && live) {
-        uint32_t total = 0;
-        for (int c = 0; c < 9; ++c) total += column_length(column(c), kcap);
-        s_lo = total * static_cast<uint32_t>(sub) / L;
-        s_hi = total * static_cast<uint32_t>(sub + 1) / L;
-    }
     int cons = 0;     // this lane's contacts already queued
     int seen = 0;     // touching candidates met by the current scan
     bool nan_f = false, nan_e = false;
-    auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
+    const float touch2 = P.touch2;
+    // One lane per grain: every slot of a batch is probed without branches
+    // (an unlisted candidate's store goes to the spare row); with more lanes
+    // per grain, only the slots in the run, with branches.
+    auto probe = [&](uint32_t q, float px, float py, float pz, bool valid = true) __attribute__((always_inline)) {
         const float d0 = x[0] - px, d1 = x[1] - py, d2 = x[2] - pz;
         float s2 = 0.f;
         s2 += d0 * d0;
         s2 += d1 * d1;
         s2 += d2 * d2;
-        if (!(s2 >= P.touch2) && q != static_cast<uint32_t>(p)) {
-            if (s2 < P.touch2) {
+        if constexpr (L == 1) {
+            const bool other = valid & (q != static_cast<uint32_t>(p));
+            const bool hit = other & (s2 < touch2);
+            const uint32_t li = static_cast<uint32_t>(seen - cons);
+            s_list[hit && li < static_cast<uint32_t>(QCAP) ? li : static_cast<uint32_t>(QCAP)][lane] = q;
+            seen += hit ? 1 : 0;
+            if (other & (s2 != s2)) {
+                nan_f = true;
+                if (__float_as_int(Sp[q].w) > id) nan_e = true;
+            }
+        } else if (!(s2 >= touch2) && q != static_cast<uint32_t>(p)) {
+            if (s2 < touch2) {
                 if (seen >= cons && seen < cons + QCAP) s_list[seen - cons][lane] = q;
                 ++seen;
             } else {
@@ -1144,7 +1192,30 @@ __global__ __launch_bounds__(64) void k_force_q(
     int grain_slots = 0;
     for (int pass = 0;; ++pass) {
         seen = 0;
-        if (live && (pass == 0 || cons < total)) walk_stencil<(L == 1 ? 4 : 8), (L != 1)>(Sp, kcap, s_lo, s_hi, column, probe);
+        if (live && (pass == 0 || cons < total)) {
+            if constexpr (L == 1) {
+                walk_stencil<4, false, true>(Sp, kcap, s_lo, s_hi, column, probe);
+            } else {
+                // The grain's nine columns: its sequence length (for the
+                // slices) and, when no cell is over capacity, the runs.
+                uint32_t cb[9], cpre[9], tot = 0;
+                bool capped = false;
+#pragma unroll
+                for (int c = 0; c < 9; ++c) {
+                    const uint4 a = column(c);
+                    capped |= column_capped(a, kcap);
+                    cb[c] = a.x;
+                    cpre[c] = tot;
+                    tot += column_length(a, kcap);
+                }
+                if (pass == 0) {
+                    s_lo = tot * static_cast<uint32_t>(sub) / L;
+                    s_hi = tot * static_cast<uint32_t>(sub + 1) / L;
+                }
+                if (capped) walk_stencil<8, true>(Sp, kcap, s_lo, s_hi, column, probe);
+                else walk_slice<(L == 4 ? 12 : 8)>(Sp, cb, cpre, s_lo, s_hi, probe);
+            }
+        }
         if (pass == 0) {
             total = seen;
             int b = wc;
