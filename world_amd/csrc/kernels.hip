@@ -119,12 +119,14 @@ __device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int 
 // and position of those at positions [s_lo, s_hi) of that sequence.  A run
 // (one column, or with GROUPED the three columns of one dx) is walked with
 // BATCH position loads in flight, while the next run's cell starts load.
-// column(c) returns column c's cell-start quad.  With FLAT, probe is called
+// column(c) returns column c's cell-start quad.  GROUPED walks only the dx
+// groups [gx_lo, gx_hi) (columns 3 gx_lo .. 3 gx_hi - 1).  With FLAT, probe is called
 // for every slot of a batch, as probe(q, x, y, z, valid), and must ignore the
 // slots past the run (valid false) itself: it can then run without branches.
 template <int BATCH, bool GROUPED, bool FLAT = false, class Column, class Probe>
 __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint32_t kcap, uint32_t s_lo,
-                                             uint32_t s_hi, Column column, Probe probe) {
+                                             uint32_t s_hi, Column column, Probe probe, int gx_lo = 0,
+                                             int gx_hi = 3) {
     uint32_t off = 0;  // sequence position of the run's first candidate
     // Candidates b0 + [0, len): a column with no cell over capacity.
     auto run1 = [&](uint32_t b0, uint32_t len) __attribute__((always_inline)) {
@@ -192,11 +194,11 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
         }
         return;
     }
-    uint4 n0 = column(0), n1 = column(1), n2 = column(2);
+    uint4 n0 = column(3 * gx_lo), n1 = column(3 * gx_lo + 1), n2 = column(3 * gx_lo + 2);
 #pragma unroll 1
-    for (int gx = 0; gx < 3; ++gx) {
+    for (int gx = gx_lo; gx < gx_hi; ++gx) {
         const uint4 a = n0, b = n1, c = n2;
-        if (gx < 2) {
+        if (gx + 1 < gx_hi) {
             n0 = column(3 * gx + 3);
             n1 = column(3 * gx + 4);
             n2 = column(3 * gx + 5);
@@ -1195,6 +1197,11 @@ __global__ __launch_bounds__(64) void k_force_q(
         if (live && (pass == 0 || cons < total)) {
             if constexpr (L == 1) {
                 walk_stencil<4, false, true>(Sp, kcap, s_lo, s_hi, column, probe);
+            } else if constexpr (L == 4) {
+                // Lanes 0, 1, 2 of the grain walk dx groups 0, 1, 2 (a
+                // contiguous part of the sequence each, in lane order);
+                // lane 3 lists nothing.
+                if (sub < 3) walk_stencil<12, true>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe, sub, sub + 1);
             } else {
                 // The grain's nine columns: its sequence length (for the
                 // slices) and, when no cell is over capacity, the runs.
@@ -1213,7 +1220,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                     s_hi = tot * static_cast<uint32_t>(sub + 1) / L;
                 }
                 if (capped) walk_stencil<8, true>(Sp, kcap, s_lo, s_hi, column, probe);
-                else walk_slice<(L == 4 ? 12 : 8)>(Sp, cb, cpre, s_lo, s_hi, probe);
+                else walk_slice<8>(Sp, cb, cpre, s_lo, s_hi, probe);
             }
         }
         if (pass == 0) {
