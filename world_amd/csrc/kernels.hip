@@ -64,6 +64,39 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
     return v;
 }
 
+// Wave-wide reduction of an associative, exact integer operation through DPP
+// row moves (a few cycles each, against an LDS round trip per step for
+// cross-lane shuffles): pairs, quads, half rows and rows, then rows 0-1 into
+// row 1 and rows 0-3 into row 3.  Lanes a move does not reach take
+// `identity`.  Every lane must be active; the result is lane 63's, the same
+// for every lane.
+template <class Op>
+__device__ __forceinline__ int wave_reduce_dpp(int v, int identity, Op op) {
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_dpp(int v) {
+    return wave_reduce_dpp(v, kEmptyMin, [](int a, int b) { return min(a, b); });
+}
+__device__ __forceinline__ int wave_max_dpp(int v) {
+    return wave_reduce_dpp(v, kEmptyMax, [](int a, int b) { return max(a, b); });
+}
+__device__ __forceinline__ unsigned wave_max_u_dpp(unsigned v) {
+    return static_cast<unsigned>(wave_reduce_dpp(static_cast<int>(v), 0, [](int a, int b) {
+        return static_cast<int>(max(static_cast<unsigned>(a), static_cast<unsigned>(b)));
+    }));
+}
+__device__ __forceinline__ unsigned wave_sum_u_dpp(unsigned v) {
+    return static_cast<unsigned>(wave_reduce_dpp(static_cast<int>(v), 0, [](int a, int b) {
+        return static_cast<int>(static_cast<unsigned>(a) + static_cast<unsigned>(b));
+    }));
+}
+
 
 // Set bits of a wave mask below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long mask) {
@@ -365,27 +398,28 @@ __device__ void store_block_part(BlockPart* parts, BlockMembers* pm, int slot, i
 __device__ void store_wave_part(BlockPart* parts, BlockMembers* pm, int slot, int member, const int lo[3],
                                 const int hi[3], unsigned accel, unsigned contacts) {
     const int lane = threadIdx.x & 63;
-    const int mlo = wave_min(member < 0 ? kEmptyMin : member);
-    const int mhi = wave_max(member);
+    // A lone simulation (no member partials) has member 0 or none.
+    const int mlo = pm ? wave_min_dpp(member < 0 ? kEmptyMin : member) : 0;
+    const int mhi = pm ? wave_max_dpp(member) : 0;
     const bool in0 = member >= 0 && member == mlo;
     const bool in1 = member >= 0 && member == mhi && mhi != mlo;
     int v[12];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v[k] = wave_min(in0 ? lo[k] : kEmptyMin);
-        v[3 + k] = wave_max(in0 ? hi[k] : kEmptyMax);
+        v[k] = wave_min_dpp(in0 ? lo[k] : kEmptyMin);
+        v[3 + k] = wave_max_dpp(in0 ? hi[k] : kEmptyMax);
         v[6 + k] = kEmptyMin;
         v[9 + k] = kEmptyMax;
     }
     if (mhi != mlo && mhi >= 0) {  // wave-uniform: a second member only where one starts
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
-            v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+            v[6 + k] = wave_min_dpp(in1 ? lo[k] : kEmptyMin);
+            v[9 + k] = wave_max_dpp(in1 ? hi[k] : kEmptyMax);
         }
     }
-    const unsigned am = wave_max_u(accel);
-    const unsigned ct = wave_sum_u(contacts);
+    const unsigned am = wave_max_u_dpp(accel);
+    const unsigned ct = wave_sum_u_dpp(contacts);
     if (lane == 0) {
         BlockPart bp;
 #pragma unroll
