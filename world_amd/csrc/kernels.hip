@@ -80,6 +80,18 @@ __device__ __forceinline__ int wave_reduce_dpp(int v, int identity, Op op) {
     v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
     return __builtin_amdgcn_readlane(v, 63);
 }
+// Inclusive prefix sum over the wave through DPP: shifts by 1, 2, 4, 8 within
+// rows of 16, then row 0's total into row 1 and row 2's into row 3, then
+// rows 0-1's into rows 2-3.  Every lane must be active.
+__device__ __forceinline__ int wave_incl_scan_dpp(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
 __device__ __forceinline__ int wave_min_dpp(int v) {
     return wave_reduce_dpp(v, kEmptyMin, [](int a, int b) { return min(a, b); });
 }
@@ -1295,14 +1307,9 @@ __global__ __launch_bounds__(64) void k_force_q(
                 T += __popcll(mask);
             }
         } else {
-            int incl = cnt;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += t;
-            }
+            const int incl = wave_incl_scan_dpp(cnt);
             const int q0 = incl - cnt;
-            T = __shfl(incl, 63, 64);
+            T = __builtin_amdgcn_readlane(incl, 63);
             gq0 = __shfl(q0, g * L, 64);
             const int gq1n = __shfl(q0, ((g + 1) * L) & 63, 64);
             gq1 = (g + 1) * L < 64 ? gq1n : T;
@@ -1425,8 +1432,7 @@ __global__ __launch_bounds__(64) void k_force_q(
             s_rec[1][lane] = r1;
             s_rec[2][lane] = r2;
             __syncthreads();
-            auto add = [&](int r) __attribute__((always_inline)) {
-                const float4 a0 = s_rec[0][r], a1 = s_rec[1][r], a2 = s_rec[2][r];
+            auto add_rec = [&](const float4& a0, const float4& a1, const float4& a2) __attribute__((always_inline)) {
                 f[0] += a0.x;
                 f[1] += a0.y;
                 f[2] += a0.z;
@@ -1440,6 +1446,9 @@ __global__ __launch_bounds__(64) void k_force_q(
                     e_grain += a0.w;
                     e_grain += a1.w;
                 }
+            };
+            auto add = [&](int r) __attribute__((always_inline)) {
+                add_rec(s_rec[0][r], s_rec[1][r], s_rec[2][r]);
             };
             if constexpr (L == 1) {
                 // Rows jc, jc + 1, ... overlap the round; a grain adds its
@@ -1457,8 +1466,15 @@ __global__ __launch_bounds__(64) void k_force_q(
                     cj = cn;
                 }
             } else {
+                // A grain's entries of the round, two at a time with both
+                // records' reads in flight (the second clamped to the range).
                 const int r_lo = max(gq0, base), r_hi = min(gq1, base + 64);
-                for (int r = r_lo; r < r_hi; ++r) add(r - base);
+                for (int r = r_lo; r < r_hi; r += 2) {
+                    const int r2 = min(r + 1, r_hi - 1) - base;
+                    const float4 b0 = s_rec[0][r2], b1 = s_rec[1][r2], b2 = s_rec[2][r2];
+                    add(r - base);
+                    if (r + 1 < r_hi) add_rec(b0, b1, b2);
+                }
             }
             __syncthreads();
             if (kAhead) cur = nxt;
@@ -1621,11 +1637,11 @@ __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __re
     }
     e = wave_sum(e);
     const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
-    am = wave_max_u(am);
+    am = wave_max_u_dpp(am);  // every thread of the block is active here
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        bx[k] = wave_min(bx[k]);
-        bx[3 + k] = wave_max(bx[3 + k]);
+        bx[k] = wave_min_dpp(bx[k]);
+        bx[3 + k] = wave_max_dpp(bx[3 + k]);
     }
     if (lane == 0) {
         s_dv[wave][0] = e;
