@@ -46,9 +46,26 @@ __device__ __forceinline__ unsigned wave_max_u(unsigned v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64)));
     return v;
 }
+// Deterministic sum over the wave (every lane active): a fixed tree of DPP
+// row moves, pairs, quads, half rows, rows, then rows 0-1 into row 1, rows
+// 2-3 into row 3 and rows 0-1 into row 3; the result is lane 63's, the same
+// for every lane.  Lanes a move does not reach add +0.0 (no effect on lane
+// 63's sum).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_move_f64<0xB1, 0xF>(v);   // quad_perm 1,0,3,2
+    v += dpp_move_f64<0x4E, 0xF>(v);   // quad_perm 2,3,0,1
+    v += dpp_move_f64<0x141, 0xF>(v);  // row_half_mirror
+    v += dpp_move_f64<0x140, 0xF>(v);  // row_mirror
+    v += dpp_move_f64<0x142, 0xA>(v);  // row_bcast:15
+    v += dpp_move_f64<0x143, 0xC>(v);  // row_bcast:31
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 // Deterministic sum of one double per grain over kEnergyGroup consecutive
 // grains held L lanes apart: a butterfly over the grain index, so the bits do
@@ -308,8 +325,8 @@ __device__ __forceinline__ double ewave_share(const double* __restrict__ ewave, 
     return e;
 }
 
-// Deterministic block sum of one double per thread (fixed butterfly and wave
-// order); the result is valid in thread 0.
+// Deterministic block sum of one double per thread (a fixed tree per wave,
+// then the waves in order); the result is valid in thread 0.
 __device__ double block_sum(double v) {
     __shared__ double s_part[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
