@@ -610,14 +610,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
             v[k] = sum;  // exclusive within the thread
             sum += c;
         }
-        // Wave-level exclusive scan of the thread sums.
-        uint32_t incl = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, 64));
-            if (lane >= o) incl += t;
-        }
-        const uint32_t wave_total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
-        over = wave_sum_u(over);
+        // Wave-level scan of the thread sums.
+        const uint32_t incl = static_cast<uint32_t>(wave_incl_scan_dpp(static_cast<int>(sum)));
+        const uint32_t wave_total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+        over = wave_sum_u_dpp(over);
         if (lane == 0) {
             s_wave[wave] = wave_total;
             s_over[wave] = over;
@@ -669,7 +665,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    prefix += wave_sum_u(lane <= stop && ok ? static_cast<uint32_t>(w) : 0u);
+                    prefix += wave_sum_u_dpp(lane <= stop && ok ? static_cast<uint32_t>(w) : 0u);
                     if (stop < 64) break;
                     j -= 64;
                 }
