@@ -380,8 +380,8 @@ def test_retuning_keeps_results():
 
 def test_wide_box_scan_matches_oracle(oracle_lib):
     """2,048 grains scattered over a 4 x 4 m box: the occupied box spans
-    ~12 M cells, so the cell scan has more tiles (4,096 cells each) than its
-    grid has blocks (256), and blocks take further tiles from its counter.
+    ~12 M cells, so the cell scan has more tiles (2,048 cells each) than its
+    grid has blocks (1,024), and blocks take further tiles from its counter.
     Cell order and five steps equal the oracle's."""
     c = workloads.column(16, 16, 8, half_width=2.0, ceiling=1.0)
     st, mat = _capi.initial_state(c)
@@ -393,7 +393,7 @@ def test_wide_box_scan_matches_oracle(oracle_lib):
     st = st.astype(np.float32)
     d = (c.x_max - c.x_min) / c.collision_grid_size_x
     span = [int(np.floor((st[:, a].max() - st[:, a].min()) / d)) + 3 for a in range(3)]
-    assert span[0] * span[1] * span[2] > 256 * 4096
+    assert span[0] * span[1] * span[2] > 1024 * 2048
     with ParticleDynamics(c) as sim:
         sim.set_state(st)
         ids, keys = sim.cell_order()

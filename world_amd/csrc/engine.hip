@@ -263,7 +263,7 @@ struct wd_sim {
         // that they spread over every CU; beyond, 256-thread blocks.
         const int bbs = n <= (1 << 18) ? 64 : kBlock;
         const int gb = grain_blocks(n, bbs);
-        const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 256));
+        const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 1024));
         go("k_count", k_count, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tv, key,
            slot, count, n, bin_box_out, bin_cell_out);
         go("k_scan", k_scan, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,

@@ -10,7 +10,7 @@ namespace wd {
 
 constexpr int kBlock = 256;          // threads per block of the per-grain kernels
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 constexpr int kFinalizeBlock = 1024;  // one block; one wave per ensemble member at a time
