@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(64) void k_force_q(
     // grain, row QCAP takes the stores of candidates that are not listed.
     __shared__ uint32_t s_list[L == 1 ? QCAP + 1 : QCAP][64];
     __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
-    __shared__ uint8_t s_src[L == 1 ? QCAP * 64 : 1];  // one lane per grain: owner lane of each queue entry
+    __shared__ uint8_t s_src[QCAP * 64];        // owner lane of each queue entry
     __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
     __shared__ float4 s_rec[3][64];             // per queue entry: force addends, energy addends
     __shared__ int2 s_gh[G];                    // per grain: old spring list (index, count)
@@ -1327,6 +1327,10 @@ __global__ __launch_bounds__(64) void k_force_q(
             const int gq1n = __shfl(q0, ((g + 1) * L) & 63, 64);
             gq1 = (g + 1) * L < 64 ? gq1n : T;
             s_q[lane] = q0;
+#pragma unroll
+            for (int j = 0; j < QCAP; ++j) {
+                if (j < cnt) s_src[q0 + j] = static_cast<uint8_t>(lane);
+            }
         }
         s_slot[lane] = lane_slot + cons;
         __syncthreads();
@@ -1348,11 +1352,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                 t.slot = s_slot[src] + static_cast<int>(w >> 28);
                 t.og = src;
             } else {
-                int src = 0;
-#pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    if (s_q[src + step] <= e) src += step;
-                }
+                const int src = s_src[e];
                 const int j = e - s_q[src];
                 t.q = s_list[j][src];
                 t.slot = s_slot[src] + j;
