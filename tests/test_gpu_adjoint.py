@@ -68,6 +68,32 @@ def test_adjoint_dot_product_small_window(oracle_lib):
         assert vjp == pytest.approx(jvp, rel=REL), name
 
 
+def test_overlapped_sweep_equals_sequential():
+    """With two or more segments the backward sweep recomputes the next
+    segment on a second stream, into a second set of slots, while the
+    contact adjoints of the current one run; with kernel timing on it runs
+    them one after the other.  Both give the same gradients, bit for bit,
+    and over two backward calls of the same window."""
+    c = workloads.column(6, 6, 6, friction=0.3)
+    st, _ = jittered(c, 7)
+    W = np.random.default_rng(3).normal(size=4 * nodes_of(c))
+    out = []
+    for timing in (0, 1):
+        cc = c.copy()
+        cc.kernel_timing = timing
+        with ParticleDynamics(cc) as sim:
+            sim.set_state(st)
+            sim.adjoint_record(60, 8)  # eight segments, the last one partial
+            out.append(sim.adjoint_backward(W))
+            sim.set_state(st)
+            sim.adjoint_record(60, 8)
+            again = sim.adjoint_backward(W)
+            np.testing.assert_array_equal(again[0], out[-1][0])
+            np.testing.assert_array_equal(again[1], out[-1][1])
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 @pytest.fixture(scope="module")
 def window_8k():
     """The 8,192-grain column at mu 0.3 over 200 steps (checkpoints every 20),

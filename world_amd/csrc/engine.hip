@@ -142,7 +142,14 @@ struct wd_sim {
         int nsteps = 0, every = 0, nseg = 0;
         unsigned long long overflow0 = 0;
         std::vector<Snap> ckpt;
-        std::vector<BinSlot> slot;  // every + 1 (the last one receives the final springs)
+        // Slot sets of every + 1 slots (the last receives the final springs):
+        // one set, or two when the backward sweep overlaps the recompute of
+        // the next segment with the contact adjoints of the current one.
+        std::vector<BinSlot> slot;
+        int sets = 1;
+        BinSlot& at(int set, int k) { return slot[static_cast<size_t>(set) * (every + 1) + k]; }
+        hipStream_t rstream = nullptr;       // recompute stream of the overlapped sweep
+        hipEvent_t ev_start = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_swp[2] = {nullptr, nullptr};
         float4 *lx = nullptr, *lv = nullptr, *own_x = nullptr, *own_v = nullptr;
         float4 *lx2 = nullptr, *lv2 = nullptr;  // dL/d(state) of a segment's last step, in its cell order
         float4 *lxi[2] = {nullptr, nullptr}, *cg_d = nullptr, *cg_v = nullptr;
@@ -152,14 +159,22 @@ struct wd_sim {
         float *grid_cot = nullptr, *state_cot = nullptr;
         std::vector<void*> owned;
         // Captured segments (recompute + contact adjoints), keyed by
-        // {steps, spring parity, lxi buffer, force variant}.
-        std::vector<std::pair<std::array<int, 4>, hipGraphExec_t>> seg_graphs;
+        // {steps, spring parity, lxi buffer, force variant}; with two slot
+        // sets, recomputes keyed by {steps, parity, set, variant} and
+        // contact adjoints by {steps, lxi buffer, set, -1}.
+        std::vector<std::pair<std::array<int, 4>, hipGraphExec_t>> seg_graphs, rec_graphs, swp_graphs;
     };
     AdjointState* adj = nullptr;
 
     void drop_adjoint() {
         if (!adj) return;
         for (auto& g : adj->seg_graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto& g : adj->rec_graphs) (void)hipGraphExecDestroy(g.second);
+        for (auto& g : adj->swp_graphs) (void)hipGraphExecDestroy(g.second);
+        for (hipEvent_t e : {adj->ev_start, adj->ev_rec[0], adj->ev_rec[1], adj->ev_swp[0], adj->ev_swp[1]}) {
+            if (e) (void)hipEventDestroy(e);
+        }
+        if (adj->rstream) (void)hipStreamDestroy(adj->rstream);
         for (void* b : adj->owned) (void)hipFree(b);
         delete adj;
         adj = nullptr;
@@ -1412,10 +1427,10 @@ int adj_run_record(wd_sim* s) {
 // the checkpoint load reads only the slots and fixed buffers, so it is
 // captured once per {steps, par, cur, variant} and replayed for the other
 // segments (eagerly with kernel timing on).
-void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
+void adj_recompute_body(wd_sim* s, int set, int steps, int par) {
     auto* A = s->adj;
     for (int k = 0; k < steps; ++k) {
-        wd_sim::BinSlot& b = A->slot[k];
+        wd_sim::BinSlot& b = A->at(set, k);
         s->bin_box_out = b.mbox;  // k_count keeps the step's boxes in the slot
         s->bin_cell_out = b.mcell;
         s->Sp = b.Sp;
@@ -1424,23 +1439,29 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
         s->hidx[0] = b.hidx;
         s->H[par] = b.H;
         s->Hc[par] = b.Hc;
-        s->H[par ^ 1] = A->slot[k + 1].H;
-        s->Hc[par ^ 1] = A->slot[k + 1].Hc;
+        s->H[par ^ 1] = A->at(set, k + 1).H;
+        s->Hc[par ^ 1] = A->at(set, k + 1).Hc;
         s->enqueue_step();
         par ^= 1;
     }
+}
+
+// The contact adjoints of a recomputed segment's steps, in reverse, on
+// stream `st`; they read the slots of `set` and the adjoint buffers only.
+void adj_sweep_body(wd_sim* s, int set, int steps, int cur, hipStream_t st) {
+    auto* A = s->adj;
     const int blocks = static_cast<int>((s->n + kBlock - 1) / kBlock);
     const int agb = grain_blocks(s->n, kAdjBlock);
     // The cotangents arrive in original order (the checkpoint's); the last
     // step's contact adjoint reads them in that step's cell order.
-    hipLaunchKernelGGL(k_adj_to_cells, dim3(blocks), dim3(kBlock), 0, s->stream,
-                       (const float4*)A->slot[steps - 1].Sp, (const float4*)A->lx, (const float4*)A->lv,
+    hipLaunchKernelGGL(k_adj_to_cells, dim3(blocks), dim3(kBlock), 0, st,
+                       (const float4*)A->at(set, steps - 1).Sp, (const float4*)A->lx, (const float4*)A->lv,
                        (const float4*)A->lxi[cur], A->lx2, A->lv2, A->lxi[cur ^ 1], s->n);
     cur ^= 1;
     for (int k = steps - 1; k >= 0; --k) {
-        const wd_sim::BinSlot& b = A->slot[k];
+        const wd_sim::BinSlot& b = A->at(set, k);
         const bool last = k == steps - 1;
-        hipLaunchKernelGGL(k_adj_contacts, dim3(agb), dim3(kAdjBlock), 0, s->stream, s->dp, s->ctl,
+        hipLaunchKernelGGL(k_adj_contacts, dim3(agb), dim3(kAdjBlock), 0, st, s->dp, s->ctl,
                            (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
                            (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
                            (const int*)b.mbox, (const uint32_t*)b.mcell,
@@ -1448,11 +1469,41 @@ void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
                            (const float4*)A->lxi[cur], A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d,
                            A->cg_v, A->cg_cnt, A->cg_p, A->gparam, s->n);
         // dL/d(state before step k), at the previous step's positions
-        hipLaunchKernelGGL(k_adj_gather, dim3((s->n + kAdjBlock - 1) / kAdjBlock), dim3(kAdjBlock), 0, s->stream, A->own_x,
+        hipLaunchKernelGGL(k_adj_gather, dim3((s->n + kAdjBlock - 1) / kAdjBlock), dim3(kAdjBlock), 0, st, A->own_x,
                            A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, (const int*)b.hidx, A->lx,
                            A->lv, s->n);
         cur ^= 1;
     }
+}
+
+void adj_segment_body(wd_sim* s, int steps, int par, int cur) {
+    adj_recompute_body(s, 0, steps, par);
+    adj_sweep_body(s, 0, steps, cur, s->stream);
+}
+
+// Instantiated graph of `body` captured on stream `st`, cached under `key`.
+template <class Body>
+int adj_graph(std::vector<std::pair<std::array<int, 4>, hipGraphExec_t>>& cache, const std::array<int, 4>& key,
+              hipStream_t st, Body body, hipGraphExec_t* out) {
+    for (auto& g : cache) {
+        if (g.first == key) {
+            *out = g.second;
+            return WD_OK;
+        }
+    }
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        body();
+        e = hipStreamEndCapture(st, &g);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(WD_ERR_HIP, std::string("adjoint segment capture: ") + hipGetErrorString(e));
+    cache.push_back({key, exec});
+    *out = exec;
+    return WD_OK;
 }
 
 int adj_segment(wd_sim* s, const wd_sim::Snap& ck, int steps, int par, int cur) {
@@ -1475,8 +1526,8 @@ int adj_segment(wd_sim* s, const wd_sim::Snap& ck, int steps, int par, int cur) 
         s->Hc[0] = Hc0[0];
         s->Hc[1] = Hc0[1];
     };
-    s->H[par] = A->slot[0].H;  // the checkpoint's springs go to slot 0
-    s->Hc[par] = A->slot[0].Hc;
+    s->H[par] = A->at(0, 0).H;  // the checkpoint's springs go to slot 0
+    s->Hc[par] = A->at(0, 0).Hc;
     snap_load(s, ck);
     int rc = WD_OK;
     if (s->timing) {
@@ -1509,6 +1560,64 @@ int adj_segment(wd_sim* s, const wd_sim::Snap& ck, int steps, int par, int cur) 
     return rc;
 }
 
+// The engine's buffer pointers and stream, put back when a recompute that
+// redirected them is enqueued.
+struct EngineBufs {
+    wd_sim* s;
+    float4 *Sp, *Sv, *H[2];
+    uint32_t* start;
+    int *hidx0, *Hc[2];
+    hipStream_t stream;
+    explicit EngineBufs(wd_sim* sim)
+        : s(sim), Sp(sim->Sp), Sv(sim->Sv), H{sim->H[0], sim->H[1]}, start(sim->start),
+          hidx0(sim->hidx[0]), Hc{sim->Hc[0], sim->Hc[1]}, stream(sim->stream) {}
+    ~EngineBufs() {
+        s->bin_box_out = nullptr;
+        s->bin_cell_out = nullptr;
+        s->Sp = Sp;
+        s->Sv = Sv;
+        s->start = start;
+        s->hidx[0] = hidx0;
+        s->H[0] = H[0];
+        s->H[1] = H[1];
+        s->Hc[0] = Hc[0];
+        s->Hc[1] = Hc[1];
+        s->stream = stream;
+    }
+};
+
+// Recomputes a segment from checkpoint `ck` into slot set `set` on the
+// recompute stream, then records ev_rec[set] there.
+int adj_recompute(wd_sim* s, const wd_sim::Snap& ck, int set, int steps, int par) {
+    auto* A = s->adj;
+    EngineBufs keep(s);
+    s->stream = A->rstream;
+    s->H[par] = A->at(set, 0).H;  // the checkpoint's springs go to the set's slot 0
+    s->Hc[par] = A->at(set, 0).Hc;
+    snap_load(s, ck);
+    hipGraphExec_t exec = nullptr;
+    const int rc = adj_graph(A->rec_graphs, {steps, par, set, s->force_var}, A->rstream,
+                             [&]() { adj_recompute_body(s, set, steps, par); }, &exec);
+    if (rc != WD_OK) return rc;
+    if (hipGraphLaunch(exec, A->rstream) != hipSuccess) return fail(WD_ERR_HIP, "adjoint recompute launch");
+    WD_HIP(hipEventRecord(A->ev_rec[set], A->rstream));
+    return WD_OK;
+}
+
+// The contact adjoints of the segment in slot set `set`, on the engine's
+// stream once its recompute is done, then records ev_swp[set].
+int adj_sweep(wd_sim* s, int set, int steps, int cur) {
+    auto* A = s->adj;
+    WD_HIP(hipStreamWaitEvent(s->stream, A->ev_rec[set], 0));
+    hipGraphExec_t exec = nullptr;
+    const int rc = adj_graph(A->swp_graphs, {steps, cur, set, -1}, s->stream,
+                             [&]() { adj_sweep_body(s, set, steps, cur, s->stream); }, &exec);
+    if (rc != WD_OK) return rc;
+    if (hipGraphLaunch(exec, s->stream) != hipSuccess) return fail(WD_ERR_HIP, "adjoint sweep launch");
+    WD_HIP(hipEventRecord(A->ev_swp[set], s->stream));
+    return WD_OK;
+}
+
 }  // namespace
 
 extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
@@ -1533,7 +1642,16 @@ extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
     A->nseg = (nsteps + A->every - 1) / A->every;
     const size_t n = static_cast<size_t>(s->n);
     A->ckpt.resize(A->nseg + 1);
-    A->slot.resize(A->every + 1);
+    A->sets = A->nseg > 1 ? 2 : 1;
+    A->slot.resize(static_cast<size_t>(A->sets) * (A->every + 1));
+    if (A->sets == 2) {
+        if (hipStreamCreateWithFlags(&A->rstream, hipStreamNonBlocking) != hipSuccess) rc = fail(WD_ERR_HIP, "adjoint stream");
+        for (hipEvent_t* e : {&A->ev_start, &A->ev_rec[0], &A->ev_rec[1], &A->ev_swp[0], &A->ev_swp[1]}) {
+            if (rc == WD_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+                rc = fail(WD_ERR_HIP, "adjoint events");
+            }
+        }
+    }
     for (auto& sn : A->ckpt) if (rc == WD_OK) rc = adj_snap(s, sn);
     for (auto& b : A->slot) {
         if (rc == WD_OK) rc = adj_alloc(s, &b.Sp, n);
@@ -1599,13 +1717,45 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
     WD_HIP(hipStreamSynchronize(s->stream));
     WD_HIP(hipMemcpy(&par, &s->ctl->hist_parity, sizeof(int), hipMemcpyDeviceToHost));
     int cur = 0;
-    for (int seg = A->nseg - 1; seg >= 0; --seg) {
-        const int s0 = seg * A->every;
-        const int K = std::min(A->nsteps, s0 + A->every) - s0;
-        rc = adj_segment(s, A->ckpt[seg], K, par, cur);
-        if (rc != WD_OK) break;
-        par ^= K & 1;
-        cur ^= (K + 1) & 1;  // K steps and the move into cell order
+    auto seg_len = [&](int seg) { return std::min(A->nsteps, (seg + 1) * A->every) - seg * A->every; };
+    if (A->sets == 2 && !s->timing) {
+        // Segment seg - 1 is recomputed on the recompute stream, into the
+        // other slot set, while the contact adjoints of segment seg run on
+        // the engine's stream.  A set is recomputed into only once the
+        // adjoints that read it last are done.
+        std::vector<int> seg_par(A->nseg), seg_cur(A->nseg);
+        for (int seg = A->nseg - 1; seg >= 0; --seg) {
+            seg_par[seg] = par;
+            seg_cur[seg] = cur;
+            par ^= seg_len(seg) & 1;
+            cur ^= (seg_len(seg) + 1) & 1;
+        }
+        WD_HIP(hipEventRecord(A->ev_start, s->stream));  // after the seed, before any recompute
+        WD_HIP(hipStreamWaitEvent(A->rstream, A->ev_start, 0));
+        rc = adj_recompute(s, A->ckpt[A->nseg - 1], (A->nseg - 1) & 1, seg_len(A->nseg - 1), seg_par[A->nseg - 1]);
+        for (int seg = A->nseg - 1; seg >= 0 && rc == WD_OK; --seg) {
+            if (seg >= 1) {
+                const int other = (seg - 1) & 1;
+                if (seg + 1 < A->nseg && hipStreamWaitEvent(A->rstream, A->ev_swp[other], 0) != hipSuccess) {
+                    rc = fail(WD_ERR_HIP, "adjoint stream wait");
+                    break;
+                }
+                rc = adj_recompute(s, A->ckpt[seg - 1], other, seg_len(seg - 1), seg_par[seg - 1]);
+                if (rc != WD_OK) break;
+            }
+            rc = adj_sweep(s, seg & 1, seg_len(seg), seg_cur[seg]);
+        }
+        // The engine's stream is past every recompute (the sweep of segment
+        // 0 waited on the last); on an error, let the other stream drain.
+        if (rc != WD_OK) (void)hipStreamSynchronize(A->rstream);
+    } else {
+        for (int seg = A->nseg - 1; seg >= 0; --seg) {
+            const int K = seg_len(seg);
+            rc = adj_segment(s, A->ckpt[seg], K, par, cur);
+            if (rc != WD_OK) break;
+            par ^= K & 1;
+            cur ^= (K + 1) & 1;  // K steps and the move into cell order
+        }
     }
     s->energy_on = energy;
     if (rc != WD_OK) return rc;
