@@ -286,7 +286,11 @@ int wd_set_friction(wd_sim* sim, int member, float mu);
 int wd_set_normal_stiffness(wd_sim* sim, float k_n);
 
 /* Advances nsteps steps from the current state (as wd_step) and keeps a
- * checkpoint of the state and its springs every checkpoint_every steps. */
+ * checkpoint of the state and its springs every checkpoint_every steps.
+ * The backward pass holds per-step slots for checkpoint_every + 1 steps
+ * (cell-sorted state, cell starts, springs: about 420 B per grain each),
+ * twice when the window has more than one segment (the next segment is
+ * recomputed beside the current one's adjoints). */
 int wd_adjoint_record(wd_sim* sim, int nsteps, int checkpoint_every);
 
 /* Given the cotangent of the window's final CIC grids (members x 4 x nodes
