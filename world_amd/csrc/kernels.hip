@@ -34,18 +34,6 @@ namespace {
 constexpr int kEmptyMin = 0x7FFFFFFF;
 constexpr int kEmptyMax = (-0x7FFFFFFF - 1);
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ unsigned wave_max_u(unsigned v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64)));
-    return v;
-}
 // Deterministic sum over the wave (every lane active): a fixed tree of DPP
 // row moves, pairs, quads, half rows, rows, then rows 0-1 into row 1, rows
 // 2-3 into row 3 and rows 0-1 into row 3; the result is lane 63's, the same
@@ -74,10 +62,6 @@ template <int L>
 __device__ __forceinline__ double group_energy(double v) {
 #pragma unroll
     for (int o = L; o < kEnergyGroup * L; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
-    for (int o = 32; o > 0; o >>= 1) v += static_cast<unsigned>(__shfl_xor(static_cast<int>(v), o, 64));
     return v;
 }
 
@@ -353,8 +337,8 @@ __device__ void store_block_part(BlockPart* parts, BlockMembers* pm, int slot, i
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     // block's member range
-    int mlo = wave_min(member < 0 ? kEmptyMin : member);
-    int mhi = wave_max(member);
+    int mlo = wave_min_dpp(member < 0 ? kEmptyMin : member);
+    int mhi = wave_max_dpp(member);
     if (lane == 0) {
         s_m[wave][0] = mlo;
         s_m[wave][1] = mhi;
@@ -371,20 +355,20 @@ __device__ void store_block_part(BlockPart* parts, BlockMembers* pm, int slot, i
     int v[13];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v[k] = wave_min(in0 ? lo[k] : kEmptyMin);
-        v[3 + k] = wave_max(in0 ? hi[k] : kEmptyMax);
+        v[k] = wave_min_dpp(in0 ? lo[k] : kEmptyMin);
+        v[3 + k] = wave_max_dpp(in0 ? hi[k] : kEmptyMax);
         v[6 + k] = kEmptyMin;
         v[9 + k] = kEmptyMax;
     }
     if (mhi != mlo && mhi >= 0) {  // block-uniform: a second member only where one starts
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            v[6 + k] = wave_min(in1 ? lo[k] : kEmptyMin);
-            v[9 + k] = wave_max(in1 ? hi[k] : kEmptyMax);
+            v[6 + k] = wave_min_dpp(in1 ? lo[k] : kEmptyMin);
+            v[9 + k] = wave_max_dpp(in1 ? hi[k] : kEmptyMax);
         }
     }
-    v[12] = static_cast<int>(wave_max_u(accel));
-    const unsigned cw = wave_sum_u(contacts);
+    v[12] = static_cast<int>(wave_max_u_dpp(accel));
+    const unsigned cw = wave_sum_u_dpp(contacts);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 13; ++k) s_box[wave][k] = v[k];
@@ -1593,8 +1577,8 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v[k] = wave_min(lo[k]);
-        v[3 + k] = wave_max(hi[k]);
+        v[k] = wave_min_dpp(lo[k]);
+        v[3 + k] = wave_max_dpp(hi[k]);
     }
 }
 
@@ -1750,8 +1734,8 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
             int w6[6];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                w6[k] = wave_min(valid ? bp->box0[k] : kEmptyMin);
-                w6[3 + k] = wave_max(valid ? bp->box0[3 + k] : kEmptyMax);
+                w6[k] = wave_min_dpp(valid ? bp->box0[k] : kEmptyMin);
+                w6[3 + k] = wave_max_dpp(valid ? bp->box0[3 + k] : kEmptyMax);
             }
             if (lane == 0) merge(mw, w6);
             continue;
@@ -1783,7 +1767,7 @@ __device__ void finalize_members(const Params& P, Ctl* ctl, const BlockPart* __r
     }
     e = wave_sum(e);
     const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
-    am = wave_max_u(am);
+    am = wave_max_u_dpp(am);
     if (lane == 0) {
         s_dv[wave][0] = e;
         s_dv[wave][1] = cw;
@@ -1856,7 +1840,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     }
     const double total = block_sum(e);
     const double contacts = block_sum(static_cast<double>(csum));  // exact below 2^53
-    const unsigned amax_w = wave_max_u(am);
+    const unsigned amax_w = wave_max_u_dpp(am);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     if (lane == 0) s_box[wave][0] = static_cast<int>(amax_w);
@@ -2189,7 +2173,7 @@ __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict
                 active ? ix[1] : kEmptyMax, active ? iy[1] : kEmptyMax, active ? iz[1] : kEmptyMax,
                 active ? member : kEmptyMin, active ? member : kEmptyMax};
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r[k] = (k < 3 || k == 6) ? wave_min(r[k]) : wave_max(r[k]);
+    for (int k = 0; k < 8; ++k) r[k] = (k < 3 || k == 6) ? wave_min_dpp(r[k]) : wave_max_dpp(r[k]);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) s_red[wave][k] = r[k];
