@@ -274,9 +274,10 @@ struct wd_sim {
     // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
     void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
                      float4* nout_v) {
-        // Up to 256K grains the per-grain bin passes run one-wave blocks, so
-        // that they spread over every CU; beyond, 256-thread blocks.
-        const int bbs = n <= (1 << 18) ? 64 : kBlock;
+        // The per-grain bin passes run one-wave blocks: at 32K grains they
+        // then spread over every CU, at 1M they are ~2% faster than 256-thread
+        // blocks.
+        const int bbs = 64;
         const int gb = grain_blocks(n, bbs);
         const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 1024));
         go("k_count", k_count, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tv, key,

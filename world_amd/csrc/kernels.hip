@@ -531,13 +531,32 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
         for (int k = threadIdx.x; k < 6 * P.members; k += blockDim.x) box_out[k] = P.mbox[k];
         for (int k = threadIdx.x; k <= P.members; k += blockDim.x) cell_out[k] = P.mcell[k];
     }
+    // Grains arrive in the last step's cell order, so neighbouring lanes
+    // often share a cell: a run of equal cells in the wave takes its slots
+    // with one atomic (the order of slots inside a cell is free: the reorder
+    // ranks a cell's grains by id).
     const int i = logical_block() * blockDim.x + threadIdx.x;
-    if (i < n) {
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    uint32_t c = 0xFFFFFFFFu;  // no cell
+    if (valid) {
         const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
         int cx, cy, cz;
         unpack_cell(key[i], cx, cy, cz);
-        slot[i] = atomicAdd(&count[mc.index(cx, cy, cz)], 1u);
+        c = mc.index(cx, cy, cz);
     }
+    // The previous lane's cell (wave_shr:1; lane 0 gets none).
+    const uint32_t cprev = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(-2, static_cast<int>(c), 0x138, 0xF, 0xF, false));
+    const unsigned long long heads = __ballot(!valid || c != cprev || lane == 0);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = 63 - __clzll(static_cast<long long>(heads & upto));  // this run's first lane
+    const unsigned long long after = heads & ~upto;
+    const int e = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;  // the next run's
+    uint32_t base = 0u;
+    if (valid && lane == h) base = atomicAdd(&count[c], static_cast<uint32_t>(e - h));
+    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), h, 64));
+    if (valid) slot[i] = base + static_cast<uint32_t>(lane - h);
 }
 
 // Pass 2: start[c] = sum of count[0..c) over the concatenated member
