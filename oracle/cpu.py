@@ -67,6 +67,8 @@ def lib() -> C.CDLL:
         h.or_sim_max_accel.restype = C.c_float
         h.or_sim_history.argtypes = [vp, vp, vp, vp]
         h.or_sim_history.restype = None
+        h.or_sim_set_history.argtypes = [vp, vp, vp, vp]
+        h.or_sim_set_history.restype = None
         h.or_sim_energy.argtypes = [vp, C.c_int]
         h.or_sim_energy.restype = C.c_double
         h.or_rec_create.argtypes = [P, C.c_int, vp, vp, C.c_int, C.c_int]
@@ -223,6 +225,15 @@ class Sim:
         n = self.n
         return (partner[: n * HIST_SLOTS].reshape(n, HIST_SLOTS),
                 xi[: n * HIST_SLOTS * 3].reshape(n, HIST_SLOTS, 3), cnt[:n])
+
+    def set_history(self, partner, xi, count) -> None:
+        """Replaces the stored springs (the layout history() returns), to
+        continue from a state and springs taken mid-run."""
+        p = np.ascontiguousarray(partner, np.int32).reshape(-1)
+        x = np.ascontiguousarray(xi, np.float32).reshape(-1)
+        c = np.ascontiguousarray(count, np.int32).reshape(-1)
+        assert p.size == self.n * HIST_SLOTS and x.size == 3 * p.size and c.size == self.n
+        lib().or_sim_set_history(self._h, _p(p), _p(x), _p(c))
 
 
 class Recording:

@@ -1128,6 +1128,24 @@ void or_sim_history(void* h, int* partner, float* xi, int* count) {
     }
 }
 
+// Replaces the stored springs (the layout or_sim_history writes), so that a
+// simulation can continue from a state and springs taken mid-run.
+void or_sim_set_history(void* h, const int* partner, const float* xi, const int* count) {
+    OrSim* s = static_cast<OrSim*>(h);
+    for (int i = 0; i < s->n; ++i) {
+        HList& l = s->hist[i];
+        l.clear();
+        const int k_n = std::min(std::max(count[i], 0), kHistSlots);
+        for (int k = 0; k < k_n; ++k) {
+            const size_t o = static_cast<size_t>(i) * kHistSlots + k;
+            HEntry e;
+            e.partner = partner[o];
+            for (int a = 0; a < 3; ++a) e.xi[a] = xi[3 * o + a];
+            l.push_back(e);
+        }
+    }
+}
+
 // Energy of the current state including the stored tangential springs.
 double or_sim_energy(void* h, int precision) {
     const OrSim* s = static_cast<OrSim*>(h);

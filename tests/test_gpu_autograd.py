@@ -48,3 +48,40 @@ def test_autograd_density_loss_matches_adjoint_and_oracle(oracle_lib):
         assert vjp == pytest.approx(jvp, rel=1e-4)
     finally:
         win.close()
+
+
+def test_two_forwards_then_one_backward_and_broadcast_friction():
+    """L(f(x_a)) + L(f(x_b)) through one window: each term is differentiated
+    along its own trajectory (the same gradients, bit for bit, as two
+    separate forward/backward passes).  A scalar friction broadcast over a
+    two-member batch gets the sum of the members' gradients."""
+    import torch
+
+    cs = [workloads.column(5, 5, 6, friction=0.3), workloads.column(4, 4, 7, friction=0.3, seed=9)]
+    win = DensityWindow(cs, 30)
+    try:
+        xa = win.x0.copy()
+        xb = xa + np.random.default_rng(1).normal(0, 2e-4, xa.shape).astype(np.float32)
+        target = torch.zeros(2, win.nodes, dtype=torch.float32)
+
+        def loss_of(x, mu):
+            return ((density_window(x, mu, torch.tensor(150.0), win)[:, 0] - target) ** 2).sum()
+
+        ta = torch.tensor(xa, requires_grad=True)
+        tb = torch.tensor(xb, requires_grad=True)
+        mu = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+        (loss_of(ta, mu) + loss_of(tb, mu)).backward()
+
+        sep = []
+        for x in (xa, xb):
+            tx = torch.tensor(x, requires_grad=True)
+            tmu = torch.tensor([0.3, 0.3], dtype=torch.float64, requires_grad=True)
+            loss_of(tx, tmu).backward()
+            sep.append((tx.grad.numpy().copy(), tmu.grad.numpy().copy()))
+        np.testing.assert_array_equal(ta.grad.numpy(), sep[0][0])
+        np.testing.assert_array_equal(tb.grad.numpy(), sep[1][0])
+        assert not np.array_equal(sep[0][0], sep[1][0])
+        assert mu.grad.shape == mu.shape
+        assert float(mu.grad) == pytest.approx(float(sep[0][1].sum() + sep[1][1].sum()), rel=1e-12)
+    finally:
+        win.close()

@@ -133,3 +133,24 @@ def test_member_overflow_and_stability():
         np.testing.assert_array_equal(ens.member_stable(), want_stable)
         assert ens.grid_overflow_count() == sum(want_over)
     assert sum(want_over) > 0 and not want_stable[0] and all(want_stable[1:])
+
+
+def test_batch_parameter_setters_equal_created_parameters():
+    """Friction per member and the normal stiffness set on a live batch
+    handle (the autograd window's forward does this) give the same steps,
+    bit for bit, as a batch created with those parameters.  (Setting them
+    once dropped the member partials' buffer from the solver constants, and
+    the next reduction read through a null pointer.)"""
+    cs = members()
+    want_cfgs = [c.copy() for c in cs]
+    for m, c in enumerate(want_cfgs):
+        c.friction = 0.1 * m
+        c.normal_stiffness = 180.0
+    with Ensemble(cs) as ens, Ensemble(want_cfgs) as want:
+        for m, c in enumerate(want_cfgs):
+            ens.set_friction(c.friction, m)
+        ens.set_normal_stiffness(180.0)
+        ens.take_step(25)
+        want.take_step(25)
+        np.testing.assert_array_equal(ens.state(), want.state())
+        assert ens.member_stable().all()

@@ -378,6 +378,29 @@ def test_retuning_keeps_results():
         assert_springs_equal(tuned.history(), fixed.history())
 
 
+def test_one_step_calls_are_tuned():
+    """A caller that advances one step per call (the C++ class's take_step)
+    is tuned across its calls: after 18 one-step calls the tuner has timed
+    every variant and holds its choice, and the steps equal pinned steps."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, _ = jittered(c, seed=41)
+    with ParticleDynamics(c) as one, ParticleDynamics(c) as fixed:
+        one.set_state(st)
+        fixed.set_state(st)
+        fixed.set_force_variant(256, 0)
+        seen = set()
+        for _ in range(18):
+            one.take_step(1)
+            seen.add(one.force_variant())
+        assert seen == set(FORCE_VARIANTS)  # every candidate ran on its own step
+        chosen = one.force_variant()
+        for _ in range(5):
+            one.take_step(1)
+            assert one.force_variant() == chosen  # tuning is over until step 256
+        fixed.take_step(23)
+        np.testing.assert_array_equal(one.state(), fixed.state())
+
+
 def test_wide_box_scan_matches_oracle(oracle_lib):
     """2,048 grains scattered over a 4 x 4 m box: the occupied box spans
     ~12 M cells, so the cell scan has more tiles (2,048 cells each) than its

@@ -11,6 +11,12 @@ The reference solver has no differentiation; its surrogate side
 (surrogate/) consumes CIC grids written by the dataset driver, which is the
 quantity differentiated here.  Host buffers cross the C ABI; torch tensors
 are plumbing on either side.
+
+A window handle holds one recorded trajectory at a time.  Each forward
+through it is numbered; a backward whose forward is no longer the window's
+latest re-records that forward from its saved inputs first (the run is
+deterministic, so it is the same trajectory bit for bit), so that
+L(f(x_a)) + L(f(x_b)) differentiates each term along its own trajectory.
 """
 from __future__ import annotations
 
@@ -37,6 +43,7 @@ class DensityWindow:
         self.v0 = st[:, 3:].copy()
         self.x0 = st[:, :3].copy()
         self.nodes = self.sim.density_grid_nodes()
+        self.generation = 0  # number of the forward the handle holds
 
     def close(self) -> None:
         self.sim.close()
@@ -50,6 +57,7 @@ class DensityWindow:
         self.sim.set_normal_stiffness(float(k_n))
         self.sim.set_state(st)  # also clears the contact history
         self.sim.adjoint_record(self.nsteps, self.every)
+        self.generation += 1
         if self.members > 1:
             grids = self.sim.density_grids()
         else:
@@ -70,15 +78,26 @@ def density_window_function():
     class DensityWindowFunction(torch.autograd.Function):
         @staticmethod
         def forward(ctx, x0, mu, k_n, window: DensityWindow):
-            grids = window.run(x0.detach().cpu().numpy(), mu.detach().cpu().numpy(),
-                               float(k_n.detach().cpu()))
+            inputs = (x0.detach().cpu().numpy().copy(), mu.detach().cpu().numpy().copy(),
+                      float(k_n.detach().cpu()))
+            grids = window.run(*inputs)
             ctx.window = window
+            ctx.inputs = inputs
+            ctx.generation = window.generation
             ctx.meta = [(t.dtype, t.shape, t.device) for t in (x0, mu, k_n)]
             return torch.from_numpy(grids).to(x0.device)
 
         @staticmethod
         def backward(ctx, g):
-            gx, gmu, gk = ctx.window.backward(g.detach().cpu().numpy())
+            win = ctx.window
+            if win.generation != ctx.generation:
+                # another forward ran through the window since: record this
+                # one's trajectory again from its inputs
+                win.run(*ctx.inputs)
+                ctx.generation = win.generation
+            gx, gmu, gk = win.backward(g.detach().cpu().numpy())
+            if ctx.meta[1][1].numel() == 1:
+                gmu = np.asarray(gmu).sum()  # one friction broadcast to every member
             out = []
             for val, (dtype, shape, dev) in zip((gx, gmu, np.array(gk)), ctx.meta):
                 out.append(torch.as_tensor(np.asarray(val)).reshape(shape).to(device=dev, dtype=dtype))

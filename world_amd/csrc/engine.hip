@@ -427,68 +427,88 @@ struct wd_sim {
     }
 
     // Online autotuning of the force pass's variant (block size, lanes per
-    // grain): the first steps run eagerly, cycling through the candidates
-    // round-robin (so that the evolving state biases none of them), with the
-    // force kernels and the reductions of their partials (more partials for
-    // fewer grains per block) timed by kernel-stamped events; the first
-    // round warms up, the fastest of the rest is kept.  These are real steps
-    // of the simulation (the results do not depend on the variant).  The
-    // best variant moves as contacts build up (a settling pile has several
-    // times the contacts of the released lattice), so tuning runs again
-    // after 256, 2048, 16384, ... steps.
-    int autotune(int& nsteps) {
-        if (n == 0 || steps_run < next_tune) return WD_OK;
-        constexpr int kCand = static_cast<int>(sizeof(kForceVariants) / sizeof(kForceVariants[0]));
+    // grain): steps run eagerly, cycling through the candidates round-robin
+    // (so that the evolving state biases none of them), with the force
+    // kernels and the reductions of their partials (more partials for fewer
+    // grains per block) timed by kernel-stamped events; the first round warms
+    // up, the fastest of the rest is kept.  These are real steps of the
+    // simulation (the results do not depend on the variant).  A tuning spans
+    // as many calls as it needs: a caller that advances one step per call
+    // (the C++ class's take_step) is tuned over its first 18 calls.  The best
+    // variant moves as contacts build up (a settling pile has several times
+    // the contacts of the released lattice), so tuning runs again after 256,
+    // 2048, 16384, ... steps.
+    static constexpr int kCand = static_cast<int>(sizeof(kForceVariants) / sizeof(kForceVariants[0]));
+    struct Tuning {
+        int pos = -1;        // next (round, candidate) position; -1: no tuning in progress
+        int rounds = 0;
+        bool first = false;
+        int before = -1;     // variant the graphs were captured with
+        bool cand[kCand] = {};
+        double best[kCand] = {};
+    } tuning;
+
+    void begin_tuning() {
+        Tuning& t = tuning;
         // The first tuning warms every variant up for a round and keeps the
         // best of two more; a later one times each remaining candidate once
         // and drops those that were more than 1.5x the best last time.
-        const bool first = force_var < 0 || tune_ms[0] == 0.0;
-        const int rounds = first ? 3 : 1;
-        bool cand[kCand];
-        int ncand = 0;
+        t.first = force_var < 0 || tune_ms[0] == 0.0;
+        t.rounds = t.first ? 3 : 1;
         double last_best = 1e30;
         for (int c = 0; c < kCand; ++c) last_best = std::min(last_best, tune_ms[c]);
         for (int c = 0; c < kCand; ++c) {
-            cand[c] = first || tune_ms[c] <= 1.5 * last_best;
-            ncand += cand[c] ? 1 : 0;
+            t.cand[c] = t.first || tune_ms[c] <= 1.5 * last_best;
+            t.best[c] = 1e30;
         }
-        if (nsteps < rounds * ncand) return WD_OK;  // tune on a longer call
-        const int before = force_var;
-        double best_ms[kCand];
-        for (int c = 0; c < kCand; ++c) best_ms[c] = 1e30;
-        for (int r = 0; r < rounds; ++r) {
-            for (int c = 0; c < kCand; ++c) {
-                if (!cand[c]) continue;
-                force_var = c;
-                std::vector<ProfRec> recs;
-                prof = &recs;
-                enqueue_step();
-                prof = nullptr;
-                WD_HIP(hipStreamSynchronize(stream));
-                double t = 0.0;
-                for (auto& rec : recs) {
-                    float e = 0.f;
-                    // the force pass and the reduction of its block partials
-                    const bool counted = std::strcmp(rec.name, "k_force") == 0 ||
-                                         std::strcmp(rec.name, "k_finalize") == 0;
-                    if (counted && hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) t += e;
-                    spare.push_back(rec.a);
-                    spare.push_back(rec.b);
-                }
-                if (r > 0 || !first) best_ms[c] = std::min(best_ms[c], t);
-                time += hp.dt;
-                --nsteps;
-                ++steps_run;
+        t.before = force_var;
+        t.pos = 0;
+    }
+
+    int autotune(int& nsteps) {
+        if (n == 0) return WD_OK;
+        if (tuning.pos < 0) {
+            if (steps_run < next_tune) return WD_OK;
+            begin_tuning();
+        }
+        Tuning& t = tuning;
+        const int total = t.rounds * kCand;
+        while (nsteps > 0 && t.pos < total) {
+            const int r = t.pos / kCand, c = t.pos % kCand;
+            ++t.pos;
+            if (!t.cand[c]) continue;
+            force_var = c;
+            std::vector<ProfRec> recs;
+            prof = &recs;
+            enqueue_step();
+            prof = nullptr;
+            WD_HIP(hipStreamSynchronize(stream));
+            double ms = 0.0;
+            for (auto& rec : recs) {
+                float e = 0.f;
+                // the force pass and the reduction of its block partials
+                const bool counted = std::strcmp(rec.name, "k_force") == 0 ||
+                                     std::strcmp(rec.name, "k_finalize") == 0;
+                if (counted && hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) ms += e;
+                spare.push_back(rec.a);
+                spare.push_back(rec.b);
             }
+            if (r > 0 || !t.first) t.best[c] = std::min(t.best[c], ms);
+            time += hp.dt;
+            --nsteps;
+            ++steps_run;
         }
+        while (t.pos < total && !t.cand[t.pos % kCand]) ++t.pos;
+        if (t.pos < total) return check_device();  // continues on the next call
         int best = -1;
         for (int c = 0; c < kCand; ++c) {
-            if (cand[c] && (best < 0 || best_ms[c] < best_ms[best])) best = c;
+            if (t.cand[c] && (best < 0 || t.best[c] < t.best[best])) best = c;
         }
-        for (int c = 0; c < kCand; ++c) tune_ms[c] = best_ms[c];
+        for (int c = 0; c < kCand; ++c) tune_ms[c] = t.best[c];
         force_var = best;
         next_tune = next_tune == 0 ? 256 : 8 * next_tune;
-        if (best != before) drop_graphs();  // captured with another variant
+        if (best != t.before) drop_graphs();  // captured with another variant
+        t.pos = -1;
         return check_device();
     }
 
@@ -653,6 +673,13 @@ void fill_params(const wd_config& c, int n, Params& P) {
 }
 
 int upload_params(wd_sim* s) {
+    const Params& P = s->hp;
+    // every kernel dereferences these unconditionally (the member partials
+    // only in a batch): refuse to publish a table that would fault
+    if (!P.mgoff || !P.mbox || !P.mcell || !P.mmu || !P.mscale || !P.movf ||
+        (P.members > 1 && (!P.pmem || !P.mtab))) {
+        return fail(WD_ERR_STATE, "internal: solver constants lost a device buffer");
+    }
     WD_HIP(hipMemcpy(s->dp, &s->hp, sizeof(Params), hipMemcpyHostToDevice));
     return WD_OK;
 }
@@ -674,8 +701,9 @@ int upload_member_tables(wd_sim* s) {
     return WD_OK;
 }
 
-// Recomputes the derived constants from s->cfg (keeping the member tables
-// and the current dt) and uploads them; graphs read them through dp.
+// Recomputes the derived constants from s->cfg (keeping the current dt and
+// every member table and buffer the handle owns) and uploads them; graphs
+// read them through dp.
 int refresh_params(wd_sim* s) {
     Params keep = s->hp;
     fill_params(s->cfg, s->n, s->hp);
@@ -688,6 +716,7 @@ int refresh_params(wd_sim* s) {
     s->hp.mscale = keep.mscale;
     s->hp.movf = keep.movf;
     s->hp.mtab = keep.mtab;
+    s->hp.pmem = keep.pmem;  // member partials of a batch: k_finalize reads them
     const int rc = upload_member_tables(s);
     if (rc != WD_OK) return rc;
     return upload_params(s);
@@ -1806,7 +1835,8 @@ extern "C" int wd_set_force_variant(wd_sim* s, int block, int lanes) {
     WD_HIP(hipStreamSynchronize(s->stream));
     s->force_var = v;
     s->force_fixed = v >= 0;
-    s->next_tune = s->steps_run;  // (0, 0): tune on the next long enough call
+    s->next_tune = s->steps_run;  // (0, 0): tune from the next call on
+    s->tuning.pos = -1;
     for (double& t : s->tune_ms) t = 0.0;
     s->drop_graphs();
     return WD_OK;

@@ -40,19 +40,23 @@ def column(nx: int, ny: int, nz: int, *, half_width: float = 0.9, ceiling: float
     the floor of a (2*half_width)^2 box of height `ceiling`."""
     c = default_config()
     c.dt = dt
-    cells_xy = int(round(2 * half_width / DIAMETER))
-    cells_z = int(round(ceiling / DIAMETER)) + 2
+    # The cell domain reaches two diameters beyond every wall: the soft
+    # contacts let the lowest centres under a tall column sink slightly
+    # through the floor, and a grain reaching a side wall at 2 m/s overlaps it
+    # by more than a radius, so its centre crosses the wall's plane.  Every
+    # centre then stays inside the domain, which is what is_stable checks.
+    margin = 2 * DIAMETER
+    cells_xy = int(round(2 * (half_width + margin) / DIAMETER))
+    cells_z = int(round((ceiling + 2 * margin) / DIAMETER))
     c.collision_grid_size_x = cells_xy
     c.collision_grid_size_y = cells_xy
     c.collision_grid_size_z = cells_z
     c.particles_per_cell = 8
     c.threads_per_block = 256
     c.density_grid_size_x = c.density_grid_size_y = c.density_grid_size_z = density_nodes
-    c.x_min, c.x_max = -half_width, half_width
-    c.y_min, c.y_max = -half_width, half_width
-    # The cell domain reaches two diameters below the floor: soft contacts
-    # under a tall column push the lowest centres slightly through it.
-    c.z_min, c.z_max = -2 * DIAMETER, ceiling
+    c.x_min, c.x_max = -half_width - margin, half_width + margin
+    c.y_min, c.y_max = -half_width - margin, half_width + margin
+    c.z_min, c.z_max = -margin, ceiling + margin
     c.gravity = 9.81
     c.particle_radius = RADIUS
     c.normal_stiffness = K_N
@@ -152,4 +156,68 @@ def uneven_ensemble(members: int = 64, *, seed: int = 2024, n_min: int = 2048,
         h = max(1, int(round(n / (w * w))))
         c = column(w, w, h, friction=f, ceiling=1.0, dt=dt)
         out.append(perturbed(c, sigma_m, 1000 + e))
+    return out
+
+
+# The demo column of the reference's interactive path (demos/sand.yaml and
+# demos/fluid.yaml, which differ only in physics.friction): 25 x 25 x 52 =
+# 32,500 grains of radius 0.01 m and mass 0.8 in a 2 m box, the reference's
+# own spring parameterization (k = max_force / radius = 2e5 N/m), restitution
+# 0.3, viscous Coulomb friction, SIE at dt 4e-4; measured after whole frames
+# of 100 steps.
+DEMO_STEPS_PER_FRAME = 100
+
+
+def demo_column(friction: float = 0.5, dt: float = 4e-4) -> WdConfig:
+    """demos/sand.yaml (friction 0.5) or demos/fluid.yaml (friction 0); the
+    reference's video configuration of the same scene changes dt to 1/2520
+    (and the frame to 42 steps, CLAUDE.md:1264-1267)."""
+    c = default_config()
+    c.dt = dt
+    c.collision_grid_size_x = c.collision_grid_size_y = c.collision_grid_size_z = 100
+    c.particles_per_cell = 8
+    c.threads_per_block = 256
+    c.density_grid_size_x = c.density_grid_size_y = c.density_grid_size_z = 32
+    c.x_min, c.x_max = -1.0, 1.0
+    c.y_min, c.y_max = -1.0, 1.0
+    c.z_min, c.z_max = -1.0, 1.0
+    c.gravity = 9.81
+    c.particle_radius = 0.01
+    c.max_force = 2000.0
+    c.floor_z, c.ceiling_z = -1.0, 1.0
+    c.wall_x_min, c.wall_x_max = -1.0, 1.0
+    c.wall_y_min, c.wall_y_max = -1.0, 1.0
+    c.restitution = 0.3
+    c.friction = friction
+    c.set_masses([0.0, 0.8])
+    c.init_type = WD_INIT_CUBE
+    c.init_x0, c.init_y0, c.init_z0 = -0.242, -0.242, -0.99
+    c.init_vx0 = c.init_vy0 = c.init_vz0 = 0.0
+    c.init_jitter = 0.25
+    c.init_seed = 12345
+    c.cube_length_x, c.cube_length_y, c.cube_length_z = 0.50, 0.50, 1.05
+    c.particle_material = 1
+    c.two_particle_separation = 0.1
+    c.time_integrator = WD_SEMI_IMPLICIT_EULER
+    c.picard_iterations = 3
+    c.kernel_timing = 0
+    return c
+
+
+def demo_ensemble(friction: float, members: int, perturbation: float = 0.25,
+                  dt: float = 4e-4) -> list[WdConfig]:
+    """The demo column and members - 1 distinct realizations of it: every
+    grain offset uniformly by up to `perturbation` radii on top of the
+    jittered lattice, perturbation seed 1000 + e (the chaos driver's member
+    seeding, src/chaos.cpp:163-166); member 0 is unperturbed.  0.25, the
+    jitter's own magnitude, is the value the reference's chaos configuration
+    gives for measuring "the scatter between distinct realizations of the
+    same macroscopic state" (chaos/ensemble.yaml:92-97)."""
+    base = demo_column(friction, dt)
+    out = [base]
+    for e in range(1, members):
+        c = base.copy()
+        c.init_perturbation = perturbation
+        c.init_perturbation_seed = 1000 + e
+        out.append(c)
     return out
