@@ -340,6 +340,18 @@ int wd_get_history(wd_sim* sim, int* partner, float* xi, int* count);
  * (flat_cell / particle_cell, src/find_neighbors_kernel.cu:28-40). */
 int wd_cell_order(wd_sim* sim, int* ids_out, uint32_t* keys_out);
 
+/* Extension with no counterpart in the reference: wd_cell_order for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same outputs as
+ * wd_cell_order, staged through a fixed device array of
+ * WD_NOALLOC_MAX_GRAINS ids and keys (8 MiB per GPU the library runs on,
+ * held until the process ends) instead of per-call hipMalloc / hipFree.
+ * Returns WD_ERR_ARG for a handle of more grains.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (threads, or
+ * handles on one GPU driven concurrently) race. */
+#define WD_NOALLOC_MAX_GRAINS (1 << 20)
+int wd_cell_order_noalloc(wd_sim* sim, int* ids_out, uint32_t* keys_out);
+
 /* Neighbor rows in the reference's flat format: n rows of
  * 27 * particles_per_cell ints, original indices, -1 terminated
  * (device_neighbors, src/particle_dynamics.h:48-52;

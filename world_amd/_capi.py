@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libworld_amd.so"
 WD_MAX_MATERIALS = 16
 WD_PATH_MAX = 512
 WD_HISTORY_SLOTS = 24
+WD_NOALLOC_MAX_GRAINS = 1 << 20
 
 WD_INIT_CUBE, WD_INIT_TWO_PARTICLES, WD_INIT_SINGLE_PARTICLE, WD_INIT_FILE = range(4)
 WD_SEMI_IMPLICIT_EULER, WD_BACKWARD_EULER_PICARD = 0, 1
@@ -126,6 +127,7 @@ _SIGNATURES = [
     ("wd_overflow_count", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_stage_ms", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_cell_order", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("wd_cell_order_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_get_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <array>
 #include <vector>
@@ -57,6 +58,7 @@ struct TimedStage {
 struct wd_sim {
     wd_config cfg{};
     int n = 0;
+    int device = 0;               // the HIP device current when the handle was created
     float time = 0.f;
     hipStream_t stream = nullptr;
     Params hp{};
@@ -882,6 +884,10 @@ int create_with(const wd_config* cfgs, int members, const std::vector<int>& goff
     s->goff = goff;
     s->mcfg.assign(cfgs, cfgs + members);
     s->timing = cfgs[0].kernel_timing != 0;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return fail(WD_ERR_HIP, "wd_create: hipGetDevice failed");
+    }
     fill_params(cfgs[0], n, s->hp);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
@@ -1154,6 +1160,70 @@ extern "C" int wd_cell_order(wd_sim* s, int* ids_out, uint32_t* keys_out) {
     }
     if (ids) (void)hipFree(ids);
     if (keys) (void)hipFree(keys);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Cell order without allocating: the staging arrays of wd_cell_order live in
+// a fixed device array instead of per-call hipMalloc / hipFree.
+
+namespace {
+
+// ids in the first half, keys in the second; every call on a device shares
+// that device's copy.
+__device__ __align__(256) uint32_t scratch[2 * static_cast<size_t>(WD_NOALLOC_MAX_GRAINS)];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+uint32_t* scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (`device`), looked
+// up once per device: in host code the array's name is only a stand-in.
+int scratch_on(int device, uint32_t** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(WD_ERR_ARG, "scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            return fail(WD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = scratch_ptr[device];
+    return WD_OK;
+}
+
+}  // namespace
+
+extern "C" int wd_cell_order_noalloc(wd_sim* s, int* ids_out, uint32_t* keys_out) {
+    WD_CHECK_SIM(s);
+    if (!ids_out || !keys_out) return fail(WD_ERR_ARG, "wd_cell_order_noalloc: null output");
+    if (s->n > WD_NOALLOC_MAX_GRAINS) {
+        return fail(WD_ERR_ARG, "wd_cell_order_noalloc: the handle holds " + std::to_string(s->n) +
+                                        " grains, more than the " + std::to_string(WD_NOALLOC_MAX_GRAINS) +
+                                        " its fixed staging array holds");
+    }
+    if (s->n == 0) return WD_OK;
+    int prev = 0;
+    WD_HIP(hipGetDevice(&prev));
+    if (prev != s->device) WD_HIP(hipSetDevice(s->device));
+    uint32_t* base = nullptr;
+    int rc = scratch_on(s->device, &base);
+    if (rc == WD_OK) {
+        int* ids = reinterpret_cast<int*>(base);
+        uint32_t* keys = base + WD_NOALLOC_MAX_GRAINS;
+        s->enqueue_bin_current();
+        hipLaunchKernelGGL(k_export_order, dim3((s->n + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream,
+                           s->dp, s->Sp, ids, keys, s->n);
+        rc = s->sync();
+        if (rc == WD_OK &&
+            (hipMemcpy(ids_out, ids, sizeof(int) * s->n, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(keys_out, keys, sizeof(uint32_t) * s->n, hipMemcpyDeviceToHost) != hipSuccess)) {
+            rc = fail(WD_ERR_HIP, "wd_cell_order_noalloc: copy failed");
+        }
+    }
+    if (prev != s->device) (void)hipSetDevice(prev);
     return rc;
 }
 

@@ -215,6 +215,14 @@ class ParticleDynamics:
         check(lib().wd_cell_order(self._h, ptr(ids), ptr(keys)))
         return ids[: self.n], keys[: self.n]
 
+    def cell_order_noalloc(self) -> tuple[np.ndarray, np.ndarray]:
+        """cell_order() staged through the library's fixed device array
+        (wd_cell_order_noalloc): no device allocation per call."""
+        ids = np.zeros(max(self.n, 1), np.int32)
+        keys = np.zeros(max(self.n, 1), np.uint32)
+        check(lib().wd_cell_order_noalloc(self._h, ptr(ids), ptr(keys)))
+        return ids[: self.n], keys[: self.n]
+
     def neighbor_rows(self) -> np.ndarray:
         row = K_STENCIL_CELLS * self.particles_per_cell
         out = np.full(max(self.n, 1) * row, -1, np.int32)
