@@ -72,7 +72,12 @@ B_FORCE = 56.0
 B_STEP = 132.0
 LEGS = ("main", "runs32k", "large", "ensemble", "adjoint", "batched_adjoint", "sliding", "cpu")
 TUNE_STEPS = 18     # one online tuning round over the six force-pass variants
-CAPTURE_STEPS = 17  # one 16-step graph chunk and one single-step graph
+CAPTURE_STEPS = 31  # 16 + 8 + 4 + 2 + 1: every step graph the engine replays
+
+
+def progress(msg: str) -> None:
+    """One line on stderr per finished leg (stdout carries only the JSON)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def history_bytes(z: float) -> float:
@@ -119,6 +124,17 @@ def run_segment(sim, s0: int, k: int, every: int) -> int:
     return len(deps)
 
 
+def advance(sim, steps: int, chunk: int = 64) -> None:
+    """Untimed steps, waiting for the device every `chunk` steps so that few
+    launches are ever queued at once (a profiler attached to the process
+    records every queued dispatch)."""
+    while steps > 0:
+        k = min(chunk, steps)
+        sim.take_step(k)
+        sim.sync()
+        steps -= k
+
+
 def prepare(sim, s0: int, pin=None) -> tuple[int, int]:
     """Advances untimed to step s0 and leaves the force pass's variant fixed
     and the step graphs captured: one online tuning round ends CAPTURE_STEPS
@@ -126,15 +142,13 @@ def prepare(sim, s0: int, pin=None) -> tuple[int, int]:
     the graphs the window replays."""
     if pin:
         sim.set_force_variant(*pin)
-        if s0 > 0:
-            sim.take_step(s0)
+        advance(sim, s0)
         sim.sync()
         return tuple(pin)
     pre = s0 - TUNE_STEPS - CAPTURE_STEPS
     if pre < 0:
         raise ValueError(f"window start {s0} leaves no room for tuning before it")
-    if pre > 0:
-        sim.take_step(pre)
+    advance(sim, pre)
     sim.set_force_variant(0, 0)  # tune on the next call
     sim.take_step(TUNE_STEPS)
     v = sim.force_variant()
@@ -157,6 +171,7 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
         sim.enable_energy_log(s0 + k + 64)
     sim.enable_density_log(len(deposit_steps(s0, k, every)) + 1)
     variant = prepare(sim, s0, pin)
+    tile = sim.scan_tile()
     passes0, contacts0 = sim.contact_stats()
     hist = cfg.tangential_model == 1
     start = None
@@ -172,7 +187,8 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
     out = {
         "n": n, "elapsed": elapsed, "s0": s0, "k": k, "deposits": ndep,
         "stable": sim.is_stable(), "overflow": sim.grid_overflow_count(),
-        "variant": variant, "contacts_per_grain": z, "history": hist,
+        "variant": variant, "scan_tile": tile, "contacts_per_grain": z, "history": hist,
+        "eager": cfg.kernel_timing != 0,
         "tuned": sim.force_variant() == tuple(variant),
     }
     if keep_state:
@@ -183,10 +199,10 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
     if profile:
         prof = ParticleDynamics(cfg)
         prof.set_force_variant(*variant)
+        prof.set_scan_tile(tile)
         if energy:
             prof.enable_energy_log(s0 + k + 64)
-        if s0 > 0:
-            prof.take_step(s0)
+        advance(prof, s0)
         out["kernels_ms"] = prof.profile_step(k)
         prof.close()
     return out
@@ -259,7 +275,8 @@ def window_config(run: dict, every: int) -> dict:
             "untimed_steps_before": run["s0"], "deposits_in_window": run["deposits"],
             "deposit_every": every, "force_variant": {"block": run["variant"][0],
                                                       "lanes_per_grain": run["variant"][1]},
-            "tuned_before_window": True, "graphs_captured_before_window": True}
+            "scan_tile_cells": run["scan_tile"],
+            "tuned_before_window": True, "launch": "eager" if run["eager"] else "graphs captured before the window"}
 
 
 def runs32k_leg(steps: int = 5000) -> dict:
@@ -272,12 +289,14 @@ def runs32k_leg(steps: int = 5000) -> dict:
     return out
 
 
-def large_leg(k: int, warmup: int, pin, full_steps: int) -> dict:
+def large_leg(k: int, warmup: int, pin, full_steps: int, eager: bool = False) -> dict:
     """The 1,048,576-grain column: a window of whole deposit periods (at
     least k steps) around step 5,000, and the configured run."""
     from world_amd import workloads
 
     cfg = workloads.column_collapse_1m()
+    if eager:
+        cfg.kernel_timing = 1
     kl = 100 * max(1, -(-k // 100))
     s0 = window_start(5000, kl, 100, warmup)
     r = window(cfg, kl, s0, 100, False, pin)
@@ -535,6 +554,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the event-timed replay of each window (no per-kernel means)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel of the headline and 1M legs eagerly (the engine's "
+                         "kernel-timing mode) instead of replaying step graphs: for profiler runs, "
+                         "since rocprofv3 has crashed following long runs of graph launches")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 256 MiB device copy (counter calibration)")
     ap.add_argument("--pin-32k", default="", metavar="BLOCK/LANES",
@@ -563,50 +586,66 @@ def main() -> int:
         copy_probe()
     k = args.steps
     cfg = workloads.column_collapse_32k(friction=0.5)
+    if args.eager:
+        cfg.kernel_timing = 1
     s0 = window_start(2500, k, 40, args.warmup)
-    main_run = window(cfg, k, s0, 40, True, pin_of(args.pin_32k), profile=not args.no_profile,
-                      keep_state="cpu" in legs)
-    n = main_run["n"]
-    value = n * k / main_run["elapsed"]
-    cfg_block = {"workload": "column_collapse_32k", "grains": n, "integrator": "semi_implicit_euler",
-                 "friction": 0.5, "tangential_model": "spring_history", "dt": 1e-3,
-                 "cic_nodes": "64^3", "cic_every": 40, "energy_every": 1, "parallelism": "single_gpu"}
-    cfg_block.update(window_config(main_run, 40))
-    out = {
-        "metric": "grain-steps/s (32,768-grain column collapse, SIE, mu 0.5, energy every step, "
-                  "64^3 CIC every 40 steps)",
-        "value": value,
-        "unit": "grain-steps/s",
-        "n_gpus": 1,
-        "steps": k,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * main_run["elapsed"] / k,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded jittered lattice)",
-        "config": cfg_block,
-        "roofline": roofline_of(main_run, "column_collapse_32k"),
-        "step_roofline": step_roofline(main_run, 64 ** 3, 40, value),
-        "kernels_ms": main_run["kernels_ms"],
-        "stable": main_run["stable"],
-        "overflow_cells": main_run["overflow"],
-    }
+    out = {"steps": k, "warmup": args.warmup}
+    main_run = None
+    if "main" in legs:
+        main_run = window(cfg, k, s0, 40, True, pin_of(args.pin_32k), profile=not args.no_profile,
+                          keep_state="cpu" in legs)
+        n = main_run["n"]
+        value = n * k / main_run["elapsed"]
+        progress(f"headline window steps {s0 + 1}..{s0 + k}: {1e3 * main_run['elapsed'] / k:.4f} ms/step")
+        cfg_block = {"workload": "column_collapse_32k", "grains": n, "integrator": "semi_implicit_euler",
+                     "friction": 0.5, "tangential_model": "spring_history", "dt": 1e-3,
+                     "cic_nodes": "64^3", "cic_every": 40, "energy_every": 1, "parallelism": "single_gpu"}
+        cfg_block.update(window_config(main_run, 40))
+        out = {
+            "metric": "grain-steps/s (32,768-grain column collapse, SIE, mu 0.5, energy every step, "
+                      "64^3 CIC every 40 steps)",
+            "value": value,
+            "unit": "grain-steps/s",
+            "n_gpus": 1,
+            "steps": k,
+            "warmup": args.warmup,
+            "ms_per_step": 1e3 * main_run["elapsed"] / k,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32",
+            "data": "synthetic (seeded jittered lattice)",
+            "config": cfg_block,
+            "launch": "eager" if args.eager else "graphs",
+            "roofline": roofline_of(main_run, "column_collapse_32k"),
+            "step_roofline": step_roofline(main_run, 64 ** 3, 40, value),
+            "kernels_ms": main_run["kernels_ms"],
+            "stable": main_run["stable"],
+            "overflow_cells": main_run["overflow"],
+        }
     if "runs32k" in legs:
         out["column_32k_runs"] = runs32k_leg(args.runs32k_steps)
+        progress("column_32k_runs")
     if "large" in legs:
-        out["column_1m"] = large_leg(k, args.warmup, pin_of(args.pin_1m), args.large_full_steps)
+        out["column_1m"] = large_leg(k, args.warmup, pin_of(args.pin_1m), args.large_full_steps,
+                                     args.eager)
+        progress("column_1m")
     if "ensemble" in legs:
         out["uneven_ensemble_64"] = ensemble_leg(args.ensemble_steps)
+        progress("uneven_ensemble_64")
     if "adjoint" in legs:
         out["adjoint"] = adjoint_leg()
+        progress("adjoint")
     if "batched_adjoint" in legs:
         out["batched_adjoint"] = batched_adjoint_leg()
+        progress("batched_adjoint")
     if "sliding" in legs:
         out["sliding"] = sliding_leg()
-    if "cpu" in legs:
+        progress("sliding")
+    if "cpu" in legs and main_run is not None:
         out["cpu_baseline"] = cpu_baseline(cfg, main_run, args.cpu_seconds)
+        progress("cpu_baseline")
+    progress("done")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -317,6 +317,14 @@ int wd_get_force_variant(const wd_sim* sim, int* block_out, int* lanes_out); /* 
 int wd_set_force_block(wd_sim* sim, int block); /* 0: autotune, or 64 / 128 / 256 with lanes 0 */
 int wd_get_force_block(const wd_sim* sim, int* block_out); /* 0 until tuned */
 
+/* The cell scan's tile (cells per tile of the single-pass scan of the cell
+ * counts): 1,024, 2,048 or 4,096, tuned online on the same steps as the
+ * force pass's variant (the cell starts do not depend on it); 0 returns it
+ * to the tuner.  The reference scans nothing: it compacts occupied cells
+ * with atomics (compact_cells_kernel, src/find_neighbors_kernel.cu:61-70). */
+int wd_set_scan_tile(wd_sim* sim, int cells);
+int wd_get_scan_tile(const wd_sim* sim, int* cells_out);
+
 /* Completed force passes and the contacts (wall and pair, counted from each
  * side) summed over them, since construction: contacts / (passes * n) is the
  * mean contact count per grain that the force pass's history traffic scales

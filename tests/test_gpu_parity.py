@@ -108,7 +108,7 @@ def test_graph_replay_equals_eager_launches(col4k):
     eager = col4k.copy()
     eager.kernel_timing = 1
     with ParticleDynamics(col4k) as a, ParticleDynamics(eager) as b:
-        a.take_step(37)  # 18 eager tuning steps, one 16-step graph chunk, 3 single-step replays
+        a.take_step(37)  # 18 eager tuning steps, then the 16-, 2- and 1-step graphs
         for _ in range(37):
             b.take_step()
         np.testing.assert_array_equal(a.state(), b.state())
@@ -401,11 +401,12 @@ def test_one_step_calls_are_tuned():
         np.testing.assert_array_equal(one.state(), fixed.state())
 
 
-def test_wide_box_scan_matches_oracle(oracle_lib):
+@pytest.mark.parametrize("tile", [1024, 2048, 4096])
+def test_wide_box_scan_matches_oracle(oracle_lib, tile):
     """2,048 grains scattered over a 4 x 4 m box: the occupied box spans
-    ~12 M cells, so the cell scan has more tiles (2,048 cells each) than its
-    grid has blocks (1,024), and blocks take further tiles from its counter.
-    Cell order and five steps equal the oracle's."""
+    ~12 M cells, so the cell scan has more tiles than its grid has blocks
+    (1,024) at every tile size, and blocks take further tiles from its
+    counter.  Cell order and five steps equal the oracle's."""
     c = workloads.column(16, 16, 8, half_width=2.0, ceiling=1.0)
     st, mat = _capi.initial_state(c)
     st = st.reshape(-1, 6).copy()
@@ -418,13 +419,39 @@ def test_wide_box_scan_matches_oracle(oracle_lib):
     span = [int(np.floor((st[:, a].max() - st[:, a].min()) / d)) + 3 for a in range(3)]
     assert span[0] * span[1] * span[2] > 1024 * 2048
     with ParticleDynamics(c) as sim:
+        sim.set_scan_tile(tile)
         sim.set_state(st)
         ids, keys = sim.cell_order()
         sim.take_step(5)
         got = sim.state()
+        assert sim.scan_tile() == tile
     oid, okeys, _ = oracle_lib.cell_order(c, st)
     np.testing.assert_array_equal(ids, oid)
     np.testing.assert_array_equal(keys.astype(np.int64), okeys)
     want, over, _ = oracle_lib.steps(c, st, mat, 5, threads=8)
     assert over == 0
     np.testing.assert_array_equal(got, want)
+
+
+def test_scan_tile_is_tuned_and_keeps_results():
+    """The online tuner times the three scan tiles on its tuning steps and
+    keeps one; every tile, pinned, gives the same steps bit for bit."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, _ = jittered(c, seed=17)
+    out = []
+    for tile in (1024, 2048, 4096):
+        with ParticleDynamics(c) as sim:
+            sim.set_scan_tile(tile)
+            sim.set_state(st)
+            sim.take_step(40)
+            out.append(sim.state())
+            assert sim.scan_tile() == tile
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
+    with ParticleDynamics(c) as tuned:
+        tuned.set_state(st)
+        tuned.take_step(40)  # 18 tuning steps time every force variant and scan tile
+        assert tuned.scan_tile() in (1024, 2048, 4096)
+        np.testing.assert_array_equal(tuned.state(), out[0])
+        with pytest.raises(_capi.WdError, match="cells must be"):
+            tuned.set_scan_tile(3000)

@@ -128,6 +128,8 @@ _SIGNATURES = [
     ("wd_stage_ms", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_cell_order", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("wd_cell_order_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("wd_set_scan_tile", C.c_int, [C.c_void_p, C.c_int]),
+    ("wd_get_scan_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_get_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

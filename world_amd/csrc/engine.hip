@@ -74,7 +74,8 @@ struct wd_sim {
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
     int* tmp_id = nullptr;
     unsigned long long* status = nullptr;
-    size_t scan_cells = 0, ntiles_max = 0;
+    size_t scan_cells = 0, ntiles_max = 0;  // cells allocated (whole largest tiles); most tiles
+    size_t scan_cells_used = 0;              // cells + 1 of the largest possible sub-grids
     float* st6 = nullptr;
     int* mat_d = nullptr;  // original-order (member << 4 | material) codes
     // ensemble members (1 for a single simulation)
@@ -95,12 +96,15 @@ struct wd_sim {
     double* energy_log = nullptr;
     double* ewave = nullptr;      // per-kEnergyGroup-grain energy sums of the last energy pass
     int force_var = -1;           // index into kForceVariants; -1 until the autotuner has chosen
+    int scan_var = 1;             // index into kScanItemsVariants (2,048-cell tiles until tuned)
+    bool scan_fixed = false;      // set by wd_set_scan_tile
     bool force_fixed = false;     // set by wd_set_force_block / wd_set_force_variant
     long long steps_run = 0;      // steps taken by this handle
     int* bin_box_out = nullptr;   // where k_count copies the boxes it bins with (adjoint recompute)
     uint32_t* bin_cell_out = nullptr;
     long long next_tune = 0;      // step count at which the autotuner runs (again)
     double tune_ms[sizeof(kForceVariants) / sizeof(kForceVariants[0])] = {};  // last tuning's time per variant (force pass + its reduction)
+    double scan_ms[3] = {};       // last tuning's cell-scan time per tile size
     int energy_cap = 0;
     bool energy_on = false;
     unsigned long long* cic_acc = nullptr;
@@ -114,8 +118,12 @@ struct wd_sim {
         hipEvent_t a, b;
     };
     std::vector<ProfRec>* prof = nullptr;  // set while wd_profile_step runs
-    hipGraphExec_t graph_one = nullptr, graph_chunk = nullptr;
-    int chunk = 16;
+    // Step graphs of 1, 2, 4, 8 and 16 steps: a call of k steps replays
+    // k / 16 sixteen-step graphs and one graph per set bit of k mod 16, so a
+    // short call costs at most five launches however it is split.
+    static constexpr int kGraphs = 5;
+    hipGraphExec_t graph_pow[kGraphs] = {};
+    hipGraphExec_t graph_cic_log = nullptr;  // one logged deposit (captured with the log buffer)
     bool timing = false;
     float stage_ms[4] = {0.f, 0.f, 0.f, 0.f};
     std::vector<TimedStage> pending;
@@ -186,9 +194,12 @@ struct wd_sim {
 
     void release() {
         drop_adjoint();
-        if (graph_one) (void)hipGraphExecDestroy(graph_one);
-        if (graph_chunk) (void)hipGraphExecDestroy(graph_chunk);
-        graph_one = graph_chunk = nullptr;
+        for (auto& g : graph_pow) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+        if (graph_cic_log) (void)hipGraphExecDestroy(graph_cic_log);
+        graph_cic_log = nullptr;
         for (auto& t : pending) {
             (void)hipEventDestroy(t.a);
             (void)hipEventDestroy(t.b);
@@ -281,11 +292,19 @@ struct wd_sim {
         // blocks.
         const int bbs = 64;
         const int gb = grain_blocks(n, bbs);
-        const int scan_grid = static_cast<int>(std::min<size_t>(ntiles_max, 1024));
         go("k_count", k_count, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tv, key,
            slot, count, n, bin_box_out, bin_cell_out);
-        go("k_scan", k_scan, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
-           status);
+        // Tiles of the chosen size covering every cell the handle can hold,
+        // on up to 1,024 blocks; blocks past the current box's tiles return.
+        const size_t tile = static_cast<size_t>(kScanThreads) * kScanItemsVariants[scan_var];
+        const int scan_grid = static_cast<int>(std::min<size_t>((scan_cells_used + tile - 1) / tile, 1024));
+        auto scan = [&](auto kern) {
+            go("k_scan", kern, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
+               status);
+        };
+        if (scan_var == 0) scan(k_scan<4>);
+        else if (scan_var == 1) scan(k_scan<8>);
+        else scan(k_scan<16>);
         go("k_scatter", k_scatter, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
            (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
         // Picard iterations after the first carry the spring index of the
@@ -366,10 +385,13 @@ struct wd_sim {
            (const float4*)Tv, (const double*)ewave, energy_log, flags);
     }
 
-    // CIC deposit of the current state into `out` (4 * nodes floats), queued.
-    void enqueue_cic(float* out) {
+    // CIC deposit of the current state into `out` (4 * nodes floats), queued;
+    // with `log` set, into the density log's next slot, counted on the
+    // device (so that one captured graph serves every record).
+    void enqueue_cic(float* out, bool log = false) {
         const int zb = static_cast<int>(std::min<size_t>((cic_count + kBlock - 1) / kBlock, 4096));
-        go("k_cic_zero", k_cic_zero, dim3(zb), dim3(kBlock), cic_acc, cic_count);
+        unsigned* const next = log ? &ctl->cic_log_next : nullptr;
+        go("k_cic_zero", k_cic_zero, dim3(zb), dim3(kBlock), cic_acc, cic_count, next);
         if (n > 0) {
             const int sb = (n + kBlock - 1) / kBlock;
             go("k_cic_scale", k_cic_scale, dim3(sb), dim3(kBlock), (const Params*)dp, parts,
@@ -379,7 +401,7 @@ struct wd_sim {
                (const float4*)Tp, (const float4*)Tv, cic_acc, n);
         }
         go("k_cic_convert", k_cic_convert, dim3(zb), dim3(kBlock), (const Params*)dp,
-           (const unsigned long long*)cic_acc, out, cic_count);
+           (const unsigned long long*)cic_acc, out, cic_count, (const unsigned*)next);
     }
 
     void enqueue_step() {
@@ -412,9 +434,10 @@ struct wd_sim {
     }
 
     void drop_graphs() {
-        if (graph_one) (void)hipGraphExecDestroy(graph_one);
-        if (graph_chunk) (void)hipGraphExecDestroy(graph_chunk);
-        graph_one = graph_chunk = nullptr;
+        for (auto& g : graph_pow) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
 
     int capture(int steps, hipGraphExec_t* out) {
@@ -446,8 +469,11 @@ struct wd_sim {
         int rounds = 0;
         bool first = false;
         int before = -1;     // variant the graphs were captured with
+        int scan_before = 1;
+        int scans = 0;       // tuning steps taken (each also times one scan tile size)
         bool cand[kCand] = {};
         double best[kCand] = {};
+        double scan_best[3] = {};
     } tuning;
 
     void begin_tuning() {
@@ -464,6 +490,9 @@ struct wd_sim {
             t.best[c] = 1e30;
         }
         t.before = force_var;
+        t.scan_before = scan_var;
+        t.scans = 0;
+        for (int v = 0; v < 3; ++v) t.scan_best[v] = 1e30;
         t.pos = 0;
     }
 
@@ -480,22 +509,32 @@ struct wd_sim {
             ++t.pos;
             if (!t.cand[c]) continue;
             force_var = c;
+            // The cell scan's tile size is tuned on the same steps, cycling
+            // independently of the force variant (its time does not depend
+            // on it); its first round warms up as the force pass's does.
+            const int sv = t.scans % 3;
+            const bool scan_timed = !scan_fixed && (t.scans >= 3 || !t.first);
+            if (!scan_fixed) scan_var = sv;
+            ++t.scans;
             std::vector<ProfRec> recs;
             prof = &recs;
             enqueue_step();
             prof = nullptr;
             WD_HIP(hipStreamSynchronize(stream));
-            double ms = 0.0;
+            double ms = 0.0, scan = 0.0;
             for (auto& rec : recs) {
                 float e = 0.f;
+                const bool ok = hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess;
                 // the force pass and the reduction of its block partials
-                const bool counted = std::strcmp(rec.name, "k_force") == 0 ||
-                                     std::strcmp(rec.name, "k_finalize") == 0;
-                if (counted && hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) ms += e;
+                if (ok && (std::strcmp(rec.name, "k_force") == 0 || std::strcmp(rec.name, "k_finalize") == 0)) {
+                    ms += e;
+                }
+                if (ok && std::strcmp(rec.name, "k_scan") == 0) scan += e;
                 spare.push_back(rec.a);
                 spare.push_back(rec.b);
             }
             if (r > 0 || !t.first) t.best[c] = std::min(t.best[c], ms);
+            if (scan_timed) t.scan_best[sv] = std::min(t.scan_best[sv], scan);
             time += hp.dt;
             --nsteps;
             ++steps_run;
@@ -508,8 +547,17 @@ struct wd_sim {
         }
         for (int c = 0; c < kCand; ++c) tune_ms[c] = t.best[c];
         force_var = best;
+        if (!scan_fixed) {
+            int sb = t.scan_before;
+            for (int v = 0; v < 3; ++v) {
+                if (t.scan_best[v] < 1e30 && t.scan_best[v] < t.scan_best[sb]) sb = v;
+            }
+            for (int v = 0; v < 3; ++v) scan_ms[v] = t.scan_best[v];
+            scan_var = sb;
+        }
         next_tune = next_tune == 0 ? 256 : 8 * next_tune;
-        if (best != t.before) drop_graphs();  // captured with another variant
+        // graphs captured with another variant or tile size
+        if (best != t.before || scan_var != t.scan_before) drop_graphs();
         t.pos = -1;
         return check_device();
     }
@@ -523,18 +571,30 @@ struct wd_sim {
             if (timing) {
                 for (int s = 0; s < nsteps; ++s) enqueue_step();
             } else {
-                if (!graph_one) {
-                    int rc = capture(1, &graph_one);
-                    if (rc != WD_OK) return rc;
-                    rc = capture(chunk, &graph_chunk);
-                    if (rc != WD_OK) return rc;
-                }
+                // graphs are captured the first time a call needs them
+                auto graph = [&](int b, hipGraphExec_t* out) -> int {
+                    if (!graph_pow[b]) {
+                        const int rc = capture(1 << b, &graph_pow[b]);
+                        if (rc != WD_OK) return rc;
+                    }
+                    *out = graph_pow[b];
+                    return WD_OK;
+                };
+                hipGraphExec_t g = nullptr;
                 int left = nsteps;
-                while (left >= chunk) {
-                    WD_HIP(hipGraphLaunch(graph_chunk, stream));
-                    left -= chunk;
+                if (left >= (1 << (kGraphs - 1))) {
+                    int rc = graph(kGraphs - 1, &g);
+                    if (rc != WD_OK) return rc;
+                    for (; left >= (1 << (kGraphs - 1)); left -= 1 << (kGraphs - 1)) {
+                        WD_HIP(hipGraphLaunch(g, stream));
+                    }
                 }
-                while (left-- > 0) WD_HIP(hipGraphLaunch(graph_one, stream));
+                for (int b = kGraphs - 2; b >= 0; --b) {
+                    if (!(left & (1 << b))) continue;
+                    int rc = graph(b, &g);
+                    if (rc != WD_OK) return rc;
+                    WD_HIP(hipGraphLaunch(g, stream));
+                }
             }
             WD_HIP(hipGetLastError());
         }
@@ -748,8 +808,9 @@ int allocate(wd_sim* s) {
     const size_t cells = static_cast<size_t>(c.collision_grid_size_x + 2) *
                          (c.collision_grid_size_y + 2) * (c.collision_grid_size_z + 2) *
                          static_cast<size_t>(s->members);
-    s->ntiles_max = (cells + 1 + kScanTile - 1) / kScanTile;
-    s->scan_cells = s->ntiles_max * kScanTile;
+    s->scan_cells_used = cells + 1;
+    s->ntiles_max = (cells + 1 + kScanTileMin - 1) / kScanTileMin;
+    s->scan_cells = (cells + 1 + kScanTileMax - 1) / kScanTileMax * kScanTileMax;
     WD_TRY(dalloc(&s->count, s->scan_cells));
     WD_TRY(dalloc(&s->start, s->scan_cells));
     WD_TRY(dalloc(&s->status, s->ntiles_max));
@@ -1270,6 +1331,11 @@ extern "C" int wd_density_log_enable(wd_sim* s, int capacity) {
     if (capacity < 0) return fail(WD_ERR_ARG, "wd_density_log_enable: negative capacity");
     int rc = s->sync();
     if (rc != WD_OK) return rc;
+    WD_HIP(hipStreamSynchronize(s->stream));
+    if (s->graph_cic_log) (void)hipGraphExecDestroy(s->graph_cic_log);  // captured with the old buffer
+    s->graph_cic_log = nullptr;
+    const unsigned zero = 0u;
+    WD_HIP(hipMemcpy(&s->ctl->cic_log_next, &zero, sizeof(zero), hipMemcpyHostToDevice));
     if (s->cic_log) {
         (void)hipFree(s->cic_log);
         s->cic_log = nullptr;
@@ -1283,7 +1349,26 @@ extern "C" int wd_density_log_enable(wd_sim* s, int capacity) {
 extern "C" int wd_density_log_record(wd_sim* s) {
     WD_CHECK_SIM(s);
     if (s->cic_log_count >= s->cic_log_cap) return fail(WD_ERR_STATE, "wd_density_log_record: log full");
-    s->enqueue_cic(s->cic_log + s->cic_count * static_cast<size_t>(s->cic_log_count));
+    // The slot is taken on the device (ctl->cic_log_next, which equals
+    // cic_log_count here), so the deposit's five launches replay from one
+    // graph captured at the first record.
+    if (s->timing || s->n == 0) {
+        s->enqueue_cic(s->cic_log, true);
+    } else {
+        if (!s->graph_cic_log) {
+            hipGraph_t g = nullptr;
+            WD_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            s->enqueue_cic(s->cic_log, true);
+            WD_HIP(hipStreamEndCapture(s->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&s->graph_cic_log, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->graph_cic_log = nullptr;
+                return fail(WD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+        }
+        WD_HIP(hipGraphLaunch(s->graph_cic_log, s->stream));
+    }
     WD_HIP(hipGetLastError());
     s->cic_log_count++;
     return WD_OK;
@@ -1909,6 +1994,31 @@ extern "C" int wd_set_force_variant(wd_sim* s, int block, int lanes) {
     s->tuning.pos = -1;
     for (double& t : s->tune_ms) t = 0.0;
     s->drop_graphs();
+    return WD_OK;
+}
+
+extern "C" int wd_set_scan_tile(wd_sim* s, int cells) {
+    WD_CHECK_SIM(s);
+    int v = -1;
+    for (int k = 0; k < 3; ++k) {
+        if (kScanThreads * kScanItemsVariants[k] == cells) v = k;
+    }
+    if (cells != 0 && v < 0) {
+        return fail(WD_ERR_ARG, "wd_set_scan_tile: cells must be 0 (autotune), 1024, 2048 or 4096");
+    }
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->scan_fixed = v >= 0;
+    if (v >= 0 && v != s->scan_var) {
+        s->scan_var = v;
+        s->drop_graphs();
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_get_scan_tile(const wd_sim* s, int* cells) {
+    WD_CHECK_SIM(s);
+    if (!cells) return fail(WD_ERR_ARG, "wd_get_scan_tile: null output");
+    *cells = kScanThreads * kScanItemsVariants[s->scan_var];
     return WD_OK;
 }
 

@@ -10,8 +10,11 @@ namespace wd {
 
 constexpr int kBlock = 256;          // threads per block of the per-grain kernels
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;  // cells per scan tile
+// Cells per thread of the cell scan: tiles of 1,024, 2,048 or 4,096 cells,
+// chosen by the online tuner (the start offsets do not depend on it).
+constexpr int kScanItemsVariants[3] = {4, 8, 16};
+constexpr int kScanTileMin = kScanThreads * 4;
+constexpr int kScanTileMax = kScanThreads * 16;
 constexpr int kXcds = 8;             // MI355X accelerator complex dies
 constexpr int kFinalizeBlock = 1024;  // one block; one wave per ensemble member at a time
 constexpr int kContactList = 24;     // touching candidates buffered per grain in LDS
@@ -33,6 +36,7 @@ struct Ctl {
     double energy_last;        // last standalone energy
     unsigned long long force_passes;  // completed force passes (k_finalize)
     unsigned long long contact_sum;   // contacts (walls + pairs) summed over those passes
+    unsigned cic_log_next;            // density-log slots taken (k_cic_zero of a logged deposit)
 };
 
 // Per-block partial results of a grain pass, reduced by k_finalize, in two

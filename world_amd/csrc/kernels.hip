@@ -567,10 +567,14 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
 // the grid is claimed by a running block, which reaches it once its current
 // tile (waiting only on lower ones) is done.  So every wait ends, without
 // co-residency limits.
+template <int ITEMS>
 __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict__ Pp, Ctl* ctl,
                                                        uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ start,
                                                        unsigned long long* __restrict__ status) {
+    static_assert(ITEMS % 4 == 0, "tiles are loaded as uint4");
+    constexpr int kScanItems = ITEMS;
+    constexpr int kScanTile = kScanThreads * ITEMS;
     const int cells = static_cast<int>(Pp->mcell[Pp->members]);
     const int ntiles = (cells + 1 + kScanTile - 1) / kScanTile;
     const unsigned long long tag = static_cast<unsigned long long>(ctl->epoch & 0x3FFFFFFFu) << 34;
@@ -1970,6 +1974,9 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     template __global__ void k_reorder<M>(const Params*, const uint32_t*, const uint32_t*,       \
                                           const int*, const float4*, const float4*, float4*,     \
                                           float4*, Carry, int);
+template __global__ void k_scan<4>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
+template __global__ void k_scan<8>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
+template __global__ void k_scan<16>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
 WD_INST_REORDER(0)
 WD_INST_REORDER(1)
 WD_INST_REORDER(2)
@@ -2096,8 +2103,11 @@ __global__ __launch_bounds__(kBlock) void k_export_history(const float4* __restr
 // chosen from the largest per-grain magnitude so no node sum can overflow;
 // integer sums do not depend on the order of the atomics, so the grid is
 // bitwise reproducible.
+// log_next (null unless the deposit goes to the density log): the number of
+// log slots taken, which this deposit's k_cic_convert reads back.
 __global__ __launch_bounds__(kBlock) void k_cic_zero(unsigned long long* __restrict__ acc,
-                                                     size_t count) {
+                                                     size_t count, unsigned* __restrict__ log_next) {
+    if (log_next && blockIdx.x == 0 && threadIdx.x == 0) *log_next = *log_next + 1u;
     for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
          k += static_cast<size_t>(gridDim.x) * blockDim.x) {
         acc[k] = 0ull;
@@ -2262,10 +2272,14 @@ __global__ __launch_bounds__(kBlock) void k_cic_deposit(const Params* __restrict
     }
 }
 
+// log_next (null for a plain deposit into `out`): the deposit goes to log
+// slot *log_next - 1, `count` floats apart.
 __global__ __launch_bounds__(kBlock) void k_cic_convert(const Params* __restrict__ Pp,
                                                         const unsigned long long* __restrict__ acc,
-                                                        float* __restrict__ out, size_t count) {
+                                                        float* __restrict__ out, size_t count,
+                                                        const unsigned* __restrict__ log_next) {
     const Params& P = *Pp;
+    if (log_next) out += static_cast<size_t>(*log_next - 1u) * count;
     const size_t per = static_cast<size_t>(4) * P.dgx * P.dgy * P.dgz;
     for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
          k += static_cast<size_t>(gridDim.x) * blockDim.x) {

@@ -261,6 +261,16 @@ class ParticleDynamics:
         check(lib().wd_get_force_variant(self._h, C.byref(b), C.byref(l)))
         return b.value, l.value
 
+    def set_scan_tile(self, cells: int) -> None:
+        """0: tuned online with the force variant; 1024 / 2048 / 4096 cells
+        per tile of the cell scan: fixed."""
+        check(lib().wd_set_scan_tile(self._h, int(cells)))
+
+    def scan_tile(self) -> int:
+        v = C.c_int(0)
+        check(lib().wd_get_scan_tile(self._h, C.byref(v)))
+        return v.value
+
     def contact_stats(self) -> tuple[int, int]:
         """(force passes, contacts summed over them) since construction."""
         p, c = C.c_ulonglong(0), C.c_ulonglong(0)
