@@ -137,12 +137,14 @@ def advance(sim, steps: int, chunk: int = 64) -> None:
 
 def prepare(sim, s0: int, pin=None) -> tuple[int, int]:
     """Advances untimed to step s0 and leaves the force pass's variant fixed
-    and the step graphs captured: one online tuning round ends CAPTURE_STEPS
-    before s0, its choice is pinned, and the last CAPTURE_STEPS steps capture
-    the graphs the window replays."""
+    and the step and deposit graphs captured: one online tuning round ends
+    CAPTURE_STEPS before s0, its choice is pinned, the last CAPTURE_STEPS
+    steps capture the step graphs the window replays, and one deposit into
+    the density log's spare slot captures the deposit's graph."""
     if pin:
         sim.set_force_variant(*pin)
         advance(sim, s0)
+        sim.record_density_grid()  # captures the deposit's graph (the log's spare slot)
         sim.sync()
         return tuple(pin)
     pre = s0 - TUNE_STEPS - CAPTURE_STEPS
@@ -154,6 +156,7 @@ def prepare(sim, s0: int, pin=None) -> tuple[int, int]:
     v = sim.force_variant()
     sim.set_force_variant(*v)
     sim.take_step(CAPTURE_STEPS)
+    sim.record_density_grid()  # captures the deposit's graph (the log's spare slot)
     sim.sync()
     return v
 
