@@ -325,6 +325,24 @@ int wd_get_force_block(const wd_sim* sim, int* block_out); /* 0 until tuned */
 int wd_set_scan_tile(wd_sim* sim, int cells);
 int wd_get_scan_tile(const wd_sim* sim, int* cells_out);
 
+/* Extension, opt-in: collision cells coarser than the configured grid (the
+ * reference sizes cells from the config, collision_grid_size_*,
+ * src/sim_config.h:64-200).  Every axis gets floor(G / scale) cells, so a
+ * cell is at least `scale` times the configured one; the neighbour search
+ * still finds every contact (cells only grow) but visits them in another
+ * order, so the FP32 force sums differ from the configured grid's in the
+ * last bits: results then match the reference's FP32 arithmetic on the
+ * coarser grid bit for bit, and the forces of the configured grid within
+ * the FP32 rounding of their sums (1e-5 of the summed term magnitudes).
+ * A coarser cell holds more grains: particles_per_cell may then be
+ * exceeded.  wd_tune_cell_scale times `steps` real steps on each of the
+ * scales 1, 1.25 and 1.5 (the first of each warming up) and keeps the
+ * fastest whose steps overflowed no cell; it advances the simulation by
+ * 3 * steps steps.  Scale 1 (the default) is the configured grid. */
+int wd_set_cell_scale(wd_sim* sim, float scale);
+int wd_get_cell_scale(const wd_sim* sim, float* scale_out, int* grid_out /* 3 ints, may be null */);
+int wd_tune_cell_scale(wd_sim* sim, int steps, float* chosen_out);
+
 /* Completed force passes and the contacts (wall and pair, counted from each
  * side) summed over them, since construction: contacts / (passes * n) is the
  * mean contact count per grain that the force pass's history traffic scales

@@ -455,3 +455,89 @@ def test_scan_tile_is_tuned_and_keeps_results():
         np.testing.assert_array_equal(tuned.state(), out[0])
         with pytest.raises(_capi.WdError, match="cells must be"):
             tuned.set_scan_tile(3000)
+
+
+def coarse_cfg(c, scale):
+    """c with the collision grid the cell scale gives (floor(G / scale) cells
+    per axis)."""
+    import math
+
+    out = c.copy()
+    out.collision_grid_size_x = max(1, math.floor(c.collision_grid_size_x / scale))
+    out.collision_grid_size_y = max(1, math.floor(c.collision_grid_size_y / scale))
+    out.collision_grid_size_z = max(1, math.floor(c.collision_grid_size_z / scale))
+    return out
+
+
+@pytest.mark.parametrize("scale", [1.25, 1.5])
+def test_cell_scale_is_the_oracle_on_the_coarser_grid(oracle_lib, scale):
+    """Coarser cells (opt-in): cell order and 30 steps with spring history
+    equal the oracle run on the coarser grid bit for bit."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, mat = jittered(c, sigma_x=0.002, sigma_v=0.4, seed=23)
+    cc = coarse_cfg(c, scale)
+    with ParticleDynamics(c) as sim:
+        sim.set_state(st)
+        sim.set_cell_scale(scale)
+        got_scale, grid = sim.cell_scale()
+        assert got_scale == pytest.approx(scale)
+        assert grid == (cc.collision_grid_size_x, cc.collision_grid_size_y, cc.collision_grid_size_z)
+        ids, keys = sim.cell_order()
+        sim.take_step(30)
+        got = sim.state()
+        hist = sim.history()
+    oid, okeys, _ = oracle_lib.cell_order(cc, st)
+    np.testing.assert_array_equal(ids, oid)
+    np.testing.assert_array_equal(keys.astype(np.int64), okeys)
+    o = oracle_lib.Sim(cc, st, mat)
+    o.steps(30, 8)
+    np.testing.assert_array_equal(got, o.state())
+    assert_springs_equal(hist, o.history())
+
+
+def test_cell_scale_forces_within_fp64_of_the_configured_grid(oracle_lib):
+    """The same contacts summed in another order: the forces of both grids
+    lie within 1e-5 of the FP64 evaluation on the configured grid."""
+    c = viscous_cfg(workloads.column_collapse_4k())
+    st, mat = jittered(c, sigma_x=0.0015, sigma_v=0.3, seed=29)
+    with ParticleDynamics(c) as fine, ParticleDynamics(c) as coarse:
+        fine.set_state(st)
+        coarse.set_state(st)
+        coarse.set_cell_scale(1.5)
+        f_fine = fine.rhs()
+        f_coarse = coarse.rhs()
+    r64, scale = oracle_lib.rhs(c, st, mat, 64)
+    for f in (f_fine, f_coarse):
+        err = np.linalg.norm(f[:, 3:].astype(np.float64) - r64[:, 3:], axis=1)
+        assert np.all(err <= 1e-5 * scale)
+
+
+@pytest.mark.parametrize("model", ["history", "viscous"])
+def test_tune_cell_scale_restores_the_state(model):
+    """The opt-in tuner times each candidate scale on steps from one
+    snapshot and restores it: the state, springs, clock and counters are
+    those before the call, and the steps after it equal steps on the chosen
+    grid set without tuning."""
+    c = workloads.column_collapse_32k(friction=0.5)
+    if model == "viscous":
+        c = viscous_cfg(c)
+    with ParticleDynamics(c) as sim, ParticleDynamics(c) as ref:
+        sim.take_step(300)
+        ref.take_step(300)
+        before = sim.state()
+        t0, over0, stats0 = sim.time, sim.grid_overflow_count(), sim.contact_stats()
+        chosen = sim.tune_cell_scale(6)
+        assert chosen in (1.0, 1.25, 1.5)
+        assert sim.cell_scale()[0] == pytest.approx(chosen)
+        np.testing.assert_array_equal(sim.state(), before)
+        assert sim.time == t0 and sim.grid_overflow_count() == over0 and sim.contact_stats() == stats0
+        ref.set_cell_scale(chosen)
+        sim.take_step(40)
+        ref.take_step(40)
+        np.testing.assert_array_equal(sim.state(), ref.state())
+        if model == "history":
+            assert_springs_equal(sim.history(), ref.history())
+        assert sim.is_stable()
+    with ParticleDynamics(c) as sim:
+        with pytest.raises(_capi.WdError, match="scale must be"):
+            sim.set_cell_scale(0.5)

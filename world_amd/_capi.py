@@ -129,6 +129,9 @@ _SIGNATURES = [
     ("wd_cell_order", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("wd_cell_order_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("wd_set_scan_tile", C.c_int, [C.c_void_p, C.c_int]),
+    ("wd_set_cell_scale", C.c_int, [C.c_void_p, C.c_float]),
+    ("wd_get_cell_scale", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    ("wd_tune_cell_scale", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
     ("wd_get_scan_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_snap_save(const Ctl* __restrict__ ct
     dP[id] = p;
     dV[id] = Tv[i];
     const int par = ctl->hist_parity;
-    const int c = hist.cnt[par][i];
+    const int c = hist.cnt[par] ? hist.cnt[par][i] : 0;  // no springs under the viscous model
     dC[id] = c;
     for (int t = 0; t < c; ++t) dH[static_cast<size_t>(t) * n + id] = hist.buf[par][static_cast<size_t>(t) * n + i];
 }
@@ -222,9 +222,11 @@ __global__ __launch_bounds__(kBlock) void k_snap_load(const Params* __restrict__
         Tv[i] = v;
         member = member_of(P, v.w);
         const int par = ctl->hist_parity;
-        const int c = sC[i];
-        hist.cnt[par][i] = c;
-        for (int t = 0; t < c; ++t) hist.buf[par][static_cast<size_t>(t) * n + i] = sH[static_cast<size_t>(t) * n + i];
+        if (hist.cnt[par]) {  // no springs under the viscous model
+            const int c = sC[i];
+            hist.cnt[par][i] = c;
+            for (int t = 0; t < c; ++t) hist.buf[par][static_cast<size_t>(t) * n + i] = sH[static_cast<size_t>(t) * n + i];
+        }
         int cx, cy, cz;
         grain_cell(P, p.x, p.y, p.z, cx, cy, cz);
         key[i] = pack_cell(cx, cy, cz);

@@ -59,6 +59,7 @@ struct wd_sim {
     wd_config cfg{};
     int n = 0;
     int device = 0;               // the HIP device current when the handle was created
+    float cell_scale = 1.f;       // collision cells this many times the configured ones (per axis, at least)
     float time = 0.f;
     hipStream_t stream = nullptr;
     Params hp{};
@@ -766,9 +767,23 @@ int upload_member_tables(wd_sim* s) {
 // Recomputes the derived constants from s->cfg (keeping the current dt and
 // every member table and buffer the handle owns) and uploads them; graphs
 // read them through dp.
+// The collision grid of a handle: the configured one, or coarsened so that
+// every cell is at least `scale` times the configured cell along each axis.
+void grid_dims(const wd_config& c, float scale, int g[3]) {
+    const int cfg[3] = {c.collision_grid_size_x, c.collision_grid_size_y, c.collision_grid_size_z};
+    for (int a = 0; a < 3; ++a) {
+        g[a] = scale <= 1.f ? cfg[a] : std::max(1, static_cast<int>(std::floor(cfg[a] / scale)));
+    }
+}
+
 int refresh_params(wd_sim* s) {
     Params keep = s->hp;
     fill_params(s->cfg, s->n, s->hp);
+    int g[3];
+    grid_dims(s->cfg, s->cell_scale, g);
+    s->hp.gx = g[0];
+    s->hp.gy = g[1];
+    s->hp.gz = g[2];
     s->hp.dt = keep.dt;
     s->hp.members = keep.members;
     s->hp.mgoff = keep.mgoff;
@@ -2020,6 +2035,148 @@ extern "C" int wd_get_scan_tile(const wd_sim* s, int* cells) {
     if (!cells) return fail(WD_ERR_ARG, "wd_get_scan_tile: null output");
     *cells = kScanThreads * kScanItemsVariants[s->scan_var];
     return WD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Collision cell size (extension): coarser cells than the configured grid.
+
+namespace {
+
+int apply_cell_scale(wd_sim* s, float scale) {
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->cell_scale = scale;
+    int rc = refresh_params(s);
+    if (rc != WD_OK || s->n == 0) return rc;
+    // the grains' cell keys and boxes under the new grid, for the next bin
+    const int gb = grain_blocks(s->n);
+    hipLaunchKernelGGL(k_rekey, dim3(gb), dim3(kBlock), 0, s->stream, (const Params*)s->dp, s->parts,
+                       (const float4*)s->Tp, (const float4*)s->Tv, s->key, s->n);
+    s->finalize(gb, kFinBox);
+    WD_HIP(hipGetLastError());
+    return s->sync();
+}
+
+}  // namespace
+
+extern "C" int wd_set_cell_scale(wd_sim* s, float scale) {
+    WD_CHECK_SIM(s);
+    if (!(scale >= 1.f && scale <= 4.f)) return fail(WD_ERR_ARG, "wd_set_cell_scale: scale must be in [1, 4]");
+    return apply_cell_scale(s, scale);
+}
+
+extern "C" int wd_get_cell_scale(const wd_sim* s, float* scale, int* grid) {
+    WD_CHECK_SIM(s);
+    if (scale) *scale = s->cell_scale;
+    if (grid) grid_dims(s->cfg, s->cell_scale, grid);
+    return WD_OK;
+}
+
+extern "C" int wd_tune_cell_scale(wd_sim* s, int steps, float* chosen) {
+    WD_CHECK_SIM(s);
+    if (steps < 2) return fail(WD_ERR_ARG, "wd_tune_cell_scale: at least 2 steps per candidate");
+    if (s->picard()) return fail(WD_ERR_STATE, "wd_tune_cell_scale: semi-implicit Euler only");
+    if (s->n == 0) {
+        if (chosen) *chosen = s->cell_scale;
+        return WD_OK;
+    }
+    // Every candidate runs from the same state, which is restored at the end
+    // (with the counters the trial steps moved): tuning leaves the
+    // simulation where it was, and a trial grid that overflows a cell drops
+    // neighbours only in steps that are thrown away.
+    const size_t n = static_cast<size_t>(s->n);
+    wd_sim::Snap sn;
+    std::vector<void*> bufs;
+    auto alloc = [&](void** p, size_t bytes) {
+        *p = nullptr;
+        const bool ok = hipMalloc(p, bytes) == hipSuccess;
+        if (ok) bufs.push_back(*p);
+        return ok;
+    };
+    auto release = [&]() {
+        for (void* p : bufs) (void)hipFree(p);
+    };
+    if (!alloc(reinterpret_cast<void**>(&sn.P), sizeof(float4) * n) ||
+        !alloc(reinterpret_cast<void**>(&sn.V), sizeof(float4) * n) ||
+        !alloc(reinterpret_cast<void**>(&sn.H), sizeof(float4) * kHist * n) ||
+        !alloc(reinterpret_cast<void**>(&sn.C), sizeof(int) * n)) {
+        release();
+        return fail(WD_ERR_HIP, "wd_tune_cell_scale: snapshot allocation failed");
+    }
+    WD_HIP(hipStreamSynchronize(s->stream));
+    Ctl ctl0;
+    std::vector<unsigned long long> movf0(static_cast<size_t>(s->members));
+    WD_HIP(hipMemcpy(&ctl0, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    WD_HIP(hipMemcpy(movf0.data(), s->d_movf, sizeof(unsigned long long) * movf0.size(), hipMemcpyDeviceToHost));
+    snap_save(s, sn);
+    const float scale0 = s->cell_scale;
+    const bool energy = s->energy_on;
+    s->energy_on = false;
+    const float cand[3] = {1.f, 1.25f, 1.5f};
+    double best_ms = 1e30;
+    float best = 1.f;
+    int rc = WD_OK;
+    for (float sc : cand) {
+        s->cell_scale = sc;
+        rc = refresh_params(s);
+        if (rc != WD_OK) break;
+        snap_load(s, sn);  // the snapshot's keys and boxes under this grid
+        unsigned long long over0 = 0, over1 = 0;
+        rc = read_overflow(s, &over0);
+        if (rc != WD_OK) break;
+        double total = 0.0;
+        for (int k = 0; k < steps && rc == WD_OK; ++k) {
+            std::vector<wd_sim::ProfRec> recs;
+            s->prof = &recs;
+            s->enqueue_step();
+            s->prof = nullptr;
+            if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(WD_ERR_HIP, "wd_tune_cell_scale: step failed");
+            double ms = 0.0;
+            for (auto& rec : recs) {
+                float e = 0.f;
+                if (hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess) ms += e;
+                s->spare.push_back(rec.a);
+                s->spare.push_back(rec.b);
+            }
+            if (k > 0) total += ms;  // the first step warms the grid up
+        }
+        if (rc != WD_OK) break;
+        rc = read_overflow(s, &over1);
+        if (rc != WD_OK) break;
+        const double mean = total / (steps - 1);
+        if ((over1 == over0 || sc == 1.f) && mean < best_ms) {
+            best_ms = mean;
+            best = sc;
+        }
+    }
+    s->energy_on = energy;
+    s->cell_scale = rc == WD_OK ? best : scale0;
+    int rc2 = refresh_params(s);
+    if (rc2 == WD_OK) {
+        snap_load(s, sn);
+        rc2 = s->sync();
+    }
+    if (rc2 == WD_OK) {
+        // counters as before the trials (the spring generation stays where
+        // the restored springs were written)
+        Ctl now;
+        rc2 = hipMemcpy(&now, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost) == hipSuccess ? WD_OK
+                : fail(WD_ERR_HIP, "wd_tune_cell_scale: counters");
+        if (rc2 == WD_OK) {
+            ctl0.hist_parity = now.hist_parity;
+            ctl0.epoch = now.epoch;
+            ctl0.tile_counter = now.tile_counter;
+            if (hipMemcpy(s->ctl, &ctl0, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s->d_movf, movf0.data(), sizeof(unsigned long long) * movf0.size(),
+                          hipMemcpyHostToDevice) != hipSuccess) {
+                rc2 = fail(WD_ERR_HIP, "wd_tune_cell_scale: counters");
+            }
+        }
+    }
+    release();
+    if (rc != WD_OK) return rc;
+    if (rc2 != WD_OK) return rc2;
+    if (chosen) *chosen = best;
+    return s->check_device();
 }
 
 extern "C" int wd_get_force_variant(const wd_sim* s, int* block, int* lanes) {

@@ -511,6 +511,30 @@ __global__ __launch_bounds__(kBlock) void k_export(const float4* __restrict__ Tp
     }
 }
 
+// Cell keys and block boxes of the current state under the current grid
+// (after the collision grid changes; the next k_finalize publishes the boxes).
+__global__ __launch_bounds__(kBlock) void k_rekey(const Params* __restrict__ Pp,
+                                                  BlockPart* __restrict__ parts,
+                                                  const float4* __restrict__ Tp,
+                                                  const float4* __restrict__ Tv,
+                                                  uint32_t* __restrict__ key, int n) {
+    const Params& P = *Pp;
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
+    int member = -1;
+    if (i < n) {
+        const float4 p = Tp[i];
+        member = member_of(P, Tv[i].w);
+        int cx, cy, cz;
+        grain_cell(P, p.x, p.y, p.z, cx, cy, cz);
+        key[i] = pack_cell(cx, cy, cz);
+        lo[0] = hi[0] = cx;
+        lo[1] = hi[1] = cy;
+        lo[2] = hi[2] = cz;
+    }
+    store_block_part(parts, P.members > 1 ? P.pmem : nullptr, logical_block(), member, lo, hi, 0u);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
 // box_out / cell_out (null unless a caller keeps the bin of this pass): a

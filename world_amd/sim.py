@@ -271,6 +271,24 @@ class ParticleDynamics:
         check(lib().wd_get_scan_tile(self._h, C.byref(v)))
         return v.value
 
+    def set_cell_scale(self, scale: float) -> None:
+        """Opt-in coarser collision cells (1: the configured grid)."""
+        check(lib().wd_set_cell_scale(self._h, C.c_float(scale)))
+
+    def cell_scale(self) -> tuple[float, tuple[int, int, int]]:
+        """The cell scale and the collision grid it gives."""
+        v = C.c_float(0)
+        g = np.zeros(3, np.int32)
+        check(lib().wd_get_cell_scale(self._h, C.byref(v), ptr(g)))
+        return v.value, (int(g[0]), int(g[1]), int(g[2]))
+
+    def tune_cell_scale(self, steps: int) -> float:
+        """Times `steps` real steps on each of the scales 1, 1.25, 1.5 and
+        keeps the fastest that overflowed no cell (advances 3 * steps)."""
+        v = C.c_float(0)
+        check(lib().wd_tune_cell_scale(self._h, int(steps), C.byref(v)))
+        return v.value
+
     def contact_stats(self) -> tuple[int, int]:
         """(force passes, contacts summed over them) since construction."""
         p, c = C.c_ulonglong(0), C.c_ulonglong(0)
