@@ -375,12 +375,22 @@ def ensemble_leg(steps: int) -> dict:
 
 def adjoint_window(make, cot_of, window_steps: int, every: int):
     """Forward, record and backward cost of one window, all over the same
-    steps: a handle advanced 20 + window steps times a forward step over the
-    next window; a second handle records and differentiates the window
-    before it (its allocations and graph captures) and then the same window.
-    Returns seconds (forward, record, backward) for the window."""
+    steps and all with the force variant tuned and pinned and the graphs
+    captured beforehand.  A handle tunes (18 steps), pins, captures the step
+    graphs (31 steps) and runs one window untimed, then times a forward step
+    over the next window; a second handle does the same and records and
+    differentiates its first window untimed (allocations, adjoint graph
+    captures) and then the same second window.  Returns seconds (forward,
+    record, backward) for the window."""
+    def settle(h):
+        h.take_step(TUNE_STEPS)
+        h.set_force_variant(*h.force_variant())
+        h.take_step(CAPTURE_STEPS)
+        h.sync()
+
     with make() as a:
-        a.take_step(20 + window_steps)
+        settle(a)
+        a.take_step(window_steps)
         a.sync()
         t = time.perf_counter()
         a.take_step(window_steps)
@@ -388,8 +398,7 @@ def adjoint_window(make, cot_of, window_steps: int, every: int):
         fwd = time.perf_counter() - t
     with make() as h:
         cot = cot_of(h)
-        h.take_step(20)
-        h.sync()
+        settle(h)
         rec = bwd = 0.0
         for _ in range(2):
             t = time.perf_counter()
