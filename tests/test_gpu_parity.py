@@ -282,6 +282,31 @@ def test_results_do_not_depend_on_the_force_variant(picard, model):
         assert got[3] == ref[3], v
 
 
+@pytest.mark.parametrize("nmat", [1, 3, 16])
+def test_materials_match_oracle_for_every_variant(oracle_lib, nmat):
+    """Grains of up to 16 materials (pair damping from each pair's reduced
+    mass): every force-pass variant gives the oracle's states and springs.
+    A lone simulation's damping table holds the handle's materials only."""
+    c = history_cfg(workloads.column_collapse_4k(friction=0.4))
+    c.set_masses([workloads.GRAIN_MASS * (1.0 + 0.25 * k) for k in range(nmat)])
+    c.particle_material = 0
+    st, mat = jittered(c, seed=21)
+    mat = (np.arange(len(mat)) * 7 % nmat).astype(mat.dtype)
+    ora = oracle_lib.Sim(c, st, mat)
+    ora.steps(12, threads=8)
+    want_state, want_hist = ora.state(), ora.history()
+    ora.close()
+    for v in FORCE_VARIANTS:
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.host_material[:] = mat
+            sim.device_material.copy_from_host(sim.host_material)
+            sim.set_force_variant(*v)
+            sim.take_step(12)
+            np.testing.assert_array_equal(sim.state(), want_state, err_msg=str(v))
+            assert_springs_equal(sim.history(), want_hist, str(v))
+
+
 @pytest.mark.parametrize("kcap", [8, 128])
 @pytest.mark.parametrize("model", ["history", "viscous"])
 def test_dense_cluster_matches_oracle_for_every_variant(oracle_lib, kcap, model):

@@ -276,12 +276,17 @@ struct wd_sim {
     // alone, without the dispatch gap around it.
     template <typename K, typename... A>
     void go(const char* name, K kernel, dim3 grid, dim3 block, A... args) {
+        go_lds(name, kernel, grid, block, 0u, args...);
+    }
+    // The same with `lds` bytes of dynamic LDS.
+    template <typename K, typename... A>
+    void go_lds(const char* name, K kernel, dim3 grid, dim3 block, unsigned lds, A... args) {
         if (!prof) {
-            hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+            hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
             return;
         }
         hipEvent_t a = take_event(), b = take_event();
-        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, a, b, 0, args...);
+        hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, a, b, 0, args...);
         prof->push_back({name, a, b});
     }
 
@@ -340,10 +345,14 @@ struct wd_sim {
     template <int MODE, bool ENERGY, int L>
     void launch_force_q(const float4* np, const float4* nv) {
         const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}, hidx[hidx_cur]};
+        // One lane per grain keeps the damping table of the handle's
+        // materials in dynamic LDS, read when there is no per-member table.
+        const unsigned lds = L == 1 && hp.mtab == nullptr
+                ? static_cast<unsigned>(sizeof(float) * hp.nmat * hp.nmat) : 0u;
         auto force = [&](auto kern) {
-            go("k_force", kern, dim3(grain_blocks(n, 64 / L)), dim3(64), (const Params*)dp, ctl,
-               (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, parts,
-               ewave, hio, n);
+            go_lds("k_force", kern, dim3(grain_blocks(n, 64 / L)), dim3(64), lds, (const Params*)dp, ctl,
+                   (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, parts,
+                   ewave, hio, n);
         };
         if (history()) force(k_force_q<MODE, ENERGY, true, L>);
         else force(k_force_q<MODE, ENERGY, false, L>);

@@ -117,6 +117,29 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask) {
                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
 }
 
+// Index of the k-th set bit (k from 0) of a 64-bit lane mask that has more
+// than k bits set: a binary search on population counts.
+__device__ __forceinline__ int nth_lane(unsigned long long m, int k) {
+    int pos = 0;
+    int c = __popc(static_cast<uint32_t>(m));
+    if (k >= c) {
+        k -= c;
+        m >>= 32;
+        pos = 32;
+    }
+    uint32_t x = static_cast<uint32_t>(m);
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        c = __popc(x & ((1u << w) - 1u));
+        if (k >= c) {
+            k -= c;
+            x >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 // True when a column's cell-start quad (starts of cells dz = -1, 0, +1 and of
 // the next cell) has a cell holding more than kcap grains.
 __device__ __forceinline__ bool column_capped(const uint4& q4, uint32_t kcap) {
@@ -1116,17 +1139,37 @@ __global__ __launch_bounds__(64) void k_force_q(
     const Params& P = *Pp;
     // Per lane: cell-sorted slots of listed contacts; with one lane per
     // grain, row QCAP takes the stores of candidates that are not listed.
-    __shared__ uint32_t s_list[L == 1 ? QCAP + 1 : QCAP][64];
+    // One lane per grain keeps its LDS small (under 7 KB, six waves per
+    // SIMD at the register count it takes without the next round's
+    // prefetch): the owner's spring slot and old spring list come by
+    // shuffles instead of LDS, an entry's owner lane is found from its row's
+    // ballot, a round's records are packed floats, the stores of unlisted
+    // candidates go to the record area (free while the stencil is walked),
+    // and the damping table holds the handle's materials only (dynamic LDS,
+    // nmat x nmat).
+    constexpr int NR = ENERGY ? 11 : 9;         // floats per record (L == 1)
+    __shared__ uint32_t s_list[QCAP][64];
     __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
-    __shared__ uint8_t s_src[QCAP * 64];        // owner lane of each queue entry
-    __shared__ int s_slot[64];                  // per lane: spring slot of its first listed contact
-    __shared__ float4 s_rec[3][64];             // per queue entry: force addends, energy addends
-    __shared__ int2 s_gh[G];                    // per grain: old spring list (index, count)
-    __shared__ float s_cpair[kMaxMat * kMaxMat];
+    __shared__ uint8_t s_src[L == 1 ? 1 : QCAP * 64];  // owner lane of each queue entry (L > 1)
+    __shared__ unsigned long long s_rowmask[L == 1 ? QCAP : 1];  // L == 1: lanes listing row j
+    __shared__ int s_rowstart[L == 1 ? QCAP : 1];  // L == 1: queue index of row j's first entry
+    __shared__ int s_slot[L == 1 ? 1 : 64];     // per lane: spring slot of its first listed contact
+    __shared__ float4 s_rec[L == 1 ? 1 : 3][L == 1 ? 1 : 64];  // per queue entry: force, energy addends
+    __shared__ float s_r[L == 1 ? NR : 1][64];  // the same, packed (L == 1)
+    __shared__ int2 s_gh[L == 1 ? 1 : G];       // per grain: old spring list (index, count)
+    __shared__ float s_cpair[L == 1 ? 1 : kMaxMat * kMaxMat];
+    extern __shared__ float s_cdyn[];           // L == 1: c_pair of the handle's nmat materials
+    static_assert(NR * 64 * sizeof(float) >= 64 * sizeof(uint32_t), "record area holds a spare list row");
+    uint32_t* const s_spare = reinterpret_cast<uint32_t*>(&s_r[0][0]);
     // A lone simulation's pair damping table, in LDS: one lookup per contact.
     const bool lds_tab = P.mtab == nullptr;
+    const int nmat = P.nmat;
     if (lds_tab) {
-        for (int k = threadIdx.x; k < kMaxMat * kMaxMat; k += 64) s_cpair[k] = P.c_pair[k];
+        if constexpr (L == 1) {
+            for (int k = threadIdx.x; k < nmat * nmat; k += 64) s_cdyn[k] = P.c_pair[(k / nmat) * kMaxMat + k % nmat];
+        } else {
+            for (int k = threadIdx.x; k < kMaxMat * kMaxMat; k += 64) s_cpair[k] = P.c_pair[k];
+        }
         __syncthreads();
     }
     const int lane = static_cast<int>(threadIdx.x);
@@ -1228,7 +1271,7 @@ __global__ __launch_bounds__(64) void k_force_q(
             }
         }
     }
-    if (sub == 0) s_gh[g] = make_int2(hs, h_old);
+    if (L > 1 && sub == 0) s_gh[g] = make_int2(hs, h_old);
 
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
     int cx = 0, cy = 0, cz = 0;
@@ -1262,7 +1305,7 @@ __global__ __launch_bounds__(64) void k_force_q(
             const bool other = valid & (q != static_cast<uint32_t>(p));
             const bool hit = other & (s2 < touch2);
             const uint32_t li = static_cast<uint32_t>(seen - cons);
-            s_list[hit && li < static_cast<uint32_t>(QCAP) ? li : static_cast<uint32_t>(QCAP)][lane] = q;
+            *(hit && li < static_cast<uint32_t>(QCAP) ? &s_list[li][lane] : &s_spare[lane]) = q;
             seen += hit ? 1 : 0;
             if (other & (s2 != s2)) {
                 nan_f = true;
@@ -1346,7 +1389,10 @@ __global__ __launch_bounds__(64) void k_force_q(
                 if (j < cnt) {
                     const int pos = T + lanes_below(mask);
                     s_flat[pos] = q | (static_cast<uint32_t>(j) << 28);
-                    s_src[pos] = static_cast<uint8_t>(lane);
+                }
+                if (lane == 0) {
+                    s_rowmask[j] = mask;
+                    s_rowstart[j] = T;
                 }
                 T += __popcll(mask);
             }
@@ -1363,7 +1409,8 @@ __global__ __launch_bounds__(64) void k_force_q(
                 if (j < cnt) s_src[q0 + j] = static_cast<uint8_t>(lane);
             }
         }
-        s_slot[lane] = lane_slot + cons;
+        if (L > 1) s_slot[lane] = lane_slot + cons;
+        const int slot_base = lane_slot + cons;  // L == 1: read by shuffles
         __syncthreads();
         // One queue entry's indices and loads; the owner grain's own state
         // comes from its lanes' registers by shuffles.  The next round's
@@ -1371,17 +1418,21 @@ __global__ __launch_bounds__(64) void k_force_q(
         // index past the queue fetches its last entry: every load is
         // unconditional.
         struct Entry {
-            int slot, og;
+            int slot, og, hs, hold;  // hs, hold: the owner's old spring list (L == 1)
             uint32_t q;
             float4 pj, vj, hg;
         };
         auto fetch = [&](int e, Entry& t) __attribute__((always_inline)) {
             if constexpr (L == 1) {
                 const uint32_t w = s_flat[e];
-                const int src = s_src[e];
+                const int j = static_cast<int>(w >> 28);
+                // the owner: the (e - start)-th lane of the row's ballot
+                const int src = nth_lane(s_rowmask[j], e - s_rowstart[j]);
                 t.q = w & 0x0FFFFFFFu;
-                t.slot = s_slot[src] + static_cast<int>(w >> 28);
+                t.slot = __shfl(slot_base, src, 64) + j;
                 t.og = src;
+                t.hs = HIST ? __shfl(hs, src, 64) : 0;
+                t.hold = HIST ? __shfl(h_old, src, 64) : 0;
             } else {
                 const int src = s_src[e];
                 const int j = e - s_q[src];
@@ -1391,9 +1442,11 @@ __global__ __launch_bounds__(64) void k_force_q(
             }
             t.pj = Sp[t.q];
             t.vj = Sv[t.q];
-            if (HIST) t.hg = h_in[static_cast<size_t>(min(t.slot, kHist - 1)) * n + s_gh[t.og].x];
+            if (HIST) t.hg = h_in[static_cast<size_t>(min(t.slot, kHist - 1)) * n + (L == 1 ? t.hs : s_gh[t.og].x)];
         };
-        constexpr bool kAhead = true;
+        // With one lane per grain the next round is not prefetched: its
+        // loads would hold 17 more registers and cost a wave per SIMD.
+        constexpr bool kAhead = L != 1;
         Entry cur, nxt;
         int jc = 0, cjc = 0;  // one lane per grain: first row in the round, its first queue index
         if (kAhead && T > 0) fetch(min(lane, T - 1), cur);
@@ -1431,8 +1484,9 @@ __global__ __launch_bounds__(64) void k_force_q(
                 r0.z = fp * nrm[2];
                 ++ncont;
                 const float4 vj = cur.vj;
-                const float c = lds_tab ? s_cpair[omat * kMaxMat + grain_mat(vj.w)]
-                                        : mat_cpair(P, omember, omat, grain_mat(vj.w));
+                const float c = !lds_tab ? mat_cpair(P, omember, omat, grain_mat(vj.w))
+                                : L == 1 ? s_cdyn[omat * nmat + grain_mat(vj.w)]
+                                         : s_cpair[omat * kMaxMat + grain_mat(vj.w)];
                 const float vr[3] = {ov[0] - vj.x, ov[1] - vj.y, ov[2] - vj.z};
                 float vn = 0.f;
                 vn += vr[0] * nrm[0];
@@ -1442,9 +1496,13 @@ __global__ __launch_bounds__(64) void k_force_q(
                 r1.y = -(c * vn * nrm[1]);
                 r1.z = -(c * vn * nrm[2]);
                 if (HIST) {
-                    const int2 gh = s_gh[og];
                     float xi[3];
-                    h_find_at(gh.x, gh.y, jd, slot, cur.hg, xi);
+                    if constexpr (L == 1) {
+                        h_find_at(cur.hs, cur.hold, jd, slot, cur.hg, xi);
+                    } else {
+                        const int2 gh = s_gh[og];
+                        h_find_at(gh.x, gh.y, jd, slot, cur.hg, xi);
+                    }
                     if (ENERGY && jd > oid) {
                         float s2 = 0.f;
                         s2 += xi[0] * xi[0];
@@ -1472,9 +1530,25 @@ __global__ __launch_bounds__(64) void k_force_q(
                     r2.z = ft[2];
                 }
             }
-            s_rec[0][lane] = r0;
-            s_rec[1][lane] = r1;
-            s_rec[2][lane] = r2;
+            if constexpr (L == 1) {
+                s_r[0][lane] = r0.x;
+                s_r[1][lane] = r0.y;
+                s_r[2][lane] = r0.z;
+                s_r[3][lane] = r1.x;
+                s_r[4][lane] = r1.y;
+                s_r[5][lane] = r1.z;
+                s_r[6][lane] = r2.x;
+                s_r[7][lane] = r2.y;
+                s_r[8][lane] = r2.z;
+                if constexpr (ENERGY) {
+                    s_r[NR - 2][lane] = r0.w;
+                    s_r[NR - 1][lane] = r1.w;
+                }
+            } else {
+                s_rec[0][lane] = r0;
+                s_rec[1][lane] = r1;
+                s_rec[2][lane] = r2;
+            }
             __syncthreads();
             auto add_rec = [&](const float4& a0, const float4& a1, const float4& a2) __attribute__((always_inline)) {
                 f[0] += a0.x;
@@ -1492,7 +1566,14 @@ __global__ __launch_bounds__(64) void k_force_q(
                 }
             };
             auto add = [&](int r) __attribute__((always_inline)) {
-                add_rec(s_rec[0][r], s_rec[1][r], s_rec[2][r]);
+                if constexpr (L == 1) {
+                    const float4 a0 = make_float4(s_r[0][r], s_r[1][r], s_r[2][r], ENERGY ? s_r[NR - 2][r] : 0.f);
+                    const float4 a1 = make_float4(s_r[3][r], s_r[4][r], s_r[5][r], ENERGY ? s_r[NR - 1][r] : 0.f);
+                    const float4 a2 = make_float4(s_r[6][r], s_r[7][r], s_r[8][r], 0.f);
+                    add_rec(a0, a1, a2);
+                } else {
+                    add_rec(s_rec[0][r], s_rec[1][r], s_rec[2][r]);
+                }
             };
             if constexpr (L == 1) {
                 // Rows jc, jc + 1, ... overlap the round; a grain adds its
