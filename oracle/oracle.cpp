@@ -280,12 +280,13 @@ bool spring_fric(T f[3], const T n[3], const T vr[3], T vn, T fp, T c, T xi[3], 
     for (int a = 0; a < 3; ++a) s += ft[a] * ft[a];
     const T fm = sqrt(s);
     const T lim = mu * fp;
+    const T inv_kt = T(1) / kt;  // rounded once, as the engine's parameter block holds it
     const bool slip = forced_slip >= 0 ? forced_slip != 0 : (br ? br->take(fm > lim) : (fm > lim));
     if (slip) {
         const T scale = lim / fm;
         for (int a = 0; a < 3; ++a) {
             ft[a] = ft[a] * scale;
-            xi[a] = -(ft[a] + c * vt[a]) / kt;
+            xi[a] = -(ft[a] + c * vt[a]) * inv_kt;
         }
     }
     for (int a = 0; a < 3; ++a) f[a] += ft[a];

@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
     };
     for (;;) {
         seen = 0;
-        walk_stencil<4, false>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe);
+        walk_stencil<4, false>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe);
         consumed += listed;
         flush();
         if (seen <= consumed) break;

@@ -45,6 +45,7 @@ struct Params {
     float mu;
     float dt;
     float kt;              // tangential spring stiffness (history model)
+    float inv_kt;          // 1 / kt in FP32: a slip resets the spring by multiplying with it
     int history;           // 1: tangential spring with history, 0: viscous (reference)
     int dgx, dgy, dgz;     // density grid
     // Ensemble members (1 for a single simulation).  Member m owns grains
@@ -259,7 +260,7 @@ __device__ __forceinline__ int spring_friction(const Params& P, float mu, float 
         #pragma unroll
         for (int a = 0; a < 3; ++a) {
             ft[a] = ft[a] * scale;
-            xi[a] = -(ft[a] + c * vt[a]) / P.kt;
+            xi[a] = -(ft[a] + c * vt[a]) * P.inv_kt;
         }
         slip = 1;
     }

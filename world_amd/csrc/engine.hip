@@ -731,6 +731,7 @@ void fill_params(const wd_config& c, int n, Params& P) {
     P.dt = c.dt;
     P.history = c.tangential_model == WD_TANGENTIAL_SPRING_HISTORY ? 1 : 0;
     P.kt = c.tangential_stiffness_ratio * k;
+    P.inv_kt = 1.f / P.kt;
     P.dgx = c.density_grid_size_x;
     P.dgy = c.density_grid_size_y;
     P.dgz = c.density_grid_size_z;

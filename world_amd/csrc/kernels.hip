@@ -160,10 +160,35 @@ __device__ __forceinline__ const float* slot_pos(const float4* __restrict__ Sp, 
     return reinterpret_cast<const float*>(reinterpret_cast<const char*>(Sp) + (q << 4));
 }
 
+// Candidate positions for the stencil walk: from the cell-sorted state in
+// global memory, or from a range of it staged in LDS (x, y pairs and z
+// values of slots base, base + 1, ...).
+struct GlobalPos {
+    const float4* __restrict__ Sp;
+    __device__ __forceinline__ void operator()(uint32_t q, float& x, float& y, float& z) const {
+        const float* s = slot_pos(Sp, q);
+        x = s[0];
+        y = s[1];
+        z = s[2];
+    }
+};
+struct StagedPos {
+    const float2* xy;
+    const float* z;
+    uint32_t base;
+    __device__ __forceinline__ void operator()(uint32_t q, float& x, float& y, float& zz) const {
+        const float2 a = xy[q - base];
+        x = a.x;
+        y = a.y;
+        zz = z[q - base];
+    }
+};
+
 // Slot of `partner` in an old spring list (slot-major with stride n, grain
 // index hs, h_old entries), or -1 when it holds no such partner.  The
 // partner ids are loaded eight at a time, all in flight together, rather than
-// one dependent load per slot.
+// one dependent load per slot: the batch is compared with selects only, so
+// no load can sink behind an early exit.
 __device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int n, int hs, int h_old,
                                            int partner) {
     const int* __restrict__ ids = reinterpret_cast<const int*>(h_in) + 3;  // .w of each entry
@@ -174,10 +199,10 @@ __device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int 
             const int t = min(t0 + k, h_old - 1);  // slots past the list reload its last entry
             w[k] = ids[(static_cast<size_t>(t) * n + hs) * 4];
         }
+        int found = -1;  // the first match of the batch (partners are distinct)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (t0 + k < h_old && w[k] == partner) return t0 + k;
-        }
+        for (int k = 7; k >= 0; --k) found = (t0 + k < h_old && w[k] == partner) ? t0 + k : found;
+        if (found >= 0) return found;
     }
     return -1;
 }
@@ -192,10 +217,9 @@ __device__ __forceinline__ int spring_slot(const float4* __restrict__ h_in, int 
 // groups [gx_lo, gx_hi) (columns 3 gx_lo .. 3 gx_hi - 1).  With FLAT, probe is called
 // for every slot of a batch, as probe(q, x, y, z, valid), and must ignore the
 // slots past the run (valid false) itself: it can then run without branches.
-template <int BATCH, bool GROUPED, bool FLAT = false, class Column, class Probe>
-__device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint32_t kcap, uint32_t s_lo,
-                                             uint32_t s_hi, Column column, Probe probe, int gx_lo = 0,
-                                             int gx_hi = 3) {
+template <int BATCH, bool GROUPED, bool FLAT = false, class Pos, class Column, class Probe>
+__device__ __forceinline__ void walk_stencil(Pos pos, uint32_t kcap, uint32_t s_lo, uint32_t s_hi,
+                                             Column column, Probe probe, int gx_lo = 0, int gx_hi = 3) {
     uint32_t off = 0;  // sequence position of the run's first candidate
     // Candidates b0 + [0, len): a column with no cell over capacity.
     auto run1 = [&](uint32_t b0, uint32_t len) __attribute__((always_inline)) {
@@ -206,12 +230,7 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             // Every slot loads (one past the run reloads the run's last
             // candidate), so that all loads are in flight together.
 #pragma unroll
-            for (int k = 0; k < BATCH; ++k) {
-                const float* s = slot_pos(Sp, b0 + min(i + k, i1 - 1));
-                px[k] = s[0];
-                py[k] = s[1];
-                pz[k] = s[2];
-            }
+            for (int k = 0; k < BATCH; ++k) pos(b0 + min(i + k, i1 - 1), px[k], py[k], pz[k]);
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
                 if constexpr (FLAT) probe(b0 + i + k, px[k], py[k], pz[k], i + k < i1);
@@ -236,10 +255,7 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
             for (int k = 0; k < BATCH; ++k) {
                 const uint32_t t = i + k;
                 q[k] = t < l0 ? b0 + t : (t < l0 + l1 ? b1 + (t - l0) : b2 + (t - l0 - l1));
-                const float* s = slot_pos(Sp, t < i1 ? q[k] : q[0]);
-                px[k] = s[0];
-                py[k] = s[1];
-                pz[k] = s[2];
+                pos(t < i1 ? q[k] : q[0], px[k], py[k], pz[k]);
             }
 #pragma unroll
             for (int k = 0; k < BATCH; ++k) {
@@ -250,11 +266,11 @@ __device__ __forceinline__ void walk_stencil(const float4* __restrict__ Sp, uint
         off += tot;
     };
     if constexpr (!GROUPED) {
-        uint4 next = column(0);
+        uint4 next = column(3 * gx_lo);
 #pragma unroll 1
-        for (int c = 0; c < 9; ++c) {
+        for (int c = 3 * gx_lo; c < 3 * gx_hi; ++c) {
             const uint4 a = next;
-            if (c < 8) next = column(c + 1);
+            if (c + 1 < 3 * gx_hi) next = column(c + 1);
             if (!column_capped(a, kcap)) {
                 run1(a.x, a.w - a.x);
             } else {
@@ -1043,7 +1059,7 @@ __global__ __launch_bounds__(BS, 5) void k_force(
         };
         for (;;) {
             seen = 0;
-            walk_stencil<4, false>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe);
+            walk_stencil<4, false>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe);
             consumed += listed;
             flush();
             if (seen <= consumed) break;
@@ -1139,28 +1155,30 @@ __global__ __launch_bounds__(64) void k_force_q(
     const Params& P = *Pp;
     // Per lane: cell-sorted slots of listed contacts; with one lane per
     // grain, row QCAP takes the stores of candidates that are not listed.
-    // One lane per grain keeps its LDS small (under 7 KB, six waves per
+    // One lane per grain keeps its LDS small (about 7 KB: five waves per
     // SIMD at the register count it takes without the next round's
     // prefetch): the owner's spring slot and old spring list come by
-    // shuffles instead of LDS, an entry's owner lane is found from its row's
-    // ballot, a round's records are packed floats, the stores of unlisted
-    // candidates go to the record area (free while the stencil is walked),
-    // and the damping table holds the handle's materials only (dynamic LDS,
-    // nmat x nmat).
+    // shuffles instead of LDS, an entry carries its owner lane (or, past 2^22
+    // grains, it is found from its row's ballot), a round's records are
+    // packed floats, the record area holds the staged candidate positions
+    // while the stencil is walked, and the damping table holds the handle's
+    // materials only (dynamic LDS, nmat x nmat).
     constexpr int NR = ENERGY ? 11 : 9;         // floats per record (L == 1)
-    __shared__ uint32_t s_list[QCAP][64];
+    constexpr int RB = NR * 64 / 3;             // L == 1: staged slots (x, y pair and z each)
+    __shared__ uint32_t s_list[L == 1 ? QCAP + 1 : QCAP][64];  // L == 1: row QCAP is the spare row
     __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
     __shared__ uint8_t s_src[L == 1 ? 1 : QCAP * 64];  // owner lane of each queue entry (L > 1)
     __shared__ unsigned long long s_rowmask[L == 1 ? QCAP : 1];  // L == 1: lanes listing row j
     __shared__ int s_rowstart[L == 1 ? QCAP : 1];  // L == 1: queue index of row j's first entry
     __shared__ int s_slot[L == 1 ? 1 : 64];     // per lane: spring slot of its first listed contact
     __shared__ float4 s_rec[L == 1 ? 1 : 3][L == 1 ? 1 : 64];  // per queue entry: force, energy addends
-    __shared__ float s_r[L == 1 ? NR : 1][64];  // the same, packed (L == 1)
+    __shared__ __align__(16) float s_r[L == 1 ? NR : 1][64];  // the same, packed (L == 1)
     __shared__ int2 s_gh[L == 1 ? 1 : G];       // per grain: old spring list (index, count)
     __shared__ float s_cpair[L == 1 ? 1 : kMaxMat * kMaxMat];
     extern __shared__ float s_cdyn[];           // L == 1: c_pair of the handle's nmat materials
-    static_assert(NR * 64 * sizeof(float) >= 64 * sizeof(uint32_t), "record area holds a spare list row");
-    uint32_t* const s_spare = reinterpret_cast<uint32_t*>(&s_r[0][0]);
+    uint32_t* const s_spare = &s_list[L == 1 ? QCAP : 0][0];
+    float2* const s_sxy = reinterpret_cast<float2*>(&s_r[0][0]);  // staged x, y (L == 1)
+    float* const s_sz = reinterpret_cast<float*>(s_sxy + RB);      // staged z (L == 1)
     // A lone simulation's pair damping table, in LDS: one lookup per contact.
     const bool lds_tab = P.mtab == nullptr;
     const int nmat = P.nmat;
@@ -1176,6 +1194,7 @@ __global__ __launch_bounds__(64) void k_force_q(
     const int g = lane / L, sub = lane % L;
     const int lb = logical_block();
     const int p = lb * G + g;
+    const bool owner_bits = n <= (1 << 22);  // L == 1: queue entries carry their owner lane
     const bool live = p < n;
 
     const int par = HIST ? ctl->hist_parity : 0;
@@ -1325,16 +1344,70 @@ __global__ __launch_bounds__(64) void k_force_q(
     int total = 0;      // touching candidates in this lane's slice
     int lane_slot = 0;  // spring slot of this lane's first contact
     int grain_slots = 0;
+    // One lane per grain, lone simulation: the cells a wave's grains reach
+    // with one dx of the stencil are the contiguous range from its first
+    // grain's cell - (dy, dz) = (1, 1) to its last grain's + (1, 1) (the
+    // grains are in cell order and the sub-grid is padded), so their
+    // candidates are one contiguous range of slots.  Each dx group's range
+    // is staged in LDS with coalesced loads and walked from there; a range
+    // over RB slots is walked from global memory.
+    // Blocks past the last grain (the grid is rounded up to whole XCD
+    // slabs) stage nothing; a range outside the cell-start array (which the
+    // padded sub-grid rules out) is walked from global memory.
+    uint32_t c_first = 0u, c_last = 0u, Sx = 0u, Sy = 0u;
+    const bool stage_ok = L == 1 && P.members == 1 && lb * G < n;
+    if constexpr (L == 1) {
+        if (stage_ok) {
+            const MemberCells m0 = member_cells(P, 0);
+            const uint32_t c_self = live ? m0.index(cx, cy, cz) : 0u;
+            // lane 0 holds the block's first grain, lane n - 1 - lb G (or
+            // 63) its last
+            c_first = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(c_self)));
+            c_last = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c_self), min(63, n - 1 - lb * G)));
+            Sx = static_cast<uint32_t>(m0.g.by * m0.g.bz);
+            Sy = static_cast<uint32_t>(m0.g.bz);
+        }
+    }
+    const uint32_t n_cells = P.mcell[1];  // lone simulation: start[] holds n_cells + 1 entries
     for (int pass = 0;; ++pass) {
         seen = 0;
-        if (live && (pass == 0 || cons < total)) {
-            if constexpr (L == 1) {
-                walk_stencil<4, false, true>(Sp, kcap, s_lo, s_hi, column, probe);
-            } else if constexpr (L == 4) {
+        const bool walking = live && (pass == 0 || cons < total);
+        if constexpr (L == 1) {
+#pragma unroll 1
+            for (int gx = 0; gx < 3; ++gx) {
+                bool staged = false;
+                uint32_t g_lo = 0u;
+                if (stage_ok) {
+                    const uint32_t lo = c_first + static_cast<uint32_t>(gx - 1) * Sx - Sy - 1u;
+                    const uint32_t hi = c_last + static_cast<uint32_t>(gx - 1) * Sx + Sy + 2u;
+                    const bool inside = lo <= hi && hi <= n_cells;
+                    g_lo = inside ? start[lo] : 0u;
+                    const uint32_t len = inside ? start[hi] - g_lo : 0xFFFFFFFFu;
+                    if (len <= static_cast<uint32_t>(RB)) {
+                        staged = true;
+                        for (uint32_t k = static_cast<uint32_t>(lane); k < len; k += 64u) {
+                            const float4 a = Sp[g_lo + k];
+                            s_sxy[k] = make_float2(a.x, a.y);
+                            s_sz[k] = a.z;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (walking) {
+                    if (staged) {
+                        walk_stencil<4, false, true>(StagedPos{s_sxy, s_sz, g_lo}, kcap, s_lo, s_hi, column, probe, gx, gx + 1);
+                    } else {
+                        walk_stencil<4, false, true>(GlobalPos{Sp}, kcap, s_lo, s_hi, column, probe, gx, gx + 1);
+                    }
+                }
+                __syncthreads();  // the staged range is read before the next one is written
+            }
+        } else if (walking) {
+            if constexpr (L == 4) {
                 // Lanes 0, 1, 2 of the grain walk dx groups 0, 1, 2 (a
                 // contiguous part of the sequence each, in lane order);
                 // lane 3 lists nothing.
-                if (sub < 3) walk_stencil<12, true>(Sp, kcap, 0u, 0xFFFFFFFFu, column, probe, sub, sub + 1);
+                if (sub < 3) walk_stencil<12, true>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe, sub, sub + 1);
             } else {
                 // The grain's nine columns: its sequence length (for the
                 // slices) and, when no cell is over capacity, the runs.
@@ -1352,7 +1425,7 @@ __global__ __launch_bounds__(64) void k_force_q(
                     s_lo = tot * static_cast<uint32_t>(sub) / L;
                     s_hi = tot * static_cast<uint32_t>(sub + 1) / L;
                 }
-                if (capped) walk_stencil<8, true>(Sp, kcap, s_lo, s_hi, column, probe);
+                if (capped) walk_stencil<8, true>(GlobalPos{Sp}, kcap, s_lo, s_hi, column, probe);
                 else walk_slice<8>(Sp, cb, cpre, s_lo, s_hi, probe);
             }
         }
@@ -1379,8 +1452,10 @@ __global__ __launch_bounds__(64) void k_force_q(
             // order, and a round of 64 holds few of any one grain's, so the
             // per-grain sums after a round are short.  The list is compacted
             // in place: row j's entries land below index 64 (j + 1), where
-            // no unread row lies.  An entry is the partner slot (below 2^28)
-            // with its row in the top four bits.
+            // no unread row lies.  An entry is the partner slot with its row
+            // in the top four bits; below 2^22 grains the owner lane sits in
+            // bits 22-27, otherwise (slots below 2^28) the owner is found
+            // from the row's ballot.
             for (int j = 0; j < QCAP; ++j) {
                 const unsigned long long mask = __ballot(j < cnt);
                 if (mask == 0ull) break;
@@ -1388,9 +1463,10 @@ __global__ __launch_bounds__(64) void k_force_q(
                 __syncthreads();  // the row is read before any of it is overwritten
                 if (j < cnt) {
                     const int pos = T + lanes_below(mask);
-                    s_flat[pos] = q | (static_cast<uint32_t>(j) << 28);
+                    s_flat[pos] = q | (static_cast<uint32_t>(j) << 28) |
+                                  (owner_bits ? static_cast<uint32_t>(lane) << 22 : 0u);
                 }
-                if (lane == 0) {
+                if (!owner_bits && lane == 0) {
                     s_rowmask[j] = mask;
                     s_rowstart[j] = T;
                 }
@@ -1427,8 +1503,14 @@ __global__ __launch_bounds__(64) void k_force_q(
                 const uint32_t w = s_flat[e];
                 const int j = static_cast<int>(w >> 28);
                 // the owner: the (e - start)-th lane of the row's ballot
-                const int src = nth_lane(s_rowmask[j], e - s_rowstart[j]);
-                t.q = w & 0x0FFFFFFFu;
+                int src;
+                if (owner_bits) {
+                    src = static_cast<int>((w >> 22) & 63u);
+                    t.q = w & 0x003FFFFFu;
+                } else {
+                    src = nth_lane(s_rowmask[j], e - s_rowstart[j]);
+                    t.q = w & 0x0FFFFFFFu;
+                }
                 t.slot = __shfl(slot_base, src, 64) + j;
                 t.og = src;
                 t.hs = HIST ? __shfl(hs, src, 64) : 0;
@@ -1463,6 +1545,7 @@ __global__ __launch_bounds__(64) void k_force_q(
             const float ov[3] = {__shfl(v[0], ol, 64), __shfl(v[1], ol, 64), __shfl(v[2], ol, 64)};
             const int oid = __shfl(id, ol, 64);
             const float ovw = __shfl(vi4.w, ol, 64);
+            const float omu = __shfl(mu, ol, 64);
             float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
             if (e < T) {
                 const int slot = cur.slot;
@@ -1470,7 +1553,6 @@ __global__ __launch_bounds__(64) void k_force_q(
                 const int po = lb * G + og;
                 const int omat = grain_mat(ovw);
                 const int omember = member_of(P, ovw);
-                const float omu = P.members == 1 ? P.mu : P.mmu[omember];
                 const float4 pj = cur.pj;
                 float d0, d1, d2;
                 const float norm = pair_norm(ox, pj.x, pj.y, pj.z, d0, d1, d2);
