@@ -501,6 +501,43 @@ def copy_probe(mib: int = 256) -> None:
     torch.cuda.synchronize()
 
 
+WINDOW_STATE = ROOT / "gpurun_out" / "column_1m_step4950.npy"
+
+
+def save_window_state(steps: int = 4950) -> None:
+    """Advances the 1,048,576-grain column `steps` steps and saves its grain
+    state (original order), so that a short process under a counter
+    profiler can start from the 1M window (rocprofv3 on this image fails
+    in long or graph-replaying runs)."""
+    import numpy as np
+    from world_amd import workloads
+    from world_amd.sim import ParticleDynamics
+
+    with ParticleDynamics(workloads.column_collapse_1m()) as sim:
+        advance(sim, steps, 256)
+        WINDOW_STATE.parent.mkdir(parents=True, exist_ok=True)
+        np.save(WINDOW_STATE, sim.state())
+
+
+def restored_steps(k: int = 20) -> None:
+    """k eager steps of the 1M column from the saved window state, with the
+    one-lane force pass pinned.  The tangential springs are not part of
+    the saved state: they restart at zero and are rebuilt in the first
+    step, so the later of the k launches are the ones to read."""
+    import numpy as np
+    from world_amd import workloads
+    from world_amd.sim import ParticleDynamics
+
+    cfg = workloads.column_collapse_1m()
+    cfg.kernel_timing = 1
+    with ParticleDynamics(cfg) as sim:
+        sim.set_state(np.load(WINDOW_STATE))
+        sim.set_force_variant(64, 1)
+        for _ in range(k):
+            sim.take_step(1)
+        sim.sync()
+
+
 def host_cpu() -> dict:
     """The host's processor and the threads this process may use: the CPU
     set it is bound to, capped by OMP_NUM_THREADS where that is set (the
