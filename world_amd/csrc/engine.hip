@@ -74,6 +74,16 @@ struct wd_sim {
     int hidx_cur = 0;                   // hidx buffer the last bin wrote
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr;
     int* tmp_id = nullptr;
+    // Bucket binning (no scatter pass): kBucket ids per cell and two spill
+    // lists of n (cell, id) entries, allocated unless the buckets would pass
+    // 1 GiB.  Which binning runs is tuned online with the scan tile (the
+    // bucket writes are scattered: they pay at 32K grains, not at 1M); the
+    // cell order is the same either way.
+    uint32_t* bucket = nullptr;
+    uint2* spill = nullptr;
+    bool bucket_ok = false;
+    bool bucket_bin = false;
+    double bin_ms[2] = {};        // last tuning's bin time (count, scatter, reorder) per mode
     unsigned long long* status = nullptr;
     size_t scan_cells = 0, ntiles_max = 0;  // cells allocated (whole largest tiles); most tiles
     size_t scan_cells_used = 0;              // cells + 1 of the largest possible sub-grids
@@ -210,7 +220,7 @@ struct wd_sim {
         spare.clear();
         void* bufs[] = {dp, ctl, Tp, Tv, Sp, Sv, NP[0], NP[1], NV[0], NV[1], H[0], H[1], Hc[0],
                         Hc[1], key, slot, count,
-                        start, tmp_id, status, st6, mat_d, parts, energy_log, cic_acc,
+                        start, tmp_id, bucket, spill, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
                         hidx[0], hidx[1], d_mtab, d_movf, pmem};
         for (void* b : bufs) {
@@ -222,6 +232,7 @@ struct wd_sim {
         Hc[0] = Hc[1] = nullptr;
         hidx[0] = hidx[1] = nullptr;
         key = slot = count = start = nullptr; tmp_id = nullptr; status = nullptr;
+        bucket = nullptr; spill = nullptr;
         st6 = nullptr; mat_d = nullptr; parts = nullptr; pmem = nullptr; energy_log = nullptr; ewave = nullptr;
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
@@ -298,8 +309,15 @@ struct wd_sim {
         // blocks.
         const int bbs = 64;
         const int gb = grain_blocks(n, bbs);
-        go("k_count", k_count, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tv, key,
-           slot, count, n, bin_box_out, bin_cell_out);
+        if (bucket_bin) {
+            go("k_count", k_count<true>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
+               (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
+               spill);
+        } else {
+            go("k_count", k_count<false>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
+               (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
+               spill);
+        }
         // Tiles of the chosen size covering every cell the handle can hold,
         // on up to 1,024 blocks; blocks past the current box's tiles return.
         const size_t tile = static_cast<size_t>(kScanThreads) * kScanItemsVariants[scan_var];
@@ -311,21 +329,29 @@ struct wd_sim {
         if (scan_var == 0) scan(k_scan<4>);
         else if (scan_var == 1) scan(k_scan<8>);
         else scan(k_scan<16>);
-        go("k_scatter", k_scatter, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
-           (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
+        if (!bucket_bin) {
+            go("k_scatter", k_scatter, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
+               (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
+        }
         // Picard iterations after the first carry the spring index of the
         // step's first bin through every re-sort (nmode 2).
         const int hout = nmode == 2 ? hidx_cur ^ 1 : 0;
         const Carry carry{nin_p, nin_v, nout_p, nout_v, hidx[hidx_cur], history() ? hidx[hout] : nullptr};
         hidx_cur = hout;
         auto reorder = [&](auto kern) {
-            go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
-               (const uint32_t*)start, (const int*)tmp_id, (const float4*)Tp, (const float4*)Tv, Sp, Sv,
-               carry, n);
+            go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, (const Ctl*)ctl,
+               (const uint32_t*)key, (const uint32_t*)start, (const int*)tmp_id, (const uint32_t*)bucket,
+               (const uint2*)spill, (const float4*)Tp, (const float4*)Tv, Sp, Sv, carry, n);
         };
-        if (nmode == 0) reorder(k_reorder<0>);
-        else if (nmode == 1) reorder(k_reorder<1>);
-        else reorder(k_reorder<2>);
+        if (bucket_bin) {
+            if (nmode == 0) reorder(k_reorder<0, true>);
+            else if (nmode == 1) reorder(k_reorder<1, true>);
+            else reorder(k_reorder<2, true>);
+        } else {
+            if (nmode == 0) reorder(k_reorder<0, false>);
+            else if (nmode == 1) reorder(k_reorder<1, false>);
+            else reorder(k_reorder<2, false>);
+        }
     }
 
     void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr); }
@@ -483,10 +509,12 @@ struct wd_sim {
         bool first = false;
         int before = -1;     // variant the graphs were captured with
         int scan_before = 1;
-        int scans = 0;       // tuning steps taken (each also times one scan tile size)
+        bool bin_before = false;
+        int scans = 0;       // tuning steps taken (each also times one scan tile size and bin mode)
         bool cand[kCand] = {};
         double best[kCand] = {};
         double scan_best[3] = {};
+        double bin_best[2] = {};
     } tuning;
 
     void begin_tuning() {
@@ -504,8 +532,10 @@ struct wd_sim {
         }
         t.before = force_var;
         t.scan_before = scan_var;
+        t.bin_before = bucket_bin;
         t.scans = 0;
         for (int v = 0; v < 3; ++v) t.scan_best[v] = 1e30;
+        for (int v = 0; v < 2; ++v) t.bin_best[v] = 1e30;
         t.pos = 0;
     }
 
@@ -528,13 +558,17 @@ struct wd_sim {
             const int sv = t.scans % 3;
             const bool scan_timed = !scan_fixed && (t.scans >= 3 || !t.first);
             if (!scan_fixed) scan_var = sv;
+            // and the binning, alternating (with buckets on odd steps)
+            const int bv = t.scans % 2;
+            const bool bin_timed = bucket_ok && (t.scans >= 2 || !t.first);
+            if (bucket_ok) bucket_bin = bv == 1;
             ++t.scans;
             std::vector<ProfRec> recs;
             prof = &recs;
             enqueue_step();
             prof = nullptr;
             WD_HIP(hipStreamSynchronize(stream));
-            double ms = 0.0, scan = 0.0;
+            double ms = 0.0, scan = 0.0, bin = 0.0;
             for (auto& rec : recs) {
                 float e = 0.f;
                 const bool ok = hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess;
@@ -543,11 +577,16 @@ struct wd_sim {
                     ms += e;
                 }
                 if (ok && std::strcmp(rec.name, "k_scan") == 0) scan += e;
+                if (ok && (std::strcmp(rec.name, "k_count") == 0 || std::strcmp(rec.name, "k_scatter") == 0 ||
+                           std::strcmp(rec.name, "k_reorder") == 0)) {
+                    bin += e;
+                }
                 spare.push_back(rec.a);
                 spare.push_back(rec.b);
             }
             if (r > 0 || !t.first) t.best[c] = std::min(t.best[c], ms);
             if (scan_timed) t.scan_best[sv] = std::min(t.scan_best[sv], scan);
+            if (bin_timed) t.bin_best[bv] = std::min(t.bin_best[bv], bin);
             time += hp.dt;
             --nsteps;
             ++steps_run;
@@ -568,9 +607,15 @@ struct wd_sim {
             for (int v = 0; v < 3; ++v) scan_ms[v] = t.scan_best[v];
             scan_var = sb;
         }
+        if (bucket_ok) {
+            for (int v = 0; v < 2; ++v) bin_ms[v] = t.bin_best[v];
+            const int cur = t.bin_before ? 1 : 0;
+            const int other = 1 - cur;
+            bucket_bin = (t.bin_best[other] < t.bin_best[cur] ? other : cur) == 1;
+        }
         next_tune = next_tune == 0 ? 256 : 8 * next_tune;
-        // graphs captured with another variant or tile size
-        if (best != t.before || scan_var != t.scan_before) drop_graphs();
+        // graphs captured with another variant, tile size or binning
+        if (best != t.before || scan_var != t.scan_before || bucket_bin != t.bin_before) drop_graphs();
         t.pos = -1;
         return check_device();
     }
@@ -841,6 +886,12 @@ int allocate(wd_sim* s) {
     s->scan_cells = (cells + 1 + kScanTileMax - 1) / kScanTileMax * kScanTileMax;
     WD_TRY(dalloc(&s->count, s->scan_cells));
     WD_TRY(dalloc(&s->start, s->scan_cells));
+    s->bucket_ok = s->scan_cells * kBucket * sizeof(uint32_t) <= (size_t(1) << 30);
+    s->bucket_bin = s->bucket_ok && n <= (1 << 17);  // until tuned
+    if (s->bucket_ok) {
+        WD_TRY(dalloc(&s->bucket, s->scan_cells * kBucket));
+        WD_TRY(dalloc(&s->spill, static_cast<size_t>(2) * std::max(n, 1)));
+    }
     WD_TRY(dalloc(&s->status, s->ntiles_max));
     WD_TRY(dalloc(&s->st6, static_cast<size_t>(6) * n));
     WD_TRY(dalloc(&s->mat_d, n));
@@ -2178,6 +2229,9 @@ extern "C" int wd_tune_cell_scale(wd_sim* s, int steps, float* chosen) {
             ctl0.hist_parity = now.hist_parity;
             ctl0.epoch = now.epoch;
             ctl0.tile_counter = now.tile_counter;
+            ctl0.bin_par = now.bin_par;  // with the spill counters, as the last scan left them
+            ctl0.spill_n[0] = now.spill_n[0];
+            ctl0.spill_n[1] = now.spill_n[1];
             if (hipMemcpy(s->ctl, &ctl0, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(s->d_movf, movf0.data(), sizeof(unsigned long long) * movf0.size(),
                           hipMemcpyHostToDevice) != hipSuccess) {

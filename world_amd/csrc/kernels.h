@@ -22,8 +22,8 @@ constexpr int kQueueCap = 16;        // contacts listed per lane and pass by the
 constexpr int kEnergyGroup = 16;     // grains per energy partial (ewave entry)
 
 // Small device-side control block: counters, the occupied-cell bounding box
-// and per-pass results.  Only k_count (block 0) and k_finalize write it
-// outside of atomics, so no pass needs a grid-wide fence.
+// and per-pass results.  Only k_count (block 0), k_scan (block 0) and
+// k_finalize write it outside of atomics, so no pass needs a grid-wide fence.
 struct Ctl {
     unsigned tile_counter;     // scan tile dispenser; zeroed by k_count
     unsigned epoch;            // scan look-back tag; bumped by k_count
@@ -37,7 +37,16 @@ struct Ctl {
     unsigned long long force_passes;  // completed force passes (k_finalize)
     unsigned long long contact_sum;   // contacts (walls + pairs) summed over those passes
     unsigned cic_log_next;            // density-log slots taken (k_cic_zero of a logged deposit)
+    // Bucket binning: the count pass files grains past a cell's bucket in
+    // spill list bin_par (entries spill_n[bin_par]); k_scan empties the other
+    // list and flips bin_par, so the reorder reads list bin_par ^ 1.
+    unsigned bin_par;
+    unsigned spill_n[2];
 };
+
+// Ids a cell's bucket holds (bucket binning): the count pass writes a grain's
+// id at bucket[cell * kBucket + slot] when its slot is below kBucket.
+constexpr int kBucket = 8;
 
 // Per-block partial results of a grain pass, reduced by k_finalize, in two
 // arrays indexed by block.  A block covers consecutive grains, so it holds a

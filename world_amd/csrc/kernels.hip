@@ -4,10 +4,11 @@
 // original grain index in .w, velocity with the material id in .w (32 B per
 // grain, one dwordx4 load each).  A step is
 //
-//   k_count   -> slot of every grain in its cell (atomic per-cell counter)
+//   k_count   -> slot of every grain in its cell (atomic per-cell counter);
+//                with bucket binning, the grain's id into its cell's bucket
 //   k_scan    -> exclusive scan of the counts over the occupied sub-grid,
 //                one pass with decoupled look-back; counts re-zeroed here
-//   k_scatter -> original ids into cell-sorted slots
+//   k_scatter -> original ids into cell-sorted slots (not with buckets)
 //   k_reorder -> rank inside the cell by original id, move the grain to its
 //                cell-sorted position (coalesced float4 streams)
 //   k_force   -> walls + 27-cell pair contacts + integration, fused; writes
@@ -577,14 +578,21 @@ __global__ __launch_bounds__(kBlock) void k_rekey(const Params* __restrict__ Pp,
 // ---------------------------------------------------------------------------
 // Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
 // box_out / cell_out (null unless a caller keeps the bin of this pass): a
-// copy of the member boxes and cell offsets the pass bins with.
+// copy of the member boxes and cell offsets the pass bins with.  BUCKET:
+// instead of the slot, the grain's id goes into its cell's bucket (slots
+// below kBucket) or, past it, into spill list ctl->bin_par as (cell, id); the
+// reorder then ranks a cell's ids without a scatter pass.
+template <bool BUCKET>
 __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp, Ctl* ctl,
+                                                  const float4* __restrict__ Tp,
                                                   const float4* __restrict__ Tv,
                                                   const uint32_t* __restrict__ key,
                                                   uint32_t* __restrict__ slot,
                                                   uint32_t* __restrict__ count, int n,
                                                   int* __restrict__ box_out,
-                                                  uint32_t* __restrict__ cell_out) {
+                                                  uint32_t* __restrict__ cell_out,
+                                                  uint32_t* __restrict__ bucket,
+                                                  uint2* __restrict__ spill) {
     const Params& P = *Pp;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->tile_counter = 0;
@@ -602,7 +610,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     uint32_t c = 0xFFFFFFFFu;  // no cell
+    int id = 0;
     if (valid) {
+        if (BUCKET) id = __float_as_int(Tp[i].w);
         const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
         int cx, cy, cz;
         unpack_cell(key[i], cx, cy, cz);
@@ -619,7 +629,20 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
     uint32_t base = 0u;
     if (valid && lane == h) base = atomicAdd(&count[c], static_cast<uint32_t>(e - h));
     base = static_cast<uint32_t>(__shfl(static_cast<int>(base), h, 64));
-    if (valid) slot[i] = base + static_cast<uint32_t>(lane - h);
+    if (valid) {
+        const uint32_t s = base + static_cast<uint32_t>(lane - h);
+        if constexpr (BUCKET) {
+            if (s < static_cast<uint32_t>(kBucket)) {
+                bucket[static_cast<size_t>(c) * kBucket + s] = static_cast<uint32_t>(id);
+            } else {
+                const unsigned bp = ctl->bin_par;  // only k_scan writes it
+                const unsigned e = atomicAdd(&ctl->spill_n[bp], 1u);  // below n: one entry per grain
+                spill[static_cast<size_t>(bp) * n + e] = make_uint2(c, static_cast<uint32_t>(id));
+            }
+        } else {
+            slot[i] = s;
+        }
+    }
 }
 
 // Pass 2: start[c] = sum of count[0..c) over the concatenated member
@@ -647,6 +670,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     __shared__ uint32_t s_over[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // bucket binning: the next count pass spills into the other list,
+        // emptied here (its last reader, the previous reorder, is done)
+        const unsigned bp = ctl->bin_par;
+        ctl->spill_n[bp ^ 1u] = 0u;
+        ctl->bin_par = bp ^ 1u;
+    }
     if (static_cast<int>(blockIdx.x) >= ntiles) return;  // never takes a tile
     // A block scans tile blockIdx.x first, then (when the grid has fewer
     // blocks than tiles) tiles handed out by the counter past the grid, each
@@ -785,12 +815,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Params* __restrict__ P
 }
 
 // Pass 4: final position = cell start + rank of the original id among the
-// cell's ids.  NMODE 1 also copies the grain into the Picard base buffers,
+// cell's ids (from the scattered ids, or with BUCKET from the cell's bucket
+// and, for a cell of more than kBucket grains, the spill list the count pass
+// filled).  NMODE 1 also copies the grain into the Picard base buffers,
 // NMODE 2 carries the Picard base along the same permutation.
-template <int NMODE>
+template <int NMODE, bool BUCKET>
 __global__ __launch_bounds__(kBlock) void k_reorder(
-        const Params* __restrict__ Pp, const uint32_t* __restrict__ key,
+        const Params* __restrict__ Pp, const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
+        const uint32_t* __restrict__ bucket, const uint2* __restrict__ spill,
         const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
         float4* __restrict__ Sv, Carry carry, int n) {
     const Params& P = *Pp;
@@ -806,7 +839,22 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     const int id = __float_as_int(p.w);
     uint32_t rank = 0;
     if (e - b > 1) {
-        for (uint32_t q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1u : 0u;
+        if constexpr (BUCKET) {
+            const uint32_t* bk = bucket + static_cast<size_t>(c) * kBucket;
+            const uint32_t m = min(e - b, static_cast<uint32_t>(kBucket));
+            for (uint32_t q = 0; q < m; ++q) rank += (static_cast<int>(bk[q]) < id) ? 1u : 0u;
+            if (e - b > static_cast<uint32_t>(kBucket)) {
+                const unsigned bp = ctl->bin_par ^ 1u;  // the list the count pass filled
+                const unsigned ns = ctl->spill_n[bp];
+                const uint2* sp = spill + static_cast<size_t>(bp) * n;
+                for (unsigned q = 0; q < ns; ++q) {
+                    const uint2 v = sp[q];
+                    rank += (v.x == c && static_cast<int>(v.y) < id) ? 1u : 0u;
+                }
+            }
+        } else {
+            for (uint32_t q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1u : 0u;
+        }
     }
     const uint32_t dst = b + rank;
     Sp[dst] = p;
@@ -2157,16 +2205,24 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     }
 }
 
-#define WD_INST_REORDER(M)                                                                   \
-    template __global__ void k_reorder<M>(const Params*, const uint32_t*, const uint32_t*,       \
-                                          const int*, const float4*, const float4*, float4*,     \
-                                          float4*, Carry, int);
+#define WD_INST_REORDER(M, B)                                                                \
+    template __global__ void k_reorder<M, B>(const Params*, const Ctl*, const uint32_t*,          \
+                                             const uint32_t*, const int*, const uint32_t*,        \
+                                             const uint2*, const float4*, const float4*, float4*, \
+                                             float4*, Carry, int);
+template __global__ void k_count<false>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
+                                        uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*);
+template __global__ void k_count<true>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
+                                       uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*);
 template __global__ void k_scan<4>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
 template __global__ void k_scan<8>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
 template __global__ void k_scan<16>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
-WD_INST_REORDER(0)
-WD_INST_REORDER(1)
-WD_INST_REORDER(2)
+WD_INST_REORDER(0, false)
+WD_INST_REORDER(1, false)
+WD_INST_REORDER(2, false)
+WD_INST_REORDER(0, true)
+WD_INST_REORDER(1, true)
+WD_INST_REORDER(2, true)
 #undef WD_INST_REORDER
 
 #define WD_INST_FORCE(M, E, H, B)                                                               \
