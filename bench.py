@@ -175,6 +175,7 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
     sim.enable_density_log(len(deposit_steps(s0, k, every)) + 1)
     variant = prepare(sim, s0, pin)
     tile = sim.scan_tile()
+    binning = sim.binning()
     passes0, contacts0 = sim.contact_stats()
     hist = cfg.tangential_model == 1
     start = None
@@ -190,7 +191,7 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
     out = {
         "n": n, "elapsed": elapsed, "s0": s0, "k": k, "deposits": ndep,
         "stable": sim.is_stable(), "overflow": sim.grid_overflow_count(),
-        "variant": variant, "scan_tile": tile, "contacts_per_grain": z, "history": hist,
+        "variant": variant, "scan_tile": tile, "binning": binning, "contacts_per_grain": z, "history": hist,
         "eager": cfg.kernel_timing != 0,
         "tuned": sim.force_variant() == tuple(variant),
     }
@@ -203,6 +204,7 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
         prof = ParticleDynamics(cfg)
         prof.set_force_variant(*variant)
         prof.set_scan_tile(tile)
+        prof.set_binning(binning)
         if energy:
             prof.enable_energy_log(s0 + k + 64)
         advance(prof, s0)
@@ -279,6 +281,7 @@ def window_config(run: dict, every: int) -> dict:
             "deposit_every": every, "force_variant": {"block": run["variant"][0],
                                                       "lanes_per_grain": run["variant"][1]},
             "scan_tile_cells": run["scan_tile"],
+            "binning": "buckets" if run["binning"] == 2 else "scatter",
             "tuned_before_window": True, "launch": "eager" if run["eager"] else "graphs captured before the window"}
 
 

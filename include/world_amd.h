@@ -325,6 +325,21 @@ int wd_get_force_block(const wd_sim* sim, int* block_out); /* 0 until tuned */
 int wd_set_scan_tile(wd_sim* sim, int cells);
 int wd_get_scan_tile(const wd_sim* sim, int* cells_out);
 
+/* How a step bins its grains into cells.  Scatter: the count pass gives each
+ * grain a slot in its cell, a scatter pass writes the ids in cell order and
+ * the reorder ranks a grain among its cell's ids.  Buckets: the count pass
+ * writes the id into its cell's bucket (8 per cell, the rest to a spill
+ * list), and the reorder ranks from there: one launch fewer, scattered
+ * writes.  The cell order, and every result, is the same either way; the
+ * tuner times both on its steps (buckets are only allocated when they fit in
+ * 1 GiB).  The reference bins with its own atomics-and-fill kernels
+ * (src/find_neighbors_kernel.cu:43-161, 190-223). */
+#define WD_BINNING_TUNED 0
+#define WD_BINNING_SCATTER 1
+#define WD_BINNING_BUCKETS 2
+int wd_set_binning(wd_sim* sim, int mode);
+int wd_get_binning(const wd_sim* sim, int* mode_out); /* WD_BINNING_SCATTER or _BUCKETS */
+
 /* Extension, opt-in: collision cells coarser than the configured grid (the
  * reference sizes cells from the config, collision_grid_size_*,
  * src/sim_config.h:64-200).  Every axis gets floor(G / scale) cells, so a

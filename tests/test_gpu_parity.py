@@ -482,6 +482,58 @@ def test_scan_tile_is_tuned_and_keeps_results():
             tuned.set_scan_tile(3000)
 
 
+def test_binnings_keep_results():
+    """Scatter and bucket binning give the same steps bit for bit (the
+    tuner picks either); the choice can be pinned and read back."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, _ = jittered(c, seed=19)
+    out = {}
+    for mode in (_capi.WD_BINNING_SCATTER, _capi.WD_BINNING_BUCKETS):
+        with ParticleDynamics(c) as sim:
+            sim.set_binning(mode)
+            sim.set_state(st)
+            sim.take_step(40)
+            assert sim.binning() == mode
+            out[mode] = (sim.state(), sim.history())
+    np.testing.assert_array_equal(out[1][0], out[2][0])
+    assert_springs_equal(out[1][1], out[2][1])
+    with ParticleDynamics(c) as tuned:
+        tuned.set_state(st)
+        tuned.take_step(40)  # the tuning steps alternate the two binnings
+        assert tuned.binning() in (_capi.WD_BINNING_SCATTER, _capi.WD_BINNING_BUCKETS)
+        np.testing.assert_array_equal(tuned.state(), out[1][0])
+        with pytest.raises(_capi.WdError, match="mode must be"):
+            tuned.set_binning(3)
+
+
+@pytest.mark.parametrize("mode", ["scatter", "buckets"])
+def test_dense_cells_spill_and_match_oracle(oracle_lib, mode):
+    """Cells of up to ~20 grains: with bucket binning the grains past a
+    cell's eighth go to the spill list; both binnings give the oracle's
+    cell order and states over several steps."""
+    c = history_cfg(workloads.column(4, 4, 5, friction=0.5))
+    c.particles_per_cell = 128
+    st, mat = _capi.initial_state(c)
+    st = st.reshape(-1, 6).copy()
+    rng = np.random.default_rng(5)
+    r = c.particle_radius
+    st[:, :3] = (rng.uniform(-1.6 * r, 1.6 * r, (len(st), 3)) + [0.0, 0.0, 0.3]).astype(np.float32)
+    st[:, 3:] = rng.normal(0, 0.05, (len(st), 3)).astype(np.float32)
+    ora = oracle_lib.Sim(c, st, mat)
+    ora.steps(6, threads=1)
+    want = ora.state()
+    ora.close()
+    with ParticleDynamics(c) as sim:
+        sim.set_binning(_capi.WD_BINNING_SCATTER if mode == "scatter" else _capi.WD_BINNING_BUCKETS)
+        sim.set_state(st)
+        ids, keys = sim.cell_order()
+        assert np.bincount(keys.astype(np.int64)).max() > 8  # cells past a bucket
+        oid, okeys, _ = oracle_lib.cell_order(c, st)
+        np.testing.assert_array_equal(ids, oid)
+        sim.take_step(6)
+        np.testing.assert_array_equal(sim.state(), want)
+
+
 def coarse_cfg(c, scale):
     """c with the collision grid the cell scale gives (floor(G / scale) cells
     per axis)."""

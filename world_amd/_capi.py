@@ -17,6 +17,7 @@ WD_MAX_MATERIALS = 16
 WD_PATH_MAX = 512
 WD_HISTORY_SLOTS = 24
 WD_NOALLOC_MAX_GRAINS = 1 << 20
+WD_BINNING_TUNED, WD_BINNING_SCATTER, WD_BINNING_BUCKETS = 0, 1, 2
 
 WD_INIT_CUBE, WD_INIT_TWO_PARTICLES, WD_INIT_SINGLE_PARTICLE, WD_INIT_FILE = range(4)
 WD_SEMI_IMPLICIT_EULER, WD_BACKWARD_EULER_PICARD = 0, 1
@@ -133,6 +134,8 @@ _SIGNATURES = [
     ("wd_get_cell_scale", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
     ("wd_tune_cell_scale", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
     ("wd_get_scan_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("wd_set_binning", C.c_int, [C.c_void_p, C.c_int]),
+    ("wd_get_binning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_get_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

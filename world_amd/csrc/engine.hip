@@ -83,6 +83,7 @@ struct wd_sim {
     uint2* spill = nullptr;
     bool bucket_ok = false;
     bool bucket_bin = false;
+    bool bin_fixed = false;       // set by wd_set_binning
     double bin_ms[2] = {};        // last tuning's bin time (count, scatter, reorder) per mode
     unsigned long long* status = nullptr;
     size_t scan_cells = 0, ntiles_max = 0;  // cells allocated (whole largest tiles); most tiles
@@ -560,8 +561,8 @@ struct wd_sim {
             if (!scan_fixed) scan_var = sv;
             // and the binning, alternating (with buckets on odd steps)
             const int bv = t.scans % 2;
-            const bool bin_timed = bucket_ok && (t.scans >= 2 || !t.first);
-            if (bucket_ok) bucket_bin = bv == 1;
+            const bool bin_timed = bucket_ok && !bin_fixed && (t.scans >= 2 || !t.first);
+            if (bucket_ok && !bin_fixed) bucket_bin = bv == 1;
             ++t.scans;
             std::vector<ProfRec> recs;
             prof = &recs;
@@ -607,7 +608,7 @@ struct wd_sim {
             for (int v = 0; v < 3; ++v) scan_ms[v] = t.scan_best[v];
             scan_var = sb;
         }
-        if (bucket_ok) {
+        if (bucket_ok && !bin_fixed) {
             for (int v = 0; v < 2; ++v) bin_ms[v] = t.bin_best[v];
             const int cur = t.bin_before ? 1 : 0;
             const int other = 1 - cur;
@@ -2098,6 +2099,31 @@ extern "C" int wd_get_scan_tile(const wd_sim* s, int* cells) {
     WD_CHECK_SIM(s);
     if (!cells) return fail(WD_ERR_ARG, "wd_get_scan_tile: null output");
     *cells = kScanThreads * kScanItemsVariants[s->scan_var];
+    return WD_OK;
+}
+
+extern "C" int wd_set_binning(wd_sim* s, int mode) {
+    WD_CHECK_SIM(s);
+    if (mode < WD_BINNING_TUNED || mode > WD_BINNING_BUCKETS) {
+        return fail(WD_ERR_ARG, "wd_set_binning: mode must be 0 (tuned), 1 (scatter) or 2 (buckets)");
+    }
+    if (mode == WD_BINNING_BUCKETS && !s->bucket_ok) {
+        return fail(WD_ERR_ARG, "wd_set_binning: this handle's buckets would exceed 1 GiB and were not allocated");
+    }
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->bin_fixed = mode != WD_BINNING_TUNED;
+    const bool want = mode == WD_BINNING_BUCKETS;
+    if (s->bin_fixed && want != s->bucket_bin) {
+        s->bucket_bin = want;
+        s->drop_graphs();
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_get_binning(const wd_sim* s, int* mode) {
+    WD_CHECK_SIM(s);
+    if (!mode) return fail(WD_ERR_ARG, "wd_get_binning: null output");
+    *mode = s->bucket_bin ? WD_BINNING_BUCKETS : WD_BINNING_SCATTER;
     return WD_OK;
 }
 

@@ -271,6 +271,16 @@ class ParticleDynamics:
         check(lib().wd_get_scan_tile(self._h, C.byref(v)))
         return v.value
 
+    def set_binning(self, mode: int) -> None:
+        """0: tuned online; 1: scatter pass; 2: per-cell buckets (same cell
+        order and results either way)."""
+        check(lib().wd_set_binning(self._h, int(mode)))
+
+    def binning(self) -> int:
+        v = C.c_int(0)
+        check(lib().wd_get_binning(self._h, C.byref(v)))
+        return v.value
+
     def set_cell_scale(self, scale: float) -> None:
         """Opt-in coarser collision cells (1: the configured grid)."""
         check(lib().wd_set_cell_scale(self._h, C.c_float(scale)))
