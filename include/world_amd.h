@@ -340,6 +340,23 @@ int wd_get_scan_tile(const wd_sim* sim, int* cells_out);
 int wd_set_binning(wd_sim* sim, int mode);
 int wd_get_binning(const wd_sim* sim, int* mode_out); /* WD_BINNING_SCATTER or _BUCKETS */
 
+/* Fused steps (on by default for a single simulation; ensembles and the
+ * kernel-timing mode run the classic sequence).  A step is then three
+ * launches (four with scatter binning) instead of six: the force pass also
+ * counts its grains into the next step's cells, on a grid of the last
+ * occupied box widened by one cell per side; the cell scan's last block
+ * reduces the pass's block partials (box, max |a|, energy, contacts) while
+ * the others scan; the reorder completes the bin.  A grain that moves more
+ * than one cell in a step is binned exactly by a one-block fallback inside
+ * the scan; wd_fused_misses counts those steps.  Every result is the same bit
+ * for bit either way; 0 selects the classic sequence.  The reference runs
+ * its neighbour kernels, force kernel and integrator as separate launches
+ * with a synchronization after each (src/particle_dynamics.cu:264-281,
+ * src/find_neighbors_kernel.cu:190-223). */
+int wd_set_fused_steps(wd_sim* sim, int on);
+int wd_get_fused_steps(const wd_sim* sim, int* on_out); /* 1 when steps run fused */
+int wd_fused_misses(wd_sim* sim, unsigned long long* misses_out);
+
 /* Extension, opt-in: collision cells coarser than the configured grid (the
  * reference sizes cells from the config, collision_grid_size_*,
  * src/sim_config.h:64-200).  Every axis gets floor(G / scale) cells, so a

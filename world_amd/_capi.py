@@ -136,6 +136,9 @@ _SIGNATURES = [
     ("wd_get_scan_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_set_binning", C.c_int, [C.c_void_p, C.c_int]),
     ("wd_get_binning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("wd_set_fused_steps", C.c_int, [C.c_void_p, C.c_int]),
+    ("wd_get_fused_steps", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("wd_fused_misses", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong)]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("wd_rhs", C.c_int, [C.c_void_p, C.c_void_p]),
     ("wd_get_history", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

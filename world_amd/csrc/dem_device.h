@@ -62,6 +62,15 @@ struct Params {
                            // tables below serve every grain)
     unsigned long long* movf;  // [members] over-capacity cells, summed over passes
     BlockMembers* pmem;    // member part of the block partials (written when members > 1)
+    // Fused step of a lone simulation: the grid the force pass counts into
+    // (pending: the last reduced box widened by one cell per side, written by
+    // the reduction of the partials) and the grid the scan and reorder bin
+    // with (next: the pending grid as the force pass found it; the reorder
+    // makes it current).  Layouts as mbox / mcell.
+    int* pbox;
+    uint32_t* pcell;
+    int* nbox;
+    uint32_t* ncell;
     float mass[kMaxMat];
     float c_wall[kMaxMat];
     float c_pair[kMaxMat * kMaxMat];

@@ -97,6 +97,19 @@ struct wd_sim {
     int* d_mgoff = nullptr;
     int* d_mbox = nullptr;
     uint32_t* d_mcell = nullptr;
+    // Fused steps (a lone simulation): the pending and next grids of the
+    // force pass's count (6 box ints and 2 cell offsets each).
+    int* d_pbox = nullptr;
+    uint32_t* d_pcell = nullptr;
+    int* d_nbox = nullptr;
+    uint32_t* d_ncell = nullptr;
+    // A fused step is [force pass that also counts its grains into the
+    // pending grid, cell scan that also reduces the pass's partials,
+    // (scatter,) reorder]: three or four launches instead of six.  It needs
+    // the state binned with the current grid on entry and leaves it so;
+    // fused_ready records that (a classic bin restores it).
+    bool fused_on = true;         // wd_set_fused_steps
+    bool fused_ready = false;
     float* d_mmu = nullptr;
     unsigned* d_mscale = nullptr;
     float* d_mtab = nullptr;                 // per-member material tables (members > 1)
@@ -223,7 +236,7 @@ struct wd_sim {
                         Hc[1], key, slot, count,
                         start, tmp_id, bucket, spill, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
-                        hidx[0], hidx[1], d_mtab, d_movf, pmem};
+                        hidx[0], hidx[1], d_mtab, d_movf, pmem, d_pbox, d_pcell, d_nbox, d_ncell};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -238,6 +251,7 @@ struct wd_sim {
         cic_acc = nullptr; cic_out = nullptr; rhs_buf = nullptr; cic_log = nullptr;
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
         d_mscale = nullptr; d_mtab = nullptr; d_movf = nullptr;
+        d_pbox = nullptr; d_pcell = nullptr; d_nbox = nullptr; d_ncell = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -303,36 +317,62 @@ struct wd_sim {
     }
 
     // One neighbor pass: T (any order) -> S (cell order).  NMODE as k_reorder.
+    // fin_flags >= 0: a fused step's bin, whose count the force pass made
+    // (into the pending grid) and whose scan also reduces that pass's
+    // partials with these k_finalize flags (fin_parts partials of fin_gpp
+    // grains).
+    // step_bin: a classic bin that starts a step (or primes fused steps): it
+    // replaces a fused step's bin of the same state, whose pending
+    // over-capacity count k_count then clears.
     void enqueue_bin(int nmode, const float4* nin_p, const float4* nin_v, float4* nout_p,
-                     float4* nout_v) {
+                     float4* nout_v, int fin_flags = -1, int fin_parts = 0, int fin_gpp = 0,
+                     bool step_bin = false) {
+        const bool fused = fin_flags >= 0;
         // The per-grain bin passes run one-wave blocks: at 32K grains they
         // then spread over every CU, at 1M they are ~2% faster than 256-thread
         // blocks.
         const int bbs = 64;
         const int gb = grain_blocks(n, bbs);
-        if (bucket_bin) {
-            go("k_count", k_count<true>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
-               (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
-               spill);
-        } else {
-            go("k_count", k_count<false>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
-               (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
-               spill);
+        if (!fused) {
+            if (bucket_bin) {
+                go("k_count", k_count<true>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
+                   (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
+                   spill, step_bin ? 1 : 0);
+            } else {
+                go("k_count", k_count<false>, dim3(gb), dim3(bbs), (const Params*)dp, ctl, (const float4*)Tp,
+                   (const float4*)Tv, (const uint32_t*)key, slot, count, n, bin_box_out, bin_cell_out, bucket,
+                   spill, step_bin ? 1 : 0);
+            }
         }
         // Tiles of the chosen size covering every cell the handle can hold,
         // on up to 1,024 blocks; blocks past the current box's tiles return.
+        // A fused step's scan has one more block, which reduces the partials.
         const size_t tile = static_cast<size_t>(kScanThreads) * kScanItemsVariants[scan_var];
         const int scan_grid = static_cast<int>(std::min<size_t>((scan_cells_used + tile - 1) / tile, 1024));
+        ScanFuse fz{};
+        if (fused) {
+            fz.on = 1;
+            fz.parts = parts;
+            fz.nparts = fin_parts;
+            fz.ewave = ewave;
+            fz.energy_log = energy_log;
+            fz.flags = fin_flags;
+            fz.bio = bin_io();
+            fz.key = key;
+            fz.Tp = Tp;
+            fz.n = n;
+            (void)fin_gpp;
+        }
         auto scan = [&](auto kern) {
-            go("k_scan", kern, dim3(scan_grid), dim3(kScanThreads), (const Params*)dp, ctl, count, start,
-               status);
+            go("k_scan", kern, dim3(scan_grid + (fused ? 1 : 0)), dim3(kScanThreads), (const Params*)dp, ctl,
+               count, start, status, fz);
         };
         if (scan_var == 0) scan(k_scan<4>);
         else if (scan_var == 1) scan(k_scan<8>);
         else scan(k_scan<16>);
         if (!bucket_bin) {
             go("k_scatter", k_scatter, dim3(gb), dim3(bbs), (const Params*)dp, (const uint32_t*)key,
-               (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n);
+               (const uint32_t*)slot, (const float4*)Tp, (const float4*)Tv, start, tmp_id, n, fused ? 1 : 0);
         }
         // Picard iterations after the first carry the spring index of the
         // step's first bin through every re-sort (nmode 2).
@@ -340,9 +380,9 @@ struct wd_sim {
         const Carry carry{nin_p, nin_v, nout_p, nout_v, hidx[hidx_cur], history() ? hidx[hout] : nullptr};
         hidx_cur = hout;
         auto reorder = [&](auto kern) {
-            go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, (const Ctl*)ctl,
+            go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, ctl,
                (const uint32_t*)key, (const uint32_t*)start, (const int*)tmp_id, (const uint32_t*)bucket,
-               (const uint2*)spill, (const float4*)Tp, (const float4*)Tv, Sp, Sv, carry, n);
+               (const uint2*)spill, (const float4*)Tp, (const float4*)Tv, Sp, Sv, carry, n, fused ? 1 : 0);
         };
         if (bucket_bin) {
             if (nmode == 0) reorder(k_reorder<0, true>);
@@ -355,22 +395,27 @@ struct wd_sim {
         }
     }
 
-    void enqueue_bin_current() { enqueue_bin(0, nullptr, nullptr, nullptr, nullptr); }
+    void enqueue_bin_current(bool step_bin = false) {
+        enqueue_bin(0, nullptr, nullptr, nullptr, nullptr, -1, 0, 0, step_bin);
+    }
+
+    // The count a fused step's force pass makes for the next bin.
+    BinIO bin_io() const { return BinIO{count, slot, bucket, spill, bucket_bin ? 2 : 1}; }
 
     template <int MODE, bool ENERGY, int BS>
-    void launch_force_bs(const float4* np, const float4* nv, float* rhs) {
+    void launch_force_bs(const float4* np, const float4* nv, float* rhs, BinIO bio) {
         const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}, hidx[hidx_cur]};
         auto force = [&](auto kern) {
             go("k_force", kern, dim3(grain_blocks(n, BS)), dim3(BS), (const Params*)dp, ctl,
                (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, rhs,
-               parts, ewave, hio, n);
+               parts, ewave, hio, n, bio);
         };
         if (history()) force(k_force<MODE, ENERGY, true, BS>);
         else force(k_force<MODE, ENERGY, false, BS>);
     }
 
     template <int MODE, bool ENERGY, int L>
-    void launch_force_q(const float4* np, const float4* nv) {
+    void launch_force_q(const float4* np, const float4* nv, BinIO bio) {
         const HistoryIO hio{{H[0], H[1]}, {Hc[0], Hc[1]}, hidx[hidx_cur]};
         // One lane per grain keeps the damping table of the handle's
         // materials in dynamic LDS, read when there is no per-member table.
@@ -379,7 +424,7 @@ struct wd_sim {
         auto force = [&](auto kern) {
             go_lds("k_force", kern, dim3(grain_blocks(n, 64 / L)), dim3(64), lds, (const Params*)dp, ctl,
                    (const float4*)Sp, (const float4*)Sv, np, nv, (const uint32_t*)start, Tp, Tv, key, parts,
-                   ewave, hio, n);
+                   ewave, hio, n, bio);
         };
         if (history()) force(k_force_q<MODE, ENERGY, true, L>);
         else force(k_force_q<MODE, ENERGY, false, L>);
@@ -390,20 +435,26 @@ struct wd_sim {
     ForceVariant variant() const { return kForceVariants[force_var < 0 ? 2 : force_var]; }
 
     // last_pass: the pass completes a step (its springs become current).
+    // fused: the pass counts its grains for the next bin and its partials are
+    // reduced by that bin's scan: the k_finalize flags are returned in
+    // *fin (with the partial count and grains per partial), nothing is
+    // launched for them here.
     template <int MODE, bool ENERGY>
-    void launch_force(const float4* np, const float4* nv, float* rhs, bool last_pass = true) {
+    void launch_force(const float4* np, const float4* nv, float* rhs, bool last_pass = true,
+                      bool fused = false, int* fin = nullptr) {
         const ForceVariant fv = MODE == kModeRHS ? ForceVariant{kBlock, 0} : variant();
         const int bs = variant_grains(fv);
         const int gb = grain_blocks(n, bs);
+        const BinIO bio = fused && MODE != kModeRHS ? bin_io() : BinIO{nullptr, nullptr, nullptr, nullptr, 0};
         if constexpr (MODE == kModeRHS) {
-            launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
+            launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs, bio);
         } else {
-            if (fv.lanes == 1) launch_force_q<MODE, ENERGY, 1>(np, nv);
-            else if (fv.lanes == 2) launch_force_q<MODE, ENERGY, 2>(np, nv);
-            else if (fv.lanes == 4) launch_force_q<MODE, ENERGY, 4>(np, nv);
-            else if (fv.block == 64) launch_force_bs<MODE, ENERGY, 64>(np, nv, rhs);
-            else if (fv.block == 128) launch_force_bs<MODE, ENERGY, 128>(np, nv, rhs);
-            else launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs);
+            if (fv.lanes == 1) launch_force_q<MODE, ENERGY, 1>(np, nv, bio);
+            else if (fv.lanes == 2) launch_force_q<MODE, ENERGY, 2>(np, nv, bio);
+            else if (fv.lanes == 4) launch_force_q<MODE, ENERGY, 4>(np, nv, bio);
+            else if (fv.block == 64) launch_force_bs<MODE, ENERGY, 64>(np, nv, rhs, bio);
+            else if (fv.block == 128) launch_force_bs<MODE, ENERGY, 128>(np, nv, rhs, bio);
+            else launch_force_bs<MODE, ENERGY, 256>(np, nv, rhs, bio);
         }
         int flags = 0;
         if (MODE != kModeRHS) {
@@ -411,6 +462,12 @@ struct wd_sim {
             if (history() && last_pass) flags |= kFinFlipHistory;
         } else if (ENERGY) {
             flags = kFinEnergyLast;
+        }
+        if (fused && MODE != kModeRHS) {
+            fin[0] = flags | kFinPending;
+            fin[1] = gb;
+            fin[2] = bs;
+            return;
         }
         if (flags) finalize(gb, flags, bs);
     }
@@ -444,11 +501,13 @@ struct wd_sim {
            (const unsigned long long*)cic_acc, out, cic_count, (const unsigned*)next);
     }
 
+    // One step the classic way: bin the state (count, scan, scatter,
+    // reorder), then the force pass and the reduction of its partials.
     void enqueue_step() {
         if (n == 0) return;
         if (!picard()) {
             hipEvent_t a = stage_begin();
-            enqueue_bin_current();
+            enqueue_bin_current(true);
             stage_end(WD_STAGE_FIND_NEIGHBORS, a);
             a = stage_begin();
             if (energy_on) launch_force<kModeSIE, true>(nullptr, nullptr, nullptr);
@@ -463,7 +522,7 @@ struct wd_sim {
             const int b = k & 1, pb = (k - 1) & 1;
             const bool last = k + 1 == cfg.picard_iterations;
             hipEvent_t a = stage_begin();
-            if (k == 0) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0]);
+            if (k == 0) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], -1, 0, 0, true);
             else enqueue_bin(2, NP[pb], NV[pb], NP[b], NV[b]);
             stage_end(WD_STAGE_FIND_NEIGHBORS, a);
             a = stage_begin();
@@ -471,6 +530,49 @@ struct wd_sim {
             else launch_force<kModeBEP, false>(NP[b], NV[b], nullptr, last);
             stage_end(WD_STAGE_COMPUTE_RHS, a);
         }
+    }
+
+    // Fused steps: a lone simulation, stepping through graphs or the tuner
+    // (the kernel-timing mode keeps the classic order of its stages).
+    bool use_fused() const { return fused_on && members == 1 && n > 0 && !timing; }
+
+    // Makes the state binned with the current grid, as a fused step expects
+    // on entry (for Picard, with x^n in the base buffer NP[0]).
+    void prime_fused() {
+        if (fused_ready) return;
+        if (picard()) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], -1, 0, 0, true);
+        else enqueue_bin_current(true);
+        fused_ready = true;
+    }
+
+    // One fused step from a binned state: the force pass on S counts its
+    // output T into the pending grid; T is binned by scan (+ the reduction of
+    // the pass's partials) and reorder.  Picard: every iteration is such a
+    // pass and bin; the bins carry x^n along (the last one copies the new
+    // state into NP[0], the next step's base).
+    void enqueue_step_fused() {
+        if (n == 0) return;
+        int fin[3] = {0, 0, 0};
+        if (!picard()) {
+            if (energy_on) launch_force<kModeSIE, true>(nullptr, nullptr, nullptr, true, true, fin);
+            else launch_force<kModeSIE, false>(nullptr, nullptr, nullptr, true, true, fin);
+            enqueue_bin(0, nullptr, nullptr, nullptr, nullptr, fin[0], fin[1], fin[2]);
+            return;
+        }
+        for (int k = 0; k < cfg.picard_iterations; ++k) {
+            const int b = k & 1;
+            const bool last = k + 1 == cfg.picard_iterations;
+            if (energy_on && k == 0) launch_force<kModeBEP, true>(NP[b], NV[b], nullptr, last, true, fin);
+            else launch_force<kModeBEP, false>(NP[b], NV[b], nullptr, last, true, fin);
+            if (last) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], fin[0], fin[1], fin[2]);
+            else enqueue_bin(2, NP[b], NV[b], NP[b ^ 1], NV[b ^ 1], fin[0], fin[1], fin[2]);
+        }
+    }
+
+    // The step the handle runs now (fused steps need a primed state).
+    void enqueue_any_step() {
+        if (use_fused()) enqueue_step_fused();
+        else enqueue_step();
     }
 
     void drop_graphs() {
@@ -483,7 +585,7 @@ struct wd_sim {
     int capture(int steps, hipGraphExec_t* out) {
         hipGraph_t g = nullptr;
         WD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < steps; ++s) enqueue_step();
+        for (int s = 0; s < steps; ++s) enqueue_any_step();
         WD_HIP(hipStreamEndCapture(stream, &g));
         const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -516,6 +618,15 @@ struct wd_sim {
         double best[kCand] = {};
         double scan_best[3] = {};
         double bin_best[2] = {};
+        // Fused steps: the force pass also counts for the bin (bucket or slot
+        // writes) and the scan also reduces the pass's partials, so the
+        // variant, scan tile and binning are not timed apart.  The variants
+        // are timed with the tile and binning held (whole steps), then the
+        // six (tile, binning) pairs with the chosen variant: `combos` more
+        // steps (two passes the first time, the first warming up).
+        bool fused = false;
+        int combos = 0;
+        double combo_best[6] = {};
     } tuning;
 
     void begin_tuning() {
@@ -537,6 +648,10 @@ struct wd_sim {
         t.scans = 0;
         for (int v = 0; v < 3; ++v) t.scan_best[v] = 1e30;
         for (int v = 0; v < 2; ++v) t.bin_best[v] = 1e30;
+        t.fused = use_fused();
+        const bool pairs = !scan_fixed || (bucket_ok && !bin_fixed);
+        t.combos = t.fused && pairs ? (t.first ? 12 : 6) : 0;
+        for (int v = 0; v < 6; ++v) t.combo_best[v] = 1e30;
         t.pos = 0;
     }
 
@@ -548,34 +663,22 @@ struct wd_sim {
         }
         Tuning& t = tuning;
         const int total = t.rounds * kCand;
-        while (nsteps > 0 && t.pos < total) {
-            const int r = t.pos / kCand, c = t.pos % kCand;
-            ++t.pos;
-            if (!t.cand[c]) continue;
-            force_var = c;
-            // The cell scan's tile size is tuned on the same steps, cycling
-            // independently of the force variant (its time does not depend
-            // on it); its first round warms up as the force pass's does.
-            const int sv = t.scans % 3;
-            const bool scan_timed = !scan_fixed && (t.scans >= 3 || !t.first);
-            if (!scan_fixed) scan_var = sv;
-            // and the binning, alternating (with buckets on odd steps)
-            const int bv = t.scans % 2;
-            const bool bin_timed = bucket_ok && !bin_fixed && (t.scans >= 2 || !t.first);
-            if (bucket_ok && !bin_fixed) bucket_bin = bv == 1;
-            ++t.scans;
+        // One eager step with kernel-stamped events; returns the times of
+        // the force pass (+ k_finalize), the scan and the bin passes.
+        auto timed_step = [&](double& force, double& scan, double& bin) -> int {
             std::vector<ProfRec> recs;
+            if (use_fused()) prime_fused();
             prof = &recs;
-            enqueue_step();
+            enqueue_any_step();
             prof = nullptr;
             WD_HIP(hipStreamSynchronize(stream));
-            double ms = 0.0, scan = 0.0, bin = 0.0;
+            force = scan = bin = 0.0;
             for (auto& rec : recs) {
                 float e = 0.f;
                 const bool ok = hipEventElapsedTime(&e, rec.a, rec.b) == hipSuccess;
                 // the force pass and the reduction of its block partials
                 if (ok && (std::strcmp(rec.name, "k_force") == 0 || std::strcmp(rec.name, "k_finalize") == 0)) {
-                    ms += e;
+                    force += e;
                 }
                 if (ok && std::strcmp(rec.name, "k_scan") == 0) scan += e;
                 if (ok && (std::strcmp(rec.name, "k_count") == 0 || std::strcmp(rec.name, "k_scatter") == 0 ||
@@ -585,22 +688,84 @@ struct wd_sim {
                 spare.push_back(rec.a);
                 spare.push_back(rec.b);
             }
-            if (r > 0 || !t.first) t.best[c] = std::min(t.best[c], ms);
-            if (scan_timed) t.scan_best[sv] = std::min(t.scan_best[sv], scan);
-            if (bin_timed) t.bin_best[bv] = std::min(t.bin_best[bv], bin);
             time += hp.dt;
             --nsteps;
             ++steps_run;
+            return WD_OK;
+        };
+        while (nsteps > 0 && t.pos < total) {
+            const int r = t.pos / kCand, c = t.pos % kCand;
+            ++t.pos;
+            if (!t.cand[c]) continue;
+            force_var = c;
+            int sv = scan_var, bv = bucket_bin ? 1 : 0;
+            bool scan_timed = false, bin_timed = false;
+            if (!t.fused) {
+                // The cell scan's tile size is tuned on the same steps, cycling
+                // independently of the force variant (its time does not depend
+                // on it); its first round warms up as the force pass's does.
+                sv = t.scans % 3;
+                scan_timed = !scan_fixed && (t.scans >= 3 || !t.first);
+                if (!scan_fixed) scan_var = sv;
+                // and the binning, alternating (with buckets on odd steps)
+                bv = t.scans % 2;
+                bin_timed = bucket_ok && !bin_fixed && (t.scans >= 2 || !t.first);
+                if (bucket_ok && !bin_fixed) bucket_bin = bv == 1;
+            }
+            ++t.scans;
+            double ms = 0.0, scan = 0.0, bin = 0.0;
+            const int rc = timed_step(ms, scan, bin);
+            if (rc != WD_OK) return rc;
+            if (t.fused) ms += scan + bin;  // whole steps
+            if (r > 0 || !t.first) t.best[c] = std::min(t.best[c], ms);
+            if (scan_timed) t.scan_best[sv] = std::min(t.scan_best[sv], scan);
+            if (bin_timed) t.bin_best[bv] = std::min(t.bin_best[bv], bin);
         }
         while (t.pos < total && !t.cand[t.pos % kCand]) ++t.pos;
         if (t.pos < total) return check_device();  // continues on the next call
-        int best = -1;
-        for (int c = 0; c < kCand; ++c) {
-            if (t.cand[c] && (best < 0 || t.best[c] < t.best[best])) best = c;
+        if (t.pos == total) {
+            int best = -1;
+            for (int c = 0; c < kCand; ++c) {
+                if (t.cand[c] && (best < 0 || t.best[c] < t.best[best])) best = c;
+            }
+            for (int c = 0; c < kCand; ++c) tune_ms[c] = t.best[c];
+            force_var = best;
         }
-        for (int c = 0; c < kCand; ++c) tune_ms[c] = t.best[c];
-        force_var = best;
-        if (!scan_fixed) {
+        // fused: the (scan tile, binning) pairs with the chosen variant
+        while (nsteps > 0 && t.pos < total + t.combos) {
+            const int q = t.pos - total, combo = q % 6;
+            ++t.pos;
+            const int sv = combo % 3, bv = combo / 3;
+            if ((scan_fixed && sv != scan_var) || ((!bucket_ok || bin_fixed) && bv != (bucket_bin ? 1 : 0))) {
+                continue;  // a pair the handle cannot take
+            }
+            scan_var = sv;
+            bucket_bin = bv == 1;
+            double ms = 0.0, scan = 0.0, bin = 0.0;
+            const int rc = timed_step(ms, scan, bin);
+            if (rc != WD_OK) return rc;
+            if (!t.first || q >= 6) t.combo_best[combo] = std::min(t.combo_best[combo], ms + scan + bin);
+        }
+        if (t.pos < total + t.combos) return check_device();  // continues on the next call
+        if (t.combos > 0) {
+            int cb = -1;
+            for (int v = 0; v < 6; ++v) {
+                if (t.combo_best[v] < 1e30 && (cb < 0 || t.combo_best[v] < t.combo_best[cb])) cb = v;
+            }
+            if (cb >= 0) {
+                scan_var = cb % 3;
+                bucket_bin = cb / 3 == 1;
+                for (int v = 0; v < 3; ++v) scan_ms[v] = std::min(t.combo_best[v], t.combo_best[v + 3]);
+                for (int v = 0; v < 2; ++v) {
+                    bin_ms[v] = std::min(t.combo_best[3 * v], std::min(t.combo_best[3 * v + 1], t.combo_best[3 * v + 2]));
+                }
+            } else {
+                scan_var = t.scan_before;
+                bucket_bin = t.bin_before;
+            }
+        }
+        const int best = force_var;
+        if (!t.fused && !scan_fixed) {
             int sb = t.scan_before;
             for (int v = 0; v < 3; ++v) {
                 if (t.scan_best[v] < 1e30 && t.scan_best[v] < t.scan_best[sb]) sb = v;
@@ -608,7 +773,7 @@ struct wd_sim {
             for (int v = 0; v < 3; ++v) scan_ms[v] = t.scan_best[v];
             scan_var = sb;
         }
-        if (bucket_ok && !bin_fixed) {
+        if (!t.fused && bucket_ok && !bin_fixed) {
             for (int v = 0; v < 2; ++v) bin_ms[v] = t.bin_best[v];
             const int cur = t.bin_before ? 1 : 0;
             const int other = 1 - cur;
@@ -626,7 +791,9 @@ struct wd_sim {
             const int rc = autotune(nsteps);
             if (rc != WD_OK) return rc;
         }
-        if (n > 0) {
+        if (n > 0 && nsteps > 0) {
+            if (use_fused()) prime_fused();
+            else fused_ready = false;  // a classic step leaves S unbinned
             if (timing) {
                 for (int s = 0; s < nsteps; ++s) enqueue_step();
             } else {
@@ -848,6 +1015,10 @@ int refresh_params(wd_sim* s) {
     s->hp.mgoff = keep.mgoff;
     s->hp.mbox = keep.mbox;
     s->hp.mcell = keep.mcell;
+    s->hp.pbox = keep.pbox;
+    s->hp.pcell = keep.pcell;
+    s->hp.nbox = keep.nbox;
+    s->hp.ncell = keep.ncell;
     s->hp.mmu = keep.mmu;
     s->hp.mscale = keep.mscale;
     s->hp.movf = keep.movf;
@@ -906,6 +1077,12 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->d_mgoff, E + 1));
     WD_TRY(dalloc(&s->d_mbox, 6 * E));
     WD_TRY(dalloc(&s->d_mcell, E + 1));
+    WD_TRY(dalloc(&s->d_pbox, 6));
+    WD_TRY(dalloc(&s->d_pcell, 2));
+    WD_TRY(dalloc(&s->d_nbox, 6));
+    WD_TRY(dalloc(&s->d_ncell, 2));
+    WD_HIP(hipMemset(s->d_pcell, 0, sizeof(uint32_t) * 2));
+    WD_HIP(hipMemset(s->d_ncell, 0, sizeof(uint32_t) * 2));
     WD_TRY(dalloc(&s->d_mmu, E));
     WD_TRY(dalloc(&s->d_mscale, E));
     WD_TRY(dalloc(&s->d_movf, E));
@@ -921,6 +1098,10 @@ int allocate(wd_sim* s) {
     s->hp.mgoff = s->d_mgoff;
     s->hp.mbox = s->d_mbox;
     s->hp.mcell = s->d_mcell;
+    s->hp.pbox = s->d_pbox;
+    s->hp.pcell = s->d_pcell;
+    s->hp.nbox = s->d_nbox;
+    s->hp.ncell = s->d_ncell;
     s->hp.mmu = s->d_mmu;
     s->hp.mscale = s->d_mscale;
     s->hp.movf = s->d_movf;
@@ -940,6 +1121,7 @@ int allocate(wd_sim* s) {
 // Uploads an original-order state (and optionally materials) and rebuilds
 // the working layout.
 int ingest(wd_sim* s, const float* st, const int* mat) {
+    s->fused_ready = false;
     if (s->n == 0) return WD_OK;
     WD_HIP(hipStreamSynchronize(s->stream));
     WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
@@ -1110,6 +1292,7 @@ extern "C" int wd_get_material(wd_sim* s, int* out) {
 extern "C" int wd_set_material(wd_sim* s, const int* in) {
     WD_CHECK_SIM(s);
     if (!in) return fail(WD_ERR_ARG, "wd_set_material: null input");
+    s->fused_ready = false;  // the binned copy holds the old material codes
     for (int i = 0; i < s->n; ++i) {
         if (in[i] < 0 || in[i] >= s->cfg.num_masses) return fail(WD_ERR_ARG, "wd_set_material: material id out of range");
     }
@@ -1476,9 +1659,13 @@ extern "C" int wd_profile_step(wd_sim* s, int nsteps, float* kernel_ms, int capa
     std::vector<wd_sim::ProfRec> recs;
     const bool timing = s->timing;
     s->timing = false;
+    // the steps the handle runs (fused ones from a binned state)
+    const bool fused = s->use_fused();
+    if (fused) s->prime_fused();
     s->prof = &recs;
-    for (int k = 0; k < nsteps; ++k) s->enqueue_step();
+    for (int k = 0; k < nsteps; ++k) s->enqueue_any_step();
     s->prof = nullptr;
+    if (!fused) s->fused_ready = false;
     s->timing = timing;
     for (int k = 0; k < nsteps; ++k) s->time += s->hp.dt;
     rc = s->sync();
@@ -1657,6 +1844,7 @@ void snap_save(wd_sim* s, const wd_sim::Snap& sn) {
 }
 
 void snap_load(wd_sim* s, const wd_sim::Snap& sn) {
+    s->fused_ready = false;
     const int gb = grain_blocks(s->n);
     hipLaunchKernelGGL(k_snap_load, dim3(gb), dim3(kBlock), 0, s->stream, s->dp, s->ctl, s->parts,
                        sn.P, sn.V, sn.H, sn.C, s->Tp, s->Tv, s->key, hist_io(s), s->n);
@@ -2127,6 +2315,33 @@ extern "C" int wd_get_binning(const wd_sim* s, int* mode) {
     return WD_OK;
 }
 
+extern "C" int wd_set_fused_steps(wd_sim* s, int on) {
+    WD_CHECK_SIM(s);
+    if (on != 0 && on != 1) return fail(WD_ERR_ARG, "wd_set_fused_steps: on must be 0 or 1");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    if (s->fused_on != (on == 1)) {
+        s->fused_on = on == 1;
+        s->fused_ready = false;
+        s->drop_graphs();
+    }
+    return WD_OK;
+}
+
+extern "C" int wd_get_fused_steps(const wd_sim* s, int* on) {
+    WD_CHECK_SIM(s);
+    if (!on) return fail(WD_ERR_ARG, "wd_get_fused_steps: null output");
+    *on = s->use_fused() ? 1 : 0;
+    return WD_OK;
+}
+
+extern "C" int wd_fused_misses(wd_sim* s, unsigned long long* misses) {
+    WD_CHECK_SIM(s);
+    if (!misses) return fail(WD_ERR_ARG, "wd_fused_misses: null output");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    WD_HIP(hipMemcpy(misses, &s->ctl->bin_misses, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return WD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Collision cell size (extension): coarser cells than the configured grid.
 
@@ -2134,6 +2349,7 @@ namespace {
 
 int apply_cell_scale(wd_sim* s, float scale) {
     WD_HIP(hipStreamSynchronize(s->stream));
+    s->fused_ready = false;
     s->cell_scale = scale;
     int rc = refresh_params(s);
     if (rc != WD_OK || s->n == 0) return rc;

@@ -42,6 +42,28 @@ struct Ctl {
     // list and flips bin_par, so the reorder reads list bin_par ^ 1.
     unsigned bin_par;
     unsigned spill_n[2];
+    // Fused step (a lone simulation): the force pass counts its grains into
+    // the pending grid and sets bin_miss when one lies outside it; the next
+    // scan then bins the pass's grains itself, on one block, and clears it.
+    int bin_miss;
+    unsigned long long bin_misses;  // steps binned that way
+    // Over-capacity cells of a fused step's bin, which belongs to the next
+    // step: the force pass that uses it adds them to the totals (and a
+    // classic bin that replaces it clears them), so the totals count the
+    // neighbor passes of the steps taken, as in the classic sequence.
+    unsigned long long ovf_next;
+};
+
+// The count of a force pass that bins for the next step (fused step): the
+// grain's slot in its cell of the pending grid goes to slot[p] (mode 1) or its
+// id into the cell's bucket (mode 2; past kBucket, the spill list, as
+// k_count).  mode 0: the pass counts nothing.
+struct BinIO {
+    uint32_t* count;
+    uint32_t* slot;
+    uint32_t* bucket;
+    uint2* spill;
+    int mode;
 };
 
 // Ids a cell's bucket holds (bucket binning): the count pass writes a grain's
@@ -65,6 +87,23 @@ struct BlockMembers {
 };
 static_assert(sizeof(BlockPart) == 32 && sizeof(BlockMembers) == 32, "block partials are 32 B");
 
+// The cell scan of a fused step (on != 0): it scans the next grid, and its
+// last block reduces the partials of the force pass that counted the grains
+// (finalize_single with `flags`) instead of taking tiles; after a miss that
+// block bins the grains (key, Tp) itself and the other blocks return.
+struct ScanFuse {
+    int on;
+    const BlockPart* parts;
+    int nparts;
+    const double* ewave;
+    double* energy_log;
+    int flags;
+    BinIO bio;
+    const uint32_t* key;
+    const float4* Tp;
+    int n;
+};
+
 // What k_finalize publishes.
 enum FinalizeFlags {
     kFinBox = 1,        // member boxes and cell offsets <- reduced boxes
@@ -72,7 +111,8 @@ enum FinalizeFlags {
     kFinEnergyLog = 4,  // append the energy sum to the log
     kFinEnergyLast = 8, // energy_last <- energy sum
     kFinCicScale = 16,  // per-member CIC magnitudes <- reduced maxima
-    kFinFlipHistory = 32  // the step is complete: its springs become current
+    kFinFlipHistory = 32,  // the step is complete: its springs become current
+    kFinPending = 64    // with kFinBox: the box goes to the pending grid only
 };
 
 // Launch geometry helpers: grid rounded up to a multiple of the XCD count,

@@ -575,49 +575,50 @@ __global__ __launch_bounds__(kBlock) void k_rekey(const Params* __restrict__ Pp,
     store_block_part(parts, P.members > 1 ? P.pmem : nullptr, logical_block(), member, lo, hi, 0u);
 }
 
+// Cells of the padded sub-grid over occupied box bx.
+__device__ __forceinline__ uint32_t grid_cells(const int bx[6]) {
+    return static_cast<uint32_t>((bx[3] - bx[0] + 3) * (bx[4] - bx[1] + 3) * (bx[5] - bx[2] + 3));
+}
+
+// The pending grid of a lone simulation: occupied box bx widened by one cell
+// on every side (within the collision grid), so that the grains of the next
+// state can be counted into it before its box is known, unless one moves
+// more than a cell.
+__device__ void store_pending(const Params& P, const int bx[6]) {
+    const int g[3] = {P.gx, P.gy, P.gz};
+    int w[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[k] = max(0, bx[k] - 1);
+        w[3 + k] = min(g[k] - 1, bx[3 + k] + 1);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P.pbox[k] = w[k];
+    P.pcell[0] = 0u;
+    P.pcell[1] = grid_cells(w);
+}
+
+// The reduction of a lone simulation's block partials (defined with the
+// reduction kernel below).
+__device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
+                                const double* __restrict__ ewave, double* __restrict__ energy_log,
+                                int flags, int* box_out = nullptr);
+
 // ---------------------------------------------------------------------------
-// Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
-// box_out / cell_out (null unless a caller keeps the bin of this pass): a
-// copy of the member boxes and cell offsets the pass bins with.  BUCKET:
-// instead of the slot, the grain's id goes into its cell's bucket (slots
-// below kBucket) or, past it, into spill list ctl->bin_par as (cell, id); the
-// reorder then ranks a cell's ids without a scatter pass.
+// The slot of a grain (lane) in cell c: count[c] is its cell's counter, every
+// lane of the wave calls this (valid: the lane carries a grain, c is then its
+// cell; grain i, original id `id`).  Grains arrive in the last step's cell
+// order, so neighbouring lanes often share a cell: a run of equal cells in the
+// wave takes its slots with one atomic (the order of slots inside a cell is
+// free: the reorder ranks a cell's grains by id).  BUCKET: the id goes into
+// the cell's bucket (or, past kBucket, spill list ctl->bin_par), otherwise
+// the slot into slot[i].
 template <bool BUCKET>
-__global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp, Ctl* ctl,
-                                                  const float4* __restrict__ Tp,
-                                                  const float4* __restrict__ Tv,
-                                                  const uint32_t* __restrict__ key,
-                                                  uint32_t* __restrict__ slot,
-                                                  uint32_t* __restrict__ count, int n,
-                                                  int* __restrict__ box_out,
-                                                  uint32_t* __restrict__ cell_out,
-                                                  uint32_t* __restrict__ bucket,
-                                                  uint2* __restrict__ spill) {
-    const Params& P = *Pp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->tile_counter = 0;
-        ctl->epoch = ctl->epoch + 1;
-    }
-    if (blockIdx.x == 0 && box_out) {
-        for (int k = threadIdx.x; k < 6 * P.members; k += blockDim.x) box_out[k] = P.mbox[k];
-        for (int k = threadIdx.x; k <= P.members; k += blockDim.x) cell_out[k] = P.mcell[k];
-    }
-    // Grains arrive in the last step's cell order, so neighbouring lanes
-    // often share a cell: a run of equal cells in the wave takes its slots
-    // with one atomic (the order of slots inside a cell is free: the reorder
-    // ranks a cell's grains by id).
-    const int i = logical_block() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bin_grain(Ctl* ctl, uint32_t* __restrict__ count, uint32_t* __restrict__ slot,
+                                          uint32_t* __restrict__ bucket, uint2* __restrict__ spill, bool valid,
+                                          uint32_t c, int id, int i, int n) {
     const int lane = threadIdx.x & 63;
-    const bool valid = i < n;
-    uint32_t c = 0xFFFFFFFFu;  // no cell
-    int id = 0;
-    if (valid) {
-        if (BUCKET) id = __float_as_int(Tp[i].w);
-        const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
-        int cx, cy, cz;
-        unpack_cell(key[i], cx, cy, cz);
-        c = mc.index(cx, cy, cz);
-    }
+    if (!valid) c = 0xFFFFFFFFu;  // no cell
     // The previous lane's cell (wave_shr:1; lane 0 gets none).
     const uint32_t cprev = static_cast<uint32_t>(
             __builtin_amdgcn_update_dpp(-2, static_cast<int>(c), 0x138, 0xF, 0xF, false));
@@ -645,6 +646,83 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
     }
 }
 
+// Fused step: the count of grain p (new cell nx, ny, nz; `has`: the lane
+// carries one) into the pending grid, from the force pass that produced it
+// (every lane of the wave calls this).  A grain outside the pending grid is
+// not counted; it sets bin_miss, and the next scan bins every grain itself.
+__device__ __forceinline__ void fused_count(const Params& P, Ctl* ctl, const BinIO& bio, bool has, int nx,
+                                            int ny, int nz, int id, int p, int n) {
+    if (bio.mode == 0) return;
+    const int* b = P.pbox;
+    const bool inside = has && nx >= b[0] && nx <= b[3] && ny >= b[1] && ny <= b[4] && nz >= b[2] &&
+                        nz <= b[5];
+    if (has && !inside) ctl->bin_miss = 1;
+    const uint32_t c = inside ? static_cast<uint32_t>(make_subgrid(b).index(nx, ny, nz)) : 0xFFFFFFFFu;
+    if (bio.mode == 2) bin_grain<true>(ctl, bio.count, bio.slot, bio.bucket, bio.spill, inside, c, id, p, n);
+    else bin_grain<false>(ctl, bio.count, bio.slot, bio.bucket, bio.spill, inside, c, id, p, n);
+}
+
+// Fused step, block 0 of the force pass: re-arms the scan (as k_count) and
+// hands the pending grid to the scan and reorder that bin this pass.
+__device__ __forceinline__ void fused_arm(const Params& P, Ctl* ctl) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->tile_counter = 0;
+        ctl->epoch = ctl->epoch + 1;
+        const unsigned long long o = ctl->ovf_next;  // this pass uses the bin they were found in
+        if (o) {
+            ctl->overflow_lifetime = ctl->overflow_lifetime + o;
+            P.movf[0] = P.movf[0] + o;
+            ctl->ovf_next = 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.nbox[k] = P.pbox[k];
+        P.ncell[0] = P.pcell[0];
+        P.ncell[1] = P.pcell[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: slot of each grain in its cell.  Block 0 also re-arms the scan.
+// box_out / cell_out (null unless a caller keeps the bin of this pass): a
+// copy of the member boxes and cell offsets the pass bins with.  BUCKET:
+// instead of the slot, the grain's id goes into its cell's bucket (slots
+// below kBucket) or, past it, into spill list ctl->bin_par as (cell, id); the
+// reorder then ranks a cell's ids without a scatter pass.
+template <bool BUCKET>
+__global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp, Ctl* ctl,
+                                                  const float4* __restrict__ Tp,
+                                                  const float4* __restrict__ Tv,
+                                                  const uint32_t* __restrict__ key,
+                                                  uint32_t* __restrict__ slot,
+                                                  uint32_t* __restrict__ count, int n,
+                                                  int* __restrict__ box_out,
+                                                  uint32_t* __restrict__ cell_out,
+                                                  uint32_t* __restrict__ bucket,
+                                                  uint2* __restrict__ spill, int clear_next) {
+    const Params& P = *Pp;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->tile_counter = 0;
+        ctl->epoch = ctl->epoch + 1;
+        if (clear_next) ctl->ovf_next = 0ull;  // a fused bin this one replaces
+    }
+    if (blockIdx.x == 0 && box_out) {
+        for (int k = threadIdx.x; k < 6 * P.members; k += blockDim.x) box_out[k] = P.mbox[k];
+        for (int k = threadIdx.x; k <= P.members; k += blockDim.x) cell_out[k] = P.mcell[k];
+    }
+    const int i = logical_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t c = 0xFFFFFFFFu;  // no cell
+    int id = 0;
+    if (valid) {
+        if (BUCKET) id = __float_as_int(Tp[i].w);
+        const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
+        int cx, cy, cz;
+        unpack_cell(key[i], cx, cy, cz);
+        c = mc.index(cx, cy, cz);
+    }
+    bin_grain<BUCKET>(ctl, count, slot, bucket, spill, valid, c, id, i, n);
+}
+
 // Pass 2: start[c] = sum of count[0..c) over the concatenated member
 // sub-grids, written up to and including index `cells`; counts are zeroed
 // for the next pass and over-capacity cells are added to the lifetime
@@ -653,15 +731,130 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
 // the grid is claimed by a running block, which reaches it once its current
 // tile (waiting only on lower ones) is done.  So every wait ends, without
 // co-residency limits.
+// Fused step, after the force pass left a grain outside the pending grid:
+// the scan's reduction block bins the pass's grains on its own, as k_count
+// and k_scan would with the grid of their exact box bx: the counts the force
+// pass made (over the old next grid's old_cells + 1 entries) are cleared,
+// the grains counted again (spill list ctl->bin_par, then flipped, as
+// k_scan does), and the cells scanned 16 per thread.  The next grid becomes
+// the exact one; the reorder then proceeds as usual.
+__device__ void fallback_bin(const Params& P, Ctl* ctl, const ScanFuse& fz, uint32_t* __restrict__ count,
+                             uint32_t* __restrict__ start, const int* bx, uint32_t old_cells) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const uint32_t cells = grid_cells(bx);
+    for (uint32_t k = static_cast<uint32_t>(tid); k <= old_cells; k += static_cast<uint32_t>(nt)) count[k] = 0u;
+    const unsigned bp = ctl->bin_par;
+    __syncthreads();  // counts cleared, bin_par read by every thread
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.nbox[k] = bx[k];
+        P.ncell[0] = 0u;
+        P.ncell[1] = cells;
+        ctl->spill_n[0] = 0u;
+        ctl->spill_n[1] = 0u;
+    }
+    __syncthreads();
+    const SubGrid g = make_subgrid(bx);
+    for (int i = tid; i < fz.n; i += nt) {
+        int cx, cy, cz;
+        unpack_cell(fz.key[i], cx, cy, cz);
+        const uint32_t c = static_cast<uint32_t>(g.index(cx, cy, cz));
+        const uint32_t sl = atomicAdd(&count[c], 1u);
+        if (fz.bio.mode == 2) {
+            const uint32_t id = __float_as_uint(fz.Tp[i].w);
+            if (sl < static_cast<uint32_t>(kBucket)) {
+                fz.bio.bucket[static_cast<size_t>(c) * kBucket + sl] = id;
+            } else {
+                const unsigned e = atomicAdd(&ctl->spill_n[bp], 1u);
+                fz.bio.spill[static_cast<size_t>(bp) * fz.n + e] = make_uint2(c, id);
+            }
+        } else {
+            fz.bio.slot[i] = sl;
+        }
+    }
+    __syncthreads();  // every count is in (device atomics, read back below at device scope)
+    if (tid == 0) ctl->bin_par = bp ^ 1u;
+    const uint32_t kcap = static_cast<uint32_t>(P.kcap);
+    const uint32_t total = cells + 1u;
+    uint32_t carry = 0u, over = 0u;
+    for (uint32_t base = 0; base < total; base += static_cast<uint32_t>(nt) * 16u) {
+        const uint32_t i0 = base + static_cast<uint32_t>(tid) * 16u;
+        uint32_t v[16], sum = 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t idx = i0 + static_cast<uint32_t>(k);
+            const uint32_t c = idx < total ? __hip_atomic_load(&count[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                           : 0u;
+            over += c > kcap ? 1u : 0u;
+            v[k] = sum;
+            sum += c;
+        }
+        const uint32_t incl = static_cast<uint32_t>(wave_incl_scan_dpp(static_cast<int>(sum)));
+        if (lane == 63) s_w[wave] = incl;
+        __syncthreads();
+        uint32_t woff = 0u, btot = 0u;
+        for (int w = 0; w < nw; ++w) {
+            if (w < wave) woff += s_w[w];
+            btot += s_w[w];
+        }
+        const uint32_t off = carry + woff + (incl - sum);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t idx = i0 + static_cast<uint32_t>(k);
+            if (idx < total) {
+                start[idx] = off + v[k];
+                const uint32_t c = (k < 15 ? v[k + 1] : sum) - v[k];
+                if (c) count[idx] = 0u;
+            }
+        }
+        carry += btot;
+        __syncthreads();  // s_w is rewritten by the next round
+    }
+    over = wave_sum_u_dpp(over);
+    if (lane == 0 && over) atomicAdd(&ctl->ovf_next, static_cast<unsigned long long>(over));
+    if (tid == 0) ctl->bin_misses = ctl->bin_misses + 1ull;
+}
+
+// The last block of a fused step's scan: the reduction of the force pass's
+// partials (its box goes to the pending grid), then either the spill-list
+// turn k_scan's block 0 makes otherwise, or, after a miss, the whole bin.
+__device__ void scan_reduction_block(const Params& P, Ctl* ctl, uint32_t* __restrict__ count,
+                                     uint32_t* __restrict__ start, const ScanFuse& fz) {
+    __shared__ int s_bx[6];
+    __shared__ uint32_t s_old;
+    const bool miss = ctl->bin_miss != 0;
+    if (threadIdx.x == 0) s_old = P.ncell[1];  // before a miss rewrites the next grid
+    finalize_single(P, ctl, fz.parts, fz.nparts, fz.ewave, fz.energy_log, fz.flags, s_bx);
+    __syncthreads();
+    if (!miss) {
+        if (threadIdx.x == 0) {
+            const unsigned bp = ctl->bin_par;
+            ctl->spill_n[bp ^ 1u] = 0u;
+            ctl->bin_par = bp ^ 1u;
+        }
+        return;
+    }
+    fallback_bin(P, ctl, fz, count, start, s_bx, s_old);
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict__ Pp, Ctl* ctl,
                                                        uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ start,
-                                                       unsigned long long* __restrict__ status) {
+                                                       unsigned long long* __restrict__ status, ScanFuse fz) {
     static_assert(ITEMS % 4 == 0, "tiles are loaded as uint4");
     constexpr int kScanItems = ITEMS;
     constexpr int kScanTile = kScanThreads * ITEMS;
-    const int cells = static_cast<int>(Pp->mcell[Pp->members]);
+    const bool fused = fz.on != 0;
+    const int nb = static_cast<int>(gridDim.x) - (fused ? 1 : 0);  // blocks that take tiles
+    if (fused && static_cast<int>(blockIdx.x) == nb) {
+        scan_reduction_block(*Pp, ctl, count, start, fz);
+        return;
+    }
+    if (fused && ctl->bin_miss) return;  // the reduction block bins alone
+    const int cells = fused ? static_cast<int>(Pp->ncell[1]) : static_cast<int>(Pp->mcell[Pp->members]);
     const int ntiles = (cells + 1 + kScanTile - 1) / kScanTile;
     const unsigned long long tag = static_cast<unsigned long long>(ctl->epoch & 0x3FFFFFFFu) << 34;
     const uint32_t kcap = static_cast<uint32_t>(Pp->kcap);
@@ -670,7 +863,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     __shared__ uint32_t s_over[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (!fused && blockIdx.x == 0 && threadIdx.x == 0) {
         // bucket binning: the next count pass spills into the other list,
         // emptied here (its last reader, the previous reorder, is done)
         const unsigned bp = ctl->bin_par;
@@ -681,7 +874,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     // A block scans tile blockIdx.x first, then (when the grid has fewer
     // blocks than tiles) tiles handed out by the counter past the grid, each
     // claimed once the block is free.
-    const bool more = ntiles > static_cast<int>(gridDim.x);
+    const bool more = ntiles > nb;
     for (int tile = static_cast<int>(blockIdx.x); tile < ntiles;) {
         const size_t base = static_cast<size_t>(tile) * kScanTile + threadIdx.x * kScanItems;
         uint32_t v[kScanItems];
@@ -696,15 +889,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
         for (int k = 0; k < kScanItems; ++k) {
             if (v[k] > kcap) {
                 ++over;
-                // The member owning the cell (rare: over-capacity cells only).
-                const uint32_t cell = static_cast<uint32_t>(base) + k;
-                int lo = 0, hi = Pp->members - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (Pp->mcell[mid] <= cell) lo = mid;
-                    else hi = mid - 1;
+                // The member owning the cell (rare: over-capacity cells only);
+                // a fused step's bin counts when the next force pass uses it.
+                if (!fused) {
+                    const uint32_t cell = static_cast<uint32_t>(base) + k;
+                    int lo = 0, hi = Pp->members - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (Pp->mcell[mid] <= cell) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    atomicAdd(&Pp->movf[lo], 1ull);
                 }
-                atomicAdd(&Pp->movf[lo], 1ull);
             }
             const uint32_t c = v[k];
             v[k] = sum;  // exclusive within the thread
@@ -726,7 +922,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
             over_total += s_over[w];
         }
         if (threadIdx.x == 0 && over_total) {
-            atomicAdd(&ctl->overflow_lifetime, static_cast<unsigned long long>(over_total));
+            atomicAdd(fused ? &ctl->ovf_next : &ctl->overflow_lifetime, static_cast<unsigned long long>(over_total));
         }
         // Look-back by the first wave: lane k reads the status of tile
         // j - k, so one round covers 64 predecessors; the nearest one with
@@ -788,7 +984,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
             if (q.x | q.y | q.z | q.w) cz[k] = make_uint4(0u, 0u, 0u, 0u);
         }
         if (!more) break;
-        if (threadIdx.x == 0) s_tile = static_cast<int>(gridDim.x + atomicAdd(&ctl->tile_counter, 1u));
+        if (threadIdx.x == 0) s_tile = nb + static_cast<int>(atomicAdd(&ctl->tile_counter, 1u));
         __syncthreads();
         tile = s_tile;
         __syncthreads();  // every thread has read the claim before the next one is written
@@ -803,11 +999,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Params* __restrict__ P
                                                     const float4* __restrict__ Tp,
                                                     const float4* __restrict__ Tv,
                                                     const uint32_t* __restrict__ start,
-                                                    int* __restrict__ tmp_id, int n) {
+                                                    int* __restrict__ tmp_id, int n, int fused) {
     const Params& P = *Pp;
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i < n) {
-        const MemberCells mc = member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
+        const MemberCells mc = fused ? MemberCells{make_subgrid(P.nbox), 0u}
+                                     : member_cells(P, P.members == 1 ? 0 : grain_member(Tv[i].w));
         int cx, cy, cz;
         unpack_cell(key[i], cx, cy, cz);
         tmp_id[start[mc.index(cx, cy, cz)] + slot[i]] = __float_as_int(Tp[i].w);
@@ -819,19 +1016,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Params* __restrict__ P
 // and, for a cell of more than kBucket grains, the spill list the count pass
 // filled).  NMODE 1 also copies the grain into the Picard base buffers,
 // NMODE 2 carries the Picard base along the same permutation.
+// fused (a fused step): the cells are the next grid's, which block 0 then
+// makes the current one (no block of this pass reads the current grid), and
+// the force pass's miss flag is cleared.
 template <int NMODE, bool BUCKET>
 __global__ __launch_bounds__(kBlock) void k_reorder(
-        const Params* __restrict__ Pp, const Ctl* __restrict__ ctl, const uint32_t* __restrict__ key,
+        const Params* __restrict__ Pp, Ctl* ctl, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
         const uint32_t* __restrict__ bucket, const uint2* __restrict__ spill,
         const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
-        float4* __restrict__ Sv, Carry carry, int n) {
+        float4* __restrict__ Sv, Carry carry, int n, int fused) {
     const Params& P = *Pp;
+    if (fused && blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) P.mbox[k] = P.nbox[k];
+        P.mcell[0] = P.ncell[0];
+        P.mcell[1] = P.ncell[1];
+        ctl->bin_miss = 0;
+    }
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = Tp[i];
     const float4 v = Tv[i];
-    const MemberCells mc = member_cells(P, member_of(P, v.w));
+    const MemberCells mc = fused ? MemberCells{make_subgrid(P.nbox), 0u} : member_cells(P, member_of(P, v.w));
     int cx, cy, cz;
     unpack_cell(key[i], cx, cy, cz);
     const uint32_t c = mc.index(cx, cy, cz);
@@ -895,8 +1102,9 @@ __global__ __launch_bounds__(BS, 5) void k_force(
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
         float* __restrict__ rhs_out, BlockPart* __restrict__ parts, double* __restrict__ ewave,
-        HistoryIO hist, int n) {
+        HistoryIO hist, int n, BinIO bio) {
     const Params& P = *Pp;
+    if (MODE != kModeRHS && bio.mode) fused_arm(P, ctl);
     __shared__ uint32_t s_list[kContactList][BS];
     __shared__ float s_cpair[kMaxMat * kMaxMat];
     // A lone simulation's pair damping table, in LDS: one lookup per contact.
@@ -912,6 +1120,7 @@ __global__ __launch_bounds__(BS, 5) void k_force(
     unsigned ncont = 0;
     float e_grain = 0.f;
     int lane_member = -1;
+    int fc[3] = {0, 0, 0}, fid = 0;  // the grain's next cell and id (fused count)
     if (p < n) {
         const float4 pi = Sp[p];
         const float4 vi4 = Sv[p];
@@ -1153,6 +1362,10 @@ __global__ __launch_bounds__(BS, 5) void k_force(
             lo[0] = hi[0] = nx;
             lo[1] = hi[1] = ny;
             lo[2] = hi[2] = nz;
+            fc[0] = nx;
+            fc[1] = ny;
+            fc[2] = nz;
+            fid = id;
             float s2 = 0.f;
             s2 += a[0] * a[0];
             s2 += a[1] * a[1];
@@ -1166,6 +1379,7 @@ __global__ __launch_bounds__(BS, 5) void k_force(
         if ((threadIdx.x & (kEnergyGroup - 1)) == 0 && p < n) ewave[p / kEnergyGroup] = ew;
     }
     if (MODE != kModeRHS) {
+        fused_count(P, ctl, bio, p < n, fc[0], fc[1], fc[2], fid, p, n);
         store_block_part(parts, P.members > 1 ? P.pmem : nullptr, lb, lane_member, lo, hi, accel_bits, ncont);
     }
 }
@@ -1196,11 +1410,12 @@ __global__ __launch_bounds__(64) void k_force_q(
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
         float4* __restrict__ Tp, float4* __restrict__ Tv, uint32_t* __restrict__ key_out,
-        BlockPart* __restrict__ parts, double* __restrict__ ewave, HistoryIO hist, int n) {
+        BlockPart* __restrict__ parts, double* __restrict__ ewave, HistoryIO hist, int n, BinIO bio) {
     static_assert(MODE != kModeRHS && (L == 1 || L == 2 || L == 4), "queued force pass variants");
     constexpr int G = 64 / L;  // grains per wave
     constexpr int QCAP = kQueueCap;
     const Params& P = *Pp;
+    if (bio.mode) fused_arm(P, ctl);
     // Per lane: cell-sorted slots of listed contacts; with one lane per
     // grain, row QCAP takes the stores of candidates that are not listed.
     // One lane per grain keeps its LDS small (about 7 KB: five waves per
@@ -1749,6 +1964,7 @@ __global__ __launch_bounds__(64) void k_force_q(
     int lo[3] = {kEmptyMin, kEmptyMin, kEmptyMin}, hi[3] = {kEmptyMax, kEmptyMax, kEmptyMax};
     unsigned accel_bits = 0;
     int lane_member = -1;
+    int fc[3] = {0, 0, 0};  // the grain's next cell (fused count)
     if (live && sub == 0) {
         if (HIST) hist.cnt[par ^ 1][p] = min(grain_slots, kHist);
         const float a[3] = {f[0] / m, f[1] / m, f[2] / m};
@@ -1776,6 +1992,9 @@ __global__ __launch_bounds__(64) void k_force_q(
         lo[0] = hi[0] = nx;
         lo[1] = hi[1] = ny;
         lo[2] = hi[2] = nz;
+        fc[0] = nx;
+        fc[1] = ny;
+        fc[2] = nz;
         float s2 = 0.f;
         s2 += a[0] * a[0];
         s2 += a[1] * a[1];
@@ -1784,6 +2003,7 @@ __global__ __launch_bounds__(64) void k_force_q(
         if (acc == acc) accel_bits = __float_as_uint(acc);
         ncont += static_cast<unsigned>(wc);
     }
+    fused_count(P, ctl, bio, live && sub == 0, fc[0], fc[1], fc[2], id, p, n);
     if (ENERGY) {
         const double ew = group_energy<L>(live ? static_cast<double>(e_grain) : 0.0);
         if (sub == 0 && (g & (kEnergyGroup - 1)) == 0 && live) ewave[p / kEnergyGroup] = ew;
@@ -1844,9 +2064,11 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
 // and the energy groups, one block-wide combine.  The sums run in the order
 // of the general path (per thread strided, then waves in order), so the bits
 // are the same.
+// The reduction of a lone simulation's block partials (thread 0 publishes).
+// box_out (null or shared memory): thread 0 leaves the reduced box there.
 __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
                                 const double* __restrict__ ewave, double* __restrict__ energy_log,
-                                int flags) {
+                                int flags, int* box_out) {
     __shared__ int s_iv[16][8];
     __shared__ double s_dv[16][2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1919,10 +2141,17 @@ __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __re
         }
     }
     if (flags & kFinBox) {
+        if (!(flags & kFinPending)) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) P.mbox[k] = bx[k];
-        P.mcell[0] = 0u;
-        P.mcell[1] = static_cast<uint32_t>((bx[3] - bx[0] + 3) * (bx[4] - bx[1] + 3) * (bx[5] - bx[2] + 3));
+            for (int k = 0; k < 6; ++k) P.mbox[k] = bx[k];
+            P.mcell[0] = 0u;
+            P.mcell[1] = grid_cells(bx);
+        }
+        store_pending(P, bx);
+        if (box_out) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) box_out[k] = bx[k];
+        }
     }
     if (flags & kFinAccel) {
         ctl->accel_last = am;
@@ -2206,17 +2435,17 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
 }
 
 #define WD_INST_REORDER(M, B)                                                                \
-    template __global__ void k_reorder<M, B>(const Params*, const Ctl*, const uint32_t*,          \
+    template __global__ void k_reorder<M, B>(const Params*, Ctl*, const uint32_t*,                \
                                              const uint32_t*, const int*, const uint32_t*,        \
                                              const uint2*, const float4*, const float4*, float4*, \
-                                             float4*, Carry, int);
+                                             float4*, Carry, int, int);
 template __global__ void k_count<false>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
-                                        uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*);
+                                        uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*, int);
 template __global__ void k_count<true>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
-                                       uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*);
-template __global__ void k_scan<4>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
-template __global__ void k_scan<8>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
-template __global__ void k_scan<16>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*);
+                                       uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*, int);
+template __global__ void k_scan<4>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*, ScanFuse);
+template __global__ void k_scan<8>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*, ScanFuse);
+template __global__ void k_scan<16>(const Params*, Ctl*, uint32_t*, uint32_t*, unsigned long long*, ScanFuse);
 WD_INST_REORDER(0, false)
 WD_INST_REORDER(1, false)
 WD_INST_REORDER(2, false)
@@ -2229,7 +2458,7 @@ WD_INST_REORDER(2, true)
     template __global__ void k_force<M, E, H, B>(const Params*, Ctl*, const float4*, const float4*, \
                                                  const float4*, const float4*, const uint32_t*,     \
                                                  float4*, float4*, uint32_t*, float*, BlockPart*,   \
-                                                 double*, HistoryIO, int);
+                                                 double*, HistoryIO, int, BinIO);
 #define WD_INST_FORCE_B(M, E, H) WD_INST_FORCE(M, E, H, 64) WD_INST_FORCE(M, E, H, 128) WD_INST_FORCE(M, E, H, 256)
 #define WD_INST_FORCE_H(M, E) WD_INST_FORCE_B(M, E, false) WD_INST_FORCE_B(M, E, true)
 WD_INST_FORCE_H(kModeSIE, false)
@@ -2244,7 +2473,7 @@ WD_INST_FORCE(kModeRHS, true, true, 256)
     template __global__ void k_force_q<M, E, H, L>(const Params*, Ctl*, const float4*, const float4*, \
                                                    const float4*, const float4*, const uint32_t*,     \
                                                    float4*, float4*, uint32_t*, BlockPart*, double*,  \
-                                                   HistoryIO, int);
+                                                   HistoryIO, int, BinIO);
 #define WD_INST_FORCE_QL(M, E, H) WD_INST_FORCE_Q(M, E, H, 1) WD_INST_FORCE_Q(M, E, H, 2) WD_INST_FORCE_Q(M, E, H, 4)
 WD_INST_FORCE_QL(kModeSIE, false, false)
 WD_INST_FORCE_QL(kModeSIE, false, true)

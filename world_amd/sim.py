@@ -281,6 +281,25 @@ class ParticleDynamics:
         check(lib().wd_get_binning(self._h, C.byref(v)))
         return v.value
 
+    def set_fused_steps(self, on: bool) -> None:
+        """Fused steps (the default for a single simulation): the force pass
+        counts its grains for the next bin and the cell scan reduces its
+        partials, three launches per step instead of six; False selects the
+        classic sequence.  Results are the same either way."""
+        check(lib().wd_set_fused_steps(self._h, 1 if on else 0))
+
+    def fused_steps(self) -> bool:
+        v = C.c_int(0)
+        check(lib().wd_get_fused_steps(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def fused_misses(self) -> int:
+        """Fused steps whose grains were binned by the scan's one-block
+        fallback (a grain moved more than one cell in the step)."""
+        v = C.c_ulonglong(0)
+        check(lib().wd_fused_misses(self._h, C.byref(v)))
+        return v.value
+
     def set_cell_scale(self, scale: float) -> None:
         """Opt-in coarser collision cells (1: the configured grid)."""
         check(lib().wd_set_cell_scale(self._h, C.c_float(scale)))
