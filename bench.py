@@ -71,7 +71,9 @@ B_FORCE = 56.0
 # integrate (read state 24 + material 4, write state 24).
 B_STEP = 132.0
 LEGS = ("main", "runs32k", "large", "ensemble", "adjoint", "batched_adjoint", "sliding", "cpu")
-TUNE_STEPS = 18     # one online tuning round over the six force-pass variants
+# One online tuning: three rounds over the six force-pass variants, then (fused
+# steps) the six (scan tile, binning) pairs twice with the chosen variant.
+TUNE_STEPS = 30
 CAPTURE_STEPS = 31  # 16 + 8 + 4 + 2 + 1: every step graph the engine replays
 
 
@@ -379,7 +381,7 @@ def ensemble_leg(steps: int) -> dict:
 def adjoint_window(make, cot_of, window_steps: int, every: int):
     """Forward, record and backward cost of one window, all over the same
     steps and all with the force variant tuned and pinned and the graphs
-    captured beforehand.  A handle tunes (18 steps), pins, captures the step
+    captured beforehand.  A handle tunes (TUNE_STEPS steps), pins, captures the step
     graphs (31 steps) and runs one window untimed, then times a forward step
     over the next window; a second handle does the same and records and
     differentiates its first window untimed (allocations, adjoint graph

@@ -110,6 +110,7 @@ struct wd_sim {
     // fused_ready records that (a classic bin restores it).
     bool fused_on = true;         // wd_set_fused_steps
     bool fused_ready = false;
+    FinSlice* fin_slices = nullptr;  // the scan's slices of the partials, combined by the reorder
     float* d_mmu = nullptr;
     unsigned* d_mscale = nullptr;
     float* d_mtab = nullptr;                 // per-member material tables (members > 1)
@@ -236,7 +237,8 @@ struct wd_sim {
                         Hc[1], key, slot, count,
                         start, tmp_id, bucket, spill, status, st6, mat_d, parts, energy_log, cic_acc,
                         cic_out, rhs_buf, cic_log, d_mgoff, d_mbox, d_mcell, d_mmu, d_mscale, ewave,
-                        hidx[0], hidx[1], d_mtab, d_movf, pmem, d_pbox, d_pcell, d_nbox, d_ncell};
+                        hidx[0], hidx[1], d_mtab, d_movf, pmem, d_pbox, d_pcell, d_nbox, d_ncell,
+                        fin_slices};
         for (void* b : bufs) {
             if (b) (void)hipFree(b);
         }
@@ -252,6 +254,7 @@ struct wd_sim {
         d_mgoff = nullptr; d_mbox = nullptr; d_mcell = nullptr; d_mmu = nullptr;
         d_mscale = nullptr; d_mtab = nullptr; d_movf = nullptr;
         d_pbox = nullptr; d_pcell = nullptr; d_nbox = nullptr; d_ncell = nullptr;
+        fin_slices = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -346,26 +349,34 @@ struct wd_sim {
         }
         // Tiles of the chosen size covering every cell the handle can hold,
         // on up to 1,024 blocks; blocks past the current box's tiles return.
-        // A fused step's scan has one more block, which reduces the partials.
+        // A fused step's scan has kFinSliceBlocks more blocks, which reduce
+        // the force pass's partials in slices (combined by the reorder).
         const size_t tile = static_cast<size_t>(kScanThreads) * kScanItemsVariants[scan_var];
         const int scan_grid = static_cast<int>(std::min<size_t>((scan_cells_used + tile - 1) / tile, 1024));
         ScanFuse fz{};
+        ReorderFuse rf{};
         if (fused) {
+            const bool with_e = (fin_flags & (kFinEnergyLog | kFinEnergyLast)) != 0;
             fz.on = 1;
             fz.parts = parts;
             fz.nparts = fin_parts;
             fz.ewave = ewave;
-            fz.energy_log = energy_log;
-            fz.flags = fin_flags;
+            fz.ng = (n + kEnergyGroup - 1) / kEnergyGroup;
+            fz.with_e = with_e ? 1 : 0;
+            fz.slices = fin_slices;
             fz.bio = bin_io();
             fz.key = key;
             fz.Tp = Tp;
             fz.n = n;
+            rf.on = 1;
+            rf.slices = fin_slices;
+            rf.energy_log = energy_log;
+            rf.flags = fin_flags;
             (void)fin_gpp;
         }
         auto scan = [&](auto kern) {
-            go("k_scan", kern, dim3(scan_grid + (fused ? 1 : 0)), dim3(kScanThreads), (const Params*)dp, ctl,
-               count, start, status, fz);
+            go("k_scan", kern, dim3(scan_grid + (fused ? kFinSliceBlocks : 0)), dim3(kScanThreads),
+               (const Params*)dp, ctl, count, start, status, fz);
         };
         if (scan_var == 0) scan(k_scan<4>);
         else if (scan_var == 1) scan(k_scan<8>);
@@ -382,7 +393,7 @@ struct wd_sim {
         auto reorder = [&](auto kern) {
             go("k_reorder", kern, dim3(gb), dim3(bbs), (const Params*)dp, ctl,
                (const uint32_t*)key, (const uint32_t*)start, (const int*)tmp_id, (const uint32_t*)bucket,
-               (const uint2*)spill, (const float4*)Tp, (const float4*)Tv, Sp, Sv, carry, n, fused ? 1 : 0);
+               (const uint2*)spill, (const float4*)Tp, (const float4*)Tv, Sp, Sv, carry, n, rf);
         };
         if (bucket_bin) {
             if (nmode == 0) reorder(k_reorder<0, true>);
@@ -1077,6 +1088,7 @@ int allocate(wd_sim* s) {
     WD_TRY(dalloc(&s->d_mgoff, E + 1));
     WD_TRY(dalloc(&s->d_mbox, 6 * E));
     WD_TRY(dalloc(&s->d_mcell, E + 1));
+    WD_TRY(dalloc(&s->fin_slices, kFinSlices));
     WD_TRY(dalloc(&s->d_pbox, 6));
     WD_TRY(dalloc(&s->d_pcell, 2));
     WD_TRY(dalloc(&s->d_nbox, 6));
@@ -2256,6 +2268,11 @@ extern "C" int wd_set_force_variant(wd_sim* s, int block, int lanes) {
                                 "(64|128|256, 0) or (64, 1|2|4)");
     }
     WD_HIP(hipStreamSynchronize(s->stream));
+    if (s->tuning.pos >= 0) {
+        // a tuning cut short: the tile and binning it was timing go back
+        if (!s->scan_fixed) s->scan_var = s->tuning.scan_before;
+        if (!s->bin_fixed) s->bucket_bin = s->tuning.bin_before;
+    }
     s->force_var = v;
     s->force_fixed = v >= 0;
     s->next_tune = s->steps_run;  // (0, 0): tune from the next call on

@@ -87,21 +87,51 @@ struct BlockMembers {
 };
 static_assert(sizeof(BlockPart) == 32 && sizeof(BlockMembers) == 32, "block partials are 32 B");
 
+// The reduction of a lone simulation's block partials runs over kFinSlices
+// fixed slices of the partial and energy-group arrays: one wave reduces a
+// slice (lane l adds the slice's entries l, l + 64, ... in order, then the
+// fixed DPP tree), and one wave combines the slices (lane r holds slice r,
+// the same tree).  The bits of the energy sum therefore do not depend on how
+// many blocks or threads take part: k_finalize runs them on its one block, a
+// fused step on the cell scan's last kFinSliceBlocks blocks, and the
+// reorder's block 0 combines them.
+constexpr int kFinSlices = 64;
+constexpr int kFinSliceBlocks = kFinSlices / (kScanThreads / 64);
+struct FinSlice {
+    int box[6];
+    unsigned accel;
+    unsigned pad;
+    double contacts;
+    double energy;
+};
+
 // The cell scan of a fused step (on != 0): it scans the next grid, and its
-// last block reduces the partials of the force pass that counted the grains
-// (finalize_single with `flags`) instead of taking tiles; after a miss that
-// block bins the grains (key, Tp) itself and the other blocks return.
+// last kFinSliceBlocks blocks reduce the slices of the partials of the force
+// pass that counted the grains instead of taking tiles (energy groups too
+// when with_e); after a miss the last of them also bins the grains (key, Tp)
+// itself and the tile blocks return.
 struct ScanFuse {
     int on;
     const BlockPart* parts;
     int nparts;
     const double* ewave;
-    double* energy_log;
-    int flags;
+    int ng;
+    int with_e;
+    FinSlice* slices;
     BinIO bio;
     const uint32_t* key;
     const float4* Tp;
     int n;
+};
+
+// The reorder of a fused step (on != 0): its block 0 makes the next grid the
+// current one and combines the scan's slices, publishing as k_finalize would
+// with `flags`.
+struct ReorderFuse {
+    int on;
+    const FinSlice* slices;
+    double* energy_log;
+    int flags;
 };
 
 // What k_finalize publishes.

@@ -598,11 +598,127 @@ __device__ void store_pending(const Params& P, const int bx[6]) {
     P.pcell[1] = grid_cells(w);
 }
 
-// The reduction of a lone simulation's block partials (defined with the
-// reduction kernel below).
-__device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
-                                const double* __restrict__ ewave, double* __restrict__ energy_log,
-                                int flags, int* box_out = nullptr);
+// Slice r of a lone simulation's block partials (and, with with_e, of its ng
+// energy groups) reduced by one wave, every lane active: lane l folds the
+// slice's entries l, l + 64, ... in order (loads issued four at a time, the
+// index clamped into the slice so that none is predicated), then the fixed
+// DPP trees.  Lane 0 stores the slice.
+__device__ void slice_reduce(const BlockPart* __restrict__ parts, int nparts, const double* __restrict__ ewave,
+                             int ng, bool with_e, int r, FinSlice* out) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int p0 = static_cast<int>(static_cast<long long>(nparts) * r / kFinSlices);
+    const int p1 = static_cast<int>(static_cast<long long>(nparts) * (r + 1) / kFinSlices);
+    int bx[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
+    unsigned am = 0u;
+    unsigned long long cs = 0ull;
+    for (int b = p0 + lane; b < p1; b += 4 * 64) {
+        BlockPart q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = parts[min(b + 64 * k, p1 - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (b + 64 * k < p1) {
+                am = max(am, q[k].accel);
+                cs += q[k].contacts;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    bx[a] = min(bx[a], q[k].box0[a]);
+                    bx[3 + a] = max(bx[3 + a], q[k].box0[3 + a]);
+                }
+            }
+        }
+    }
+    double e = 0.0;
+    if (with_e) {
+        const int g0 = static_cast<int>(static_cast<long long>(ng) * r / kFinSlices);
+        const int g1 = static_cast<int>(static_cast<long long>(ng) * (r + 1) / kFinSlices);
+        for (int g = g0 + lane; g < g1; g += 4 * 64) {
+            double x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[k] = ewave[min(g + 64 * k, g1 - 1)];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (g + 64 * k < g1) e += x[k];
+            }
+        }
+    }
+    e = wave_sum(e);
+    const double cw = wave_sum(static_cast<double>(cs));  // exact below 2^53
+    am = wave_max_u_dpp(am);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bx[a] = wave_min_dpp(bx[a]);
+        bx[3 + a] = wave_max_dpp(bx[3 + a]);
+    }
+    if (lane == 0) {
+        FinSlice o;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) o.box[a] = bx[a];
+        o.accel = am;
+        o.pad = 0u;
+        o.contacts = cw;
+        o.energy = e;
+        *out = o;
+    }
+}
+
+// The kFinSlices slices combined by one wave (lane r: slice r, every lane
+// active), published by lane 0 as k_finalize's flags ask for a lone
+// simulation; box_out (null or shared memory): lane 0 leaves the box there.
+__device__ void slice_combine(const Params& P, Ctl* ctl, const FinSlice* sl, double* __restrict__ energy_log,
+                              int flags, int* box_out) {
+    static_assert(kFinSlices == 64, "one slice per lane");
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const FinSlice s = sl[lane];
+    int bx[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bx[a] = wave_min_dpp(s.box[a]);
+        bx[3 + a] = wave_max_dpp(s.box[3 + a]);
+    }
+    const unsigned am = wave_max_u_dpp(s.accel);
+    const double contacts = wave_sum(s.contacts);
+    const double total = wave_sum(s.energy);
+    if (lane != 0) return;
+    if (flags & kFinBox) {
+        if (!(flags & kFinPending)) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) P.mbox[k] = bx[k];
+            P.mcell[0] = 0u;
+            P.mcell[1] = grid_cells(bx);
+        }
+        store_pending(P, bx);
+    }
+    if (box_out) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) box_out[k] = bx[k];
+    }
+    if (flags & kFinAccel) {
+        ctl->accel_last = am;
+        ctl->force_passes = ctl->force_passes + 1;
+        ctl->contact_sum = ctl->contact_sum + static_cast<unsigned long long>(contacts);
+    }
+    if (flags & kFinEnergyLast) ctl->energy_last = total;
+    if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
+    if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
+        energy_log[ctl->energy_count] = total;
+        ctl->energy_count = ctl->energy_count + 1;
+    }
+}
+
+// The reduction of a lone simulation's block partials on one block: its waves
+// take the slices in turn, then wave 0 combines them.
+__device__ void slices_on_block(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
+                                const double* __restrict__ ewave, double* __restrict__ energy_log, int flags,
+                                int* box_out) {
+    __shared__ FinSlice s_sl[kFinSlices];
+    const int wave = static_cast<int>(threadIdx.x >> 6), nw = static_cast<int>(blockDim.x >> 6);
+    const bool with_e = (flags & (kFinEnergyLog | kFinEnergyLast)) != 0;
+    const int ng = with_e ? (P.n + kEnergyGroup - 1) / kEnergyGroup : 0;
+    for (int r = wave; r < kFinSlices; r += nw) slice_reduce(parts, nparts, ewave, ng, with_e, r, &s_sl[r]);
+    __syncthreads();
+    if (wave == 0) slice_combine(P, ctl, s_sl, energy_log, flags, box_out);
+}
 
 // ---------------------------------------------------------------------------
 // The slot of a grain (lane) in cell c: count[c] is its cell's counter, every
@@ -739,10 +855,15 @@ __global__ __launch_bounds__(kBlock) void k_count(const Params* __restrict__ Pp,
 // k_scan does), and the cells scanned 16 per thread.  The next grid becomes
 // the exact one; the reorder then proceeds as usual.
 __device__ void fallback_bin(const Params& P, Ctl* ctl, const ScanFuse& fz, uint32_t* __restrict__ count,
-                             uint32_t* __restrict__ start, const int* bx, uint32_t old_cells) {
+                             uint32_t* __restrict__ start) {
     __shared__ uint32_t s_w[kScanThreads / 64];
+    __shared__ int bx[6];
     const int tid = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const uint32_t old_cells = P.ncell[1];  // the grid the force pass counted into
+    // the exact box of the pass's grains (its partials; nothing published)
+    slices_on_block(P, ctl, fz.parts, fz.nparts, nullptr, nullptr, 0, bx);
+    __syncthreads();
     const uint32_t cells = grid_cells(bx);
     for (uint32_t k = static_cast<uint32_t>(tid); k <= old_cells; k += static_cast<uint32_t>(nt)) count[k] = 0u;
     const unsigned bp = ctl->bin_par;
@@ -817,18 +938,16 @@ __device__ void fallback_bin(const Params& P, Ctl* ctl, const ScanFuse& fz, uint
     if (tid == 0) ctl->bin_misses = ctl->bin_misses + 1ull;
 }
 
-// The last block of a fused step's scan: the reduction of the force pass's
-// partials (its box goes to the pending grid), then either the spill-list
-// turn k_scan's block 0 makes otherwise, or, after a miss, the whole bin.
+// A reduction block of a fused step's scan (rb of kFinSliceBlocks): its waves
+// reduce one slice each of the force pass's partials; the last one also
+// takes the spill-list turn k_scan's block 0 makes otherwise or, after a
+// miss, the whole bin.
 __device__ void scan_reduction_block(const Params& P, Ctl* ctl, uint32_t* __restrict__ count,
-                                     uint32_t* __restrict__ start, const ScanFuse& fz) {
-    __shared__ int s_bx[6];
-    __shared__ uint32_t s_old;
-    const bool miss = ctl->bin_miss != 0;
-    if (threadIdx.x == 0) s_old = P.ncell[1];  // before a miss rewrites the next grid
-    finalize_single(P, ctl, fz.parts, fz.nparts, fz.ewave, fz.energy_log, fz.flags, s_bx);
-    __syncthreads();
-    if (!miss) {
+                                     uint32_t* __restrict__ start, const ScanFuse& fz, int rb) {
+    const int r = rb * (kScanThreads / 64) + static_cast<int>(threadIdx.x >> 6);
+    slice_reduce(fz.parts, fz.nparts, fz.ewave, fz.ng, fz.with_e != 0, r, fz.slices + r);
+    if (rb != kFinSliceBlocks - 1) return;
+    if (!ctl->bin_miss) {
         if (threadIdx.x == 0) {
             const unsigned bp = ctl->bin_par;
             ctl->spill_n[bp ^ 1u] = 0u;
@@ -836,7 +955,7 @@ __device__ void scan_reduction_block(const Params& P, Ctl* ctl, uint32_t* __rest
         }
         return;
     }
-    fallback_bin(P, ctl, fz, count, start, s_bx, s_old);
+    fallback_bin(P, ctl, fz, count, start);
 }
 
 template <int ITEMS>
@@ -848,9 +967,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Params* __restrict_
     constexpr int kScanItems = ITEMS;
     constexpr int kScanTile = kScanThreads * ITEMS;
     const bool fused = fz.on != 0;
-    const int nb = static_cast<int>(gridDim.x) - (fused ? 1 : 0);  // blocks that take tiles
-    if (fused && static_cast<int>(blockIdx.x) == nb) {
-        scan_reduction_block(*Pp, ctl, count, start, fz);
+    const int nb = static_cast<int>(gridDim.x) - (fused ? kFinSliceBlocks : 0);  // blocks that take tiles
+    if (fused && static_cast<int>(blockIdx.x) >= nb) {
+        scan_reduction_block(*Pp, ctl, count, start, fz, static_cast<int>(blockIdx.x) - nb);
         return;
     }
     if (fused && ctl->bin_miss) return;  // the reduction block bins alone
@@ -1016,23 +1135,28 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Params* __restrict__ P
 // and, for a cell of more than kBucket grains, the spill list the count pass
 // filled).  NMODE 1 also copies the grain into the Picard base buffers,
 // NMODE 2 carries the Picard base along the same permutation.
-// fused (a fused step): the cells are the next grid's, which block 0 then
-// makes the current one (no block of this pass reads the current grid), and
-// the force pass's miss flag is cleared.
+// rf.on (a fused step): the cells are the next grid's, which block 0 then
+// makes the current one (no block of this pass reads the current grid); it
+// also clears the force pass's miss flag and combines the scan's slices of
+// the pass's partials.
 template <int NMODE, bool BUCKET>
 __global__ __launch_bounds__(kBlock) void k_reorder(
         const Params* __restrict__ Pp, Ctl* ctl, const uint32_t* __restrict__ key,
         const uint32_t* __restrict__ start, const int* __restrict__ tmp_id,
         const uint32_t* __restrict__ bucket, const uint2* __restrict__ spill,
         const float4* __restrict__ Tp, const float4* __restrict__ Tv, float4* __restrict__ Sp,
-        float4* __restrict__ Sv, Carry carry, int n, int fused) {
+        float4* __restrict__ Sv, Carry carry, int n, ReorderFuse rf) {
     const Params& P = *Pp;
-    if (fused && blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool fused = rf.on != 0;
+    if (fused && blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) P.mbox[k] = P.nbox[k];
-        P.mcell[0] = P.ncell[0];
-        P.mcell[1] = P.ncell[1];
-        ctl->bin_miss = 0;
+            for (int k = 0; k < 6; ++k) P.mbox[k] = P.nbox[k];
+            P.mcell[0] = P.ncell[0];
+            P.mcell[1] = P.ncell[1];
+            ctl->bin_miss = 0;
+        }
+        slice_combine(P, ctl, rf.slices, rf.energy_log, rf.flags, nullptr);
     }
     const int i = logical_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -2064,106 +2188,12 @@ __device__ void wave_member_reduce(const Params& P, const BlockPart* __restrict_
 // and the energy groups, one block-wide combine.  The sums run in the order
 // of the general path (per thread strided, then waves in order), so the bits
 // are the same.
-// The reduction of a lone simulation's block partials (thread 0 publishes).
-// box_out (null or shared memory): thread 0 leaves the reduced box there.
+// The reduction of a lone simulation's block partials (the slices, on this
+// block).
 __device__ void finalize_single(const Params& P, Ctl* ctl, const BlockPart* __restrict__ parts, int nparts,
                                 const double* __restrict__ ewave, double* __restrict__ energy_log,
-                                int flags, int* box_out) {
-    __shared__ int s_iv[16][8];
-    __shared__ double s_dv[16][2];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nw = blockDim.x >> 6;
-    unsigned am = 0u;
-    unsigned long long csum = 0;
-    int bx[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
-    // Two partials and two energy groups per round, their loads issued
-    // together (the second index clamped, so neither load is predicated).
-    const int stride = static_cast<int>(blockDim.x);
-    const bool with_e = (flags & (kFinEnergyLog | kFinEnergyLast)) != 0;
-    const int ng = with_e ? (P.n + kEnergyGroup - 1) / kEnergyGroup : 0;
-    double e = 0.0;
-    for (int b = tid; b < max(nparts, ng); b += 2 * stride) {
-        const int b2 = b + stride;
-        const BlockPart& p1 = parts[min(b, nparts - 1)];
-        const BlockPart& p2 = parts[min(b2, nparts - 1)];
-        const double e1 = with_e ? ewave[min(b, ng - 1)] : 0.0;
-        const double e2 = with_e ? ewave[min(b2, ng - 1)] : 0.0;
-        int q1[8], q2[8];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            q1[k] = p1.box0[k];
-            q2[k] = p2.box0[k];
-        }
-        q1[6] = static_cast<int>(p1.accel);
-        q1[7] = static_cast<int>(p1.contacts);
-        q2[6] = static_cast<int>(p2.accel);
-        q2[7] = static_cast<int>(p2.contacts);
-        auto fold = [&](const int q[8]) __attribute__((always_inline)) {
-            am = max(am, static_cast<unsigned>(q[6]));
-            csum += static_cast<unsigned>(q[7]);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                bx[k] = min(bx[k], q[k]);
-                bx[3 + k] = max(bx[3 + k], q[3 + k]);
-            }
-        };
-        if (b < nparts) fold(q1);
-        if (b2 < nparts) fold(q2);
-        if (b < ng) e += e1;   // groups in the order tid, tid + stride, ...
-        if (b2 < ng) e += e2;
-    }
-    e = wave_sum(e);
-    const double cw = wave_sum(static_cast<double>(csum));  // exact below 2^53
-    am = wave_max_u_dpp(am);  // every thread of the block is active here
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        bx[k] = wave_min_dpp(bx[k]);
-        bx[3 + k] = wave_max_dpp(bx[3 + k]);
-    }
-    if (lane == 0) {
-        s_dv[wave][0] = e;
-        s_dv[wave][1] = cw;
-        s_iv[wave][0] = static_cast<int>(am);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) s_iv[wave][1 + k] = bx[k];
-    }
-    __syncthreads();
-    if (tid != 0) return;
-    double total = 0.0, contacts = 0.0;
-    for (int w = 0; w < nw; ++w) {
-        total += s_dv[w][0];
-        contacts += s_dv[w][1];
-        am = max(am, static_cast<unsigned>(s_iv[w][0]));
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            bx[k] = min(bx[k], s_iv[w][1 + k]);
-            bx[3 + k] = max(bx[3 + k], s_iv[w][4 + k]);
-        }
-    }
-    if (flags & kFinBox) {
-        if (!(flags & kFinPending)) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) P.mbox[k] = bx[k];
-            P.mcell[0] = 0u;
-            P.mcell[1] = grid_cells(bx);
-        }
-        store_pending(P, bx);
-        if (box_out) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) box_out[k] = bx[k];
-        }
-    }
-    if (flags & kFinAccel) {
-        ctl->accel_last = am;
-        ctl->force_passes = ctl->force_passes + 1;
-        ctl->contact_sum = ctl->contact_sum + static_cast<unsigned long long>(contacts);
-    }
-    if (flags & kFinEnergyLast) ctl->energy_last = total;
-    if (flags & kFinFlipHistory) ctl->hist_parity = ctl->hist_parity ^ 1;
-    if ((flags & kFinEnergyLog) && ctl->energy_count < ctl->energy_capacity) {
-        energy_log[ctl->energy_count] = total;
-        ctl->energy_count = ctl->energy_count + 1;
-    }
+                                int flags) {
+    slices_on_block(P, ctl, parts, nparts, ewave, energy_log, flags, nullptr);
 }
 
 // Members k_finalize reduces through LDS (more take the general path below).
@@ -2438,7 +2468,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_finalize(const Params* __res
     template __global__ void k_reorder<M, B>(const Params*, Ctl*, const uint32_t*,                \
                                              const uint32_t*, const int*, const uint32_t*,        \
                                              const uint2*, const float4*, const float4*, float4*, \
-                                             float4*, Carry, int, int);
+                                             float4*, Carry, int, ReorderFuse);
 template __global__ void k_count<false>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
                                         uint32_t*, uint32_t*, int, int*, uint32_t*, uint32_t*, uint2*, int);
 template __global__ void k_count<true>(const Params*, Ctl*, const float4*, const float4*, const uint32_t*,
