@@ -763,19 +763,67 @@ __device__ __forceinline__ void bin_grain(Ctl* ctl, uint32_t* __restrict__ count
 }
 
 // Fused step: the count of grain p (new cell nx, ny, nz; `has`: the lane
-// carries one) into the pending grid, from the force pass that produced it
-// (every lane of the wave calls this).  A grain outside the pending grid is
-// not counted; it sets bin_miss, and the next scan bins every grain itself.
-__device__ __forceinline__ void fused_count(const Params& P, Ctl* ctl, const BinIO& bio, bool has, int nx,
-                                            int ny, int nz, int id, int p, int n) {
-    if (bio.mode == 0) return;
-    const int* b = P.pbox;
-    const bool inside = has && nx >= b[0] && nx <= b[3] && ny >= b[1] && ny <= b[4] && nz >= b[2] &&
-                        nz <= b[5];
+// carries one) into the pending grid's box pb, from the force pass that
+// produced it; every lane of the wave calls both halves, in turn.
+// fused_count_issue takes the slots (one atomic per run of equal cells, as
+// bin_grain); fused_count_finish files the grain (bucket, spill list or
+// slot).  A grain outside the pending grid is not counted; it sets bin_miss,
+// and the next scan bins every grain itself.  (Issuing the atomic earlier, to
+// overlap the pass's last reductions, measured no faster and held registers
+// across them that cost the one-lane pass a wave per SIMD.)
+struct FusedCount {
+    uint32_t c;      // the grain's cell, 0xFFFFFFFF: none
+    int h;           // its run's first lane
+    uint32_t base;   // that lane's atomic result
+    bool valid;
+};
+__device__ __forceinline__ FusedCount fused_count_issue(Ctl* ctl, const BinIO& bio, const int* pb, bool has,
+                                                        int nx, int ny, int nz) {
+    FusedCount fc{0xFFFFFFFFu, 0, 0u, false};
+    if (bio.mode == 0) return fc;
+    const bool inside = has && nx >= pb[0] && nx <= pb[3] && ny >= pb[1] && ny <= pb[4] && nz >= pb[2] &&
+                        nz <= pb[5];
     if (has && !inside) ctl->bin_miss = 1;
-    const uint32_t c = inside ? static_cast<uint32_t>(make_subgrid(b).index(nx, ny, nz)) : 0xFFFFFFFFu;
-    if (bio.mode == 2) bin_grain<true>(ctl, bio.count, bio.slot, bio.bucket, bio.spill, inside, c, id, p, n);
-    else bin_grain<false>(ctl, bio.count, bio.slot, bio.bucket, bio.spill, inside, c, id, p, n);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    fc.valid = inside;
+    if (inside) {
+        SubGrid g;
+        g.ox = pb[0] - 1;
+        g.oy = pb[1] - 1;
+        g.oz = pb[2] - 1;
+        g.bx = pb[3] - pb[0] + 3;
+        g.by = pb[4] - pb[1] + 3;
+        g.bz = pb[5] - pb[2] + 3;
+        fc.c = static_cast<uint32_t>(g.index(nx, ny, nz));
+    }
+    const uint32_t cprev = static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(-2, static_cast<int>(fc.c), 0x138, 0xF, 0xF, false));
+    const unsigned long long heads = __ballot(!inside || fc.c != cprev || lane == 0);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    fc.h = 63 - __clzll(static_cast<long long>(heads & upto));
+    const unsigned long long after = heads & ~upto;
+    const int e = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;
+    if (inside && lane == fc.h) fc.base = atomicAdd(&bio.count[fc.c], static_cast<uint32_t>(e - fc.h));
+    return fc;
+}
+__device__ __forceinline__ void fused_count_finish(Ctl* ctl, const BinIO& bio, const FusedCount& fc, int id, int p,
+                                                   int n) {
+    if (bio.mode == 0) return;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const uint32_t base = static_cast<uint32_t>(__shfl(static_cast<int>(fc.base), fc.h, 64));
+    if (!fc.valid) return;
+    const uint32_t s = base + static_cast<uint32_t>(lane - fc.h);
+    if (bio.mode == 2) {
+        if (s < static_cast<uint32_t>(kBucket)) {
+            bio.bucket[static_cast<size_t>(fc.c) * kBucket + s] = static_cast<uint32_t>(id);
+        } else {
+            const unsigned bp = ctl->bin_par;  // only k_scan writes it
+            const unsigned e = atomicAdd(&ctl->spill_n[bp], 1u);
+            bio.spill[static_cast<size_t>(bp) * n + e] = make_uint2(fc.c, static_cast<uint32_t>(id));
+        }
+    } else {
+        bio.slot[p] = s;
+    }
 }
 
 // Fused step, block 0 of the force pass: re-arms the scan (as k_count) and
@@ -1498,12 +1546,15 @@ __global__ __launch_bounds__(BS, 5) void k_force(
             if (acc == acc) accel_bits = __float_as_uint(acc);
         }
     }
+    if (MODE != kModeRHS) {
+        const FusedCount fcount = fused_count_issue(ctl, bio, bio.mode ? P.pbox : nullptr, p < n, fc[0], fc[1], fc[2]);
+        fused_count_finish(ctl, bio, fcount, fid, p, n);
+    }
     if (ENERGY) {
         const double ew = group_energy<1>(static_cast<double>(e_grain));
         if ((threadIdx.x & (kEnergyGroup - 1)) == 0 && p < n) ewave[p / kEnergyGroup] = ew;
     }
     if (MODE != kModeRHS) {
-        fused_count(P, ctl, bio, p < n, fc[0], fc[1], fc[2], fid, p, n);
         store_block_part(parts, P.members > 1 ? P.pmem : nullptr, lb, lane_member, lo, hi, accel_bits, ncont);
     }
 }
@@ -1528,8 +1579,13 @@ __global__ __launch_bounds__(BS, 5) void k_force(
 // are drained, so the order holds.  A NaN candidate (a diverged state) is
 // not queued: it makes the force (and, from a higher id, the energy) NaN
 // whatever the order, which is applied at the end.  Blocks are one wave.
+// Waves per SIMD each queued variant is compiled for: four lanes per grain
+// need ~166 VGPRs, and three waves (168 at most) rather than two.
+template <int L>
+constexpr int kForceQWaves = L == 4 ? 3 : 1;
+
 template <int MODE, bool ENERGY, bool HIST, int L>
-__global__ __launch_bounds__(64) void k_force_q(
+__global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const float4* __restrict__ Np,
         const float4* __restrict__ Nv, const uint32_t* __restrict__ start,
@@ -2127,7 +2183,11 @@ __global__ __launch_bounds__(64) void k_force_q(
         if (acc == acc) accel_bits = __float_as_uint(acc);
         ncont += static_cast<unsigned>(wc);
     }
-    fused_count(P, ctl, bio, live && sub == 0, fc[0], fc[1], fc[2], id, p, n);
+    // (issued and filed here: holding the count's registers across the
+    // reductions below costs the one-lane pass a wave per SIMD)
+    const FusedCount fcount =
+            fused_count_issue(ctl, bio, bio.mode ? P.pbox : nullptr, live && sub == 0, fc[0], fc[1], fc[2]);
+    fused_count_finish(ctl, bio, fcount, id, p, n);
     if (ENERGY) {
         const double ew = group_energy<L>(live ? static_cast<double>(e_grain) : 0.0);
         if (sub == 0 && (g & (kEnergyGroup - 1)) == 0 && live) ewave[p / kEnergyGroup] = ew;
