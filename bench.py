@@ -5,7 +5,10 @@ Headline (`value`): the 32,768-grain (32^3 jittered lattice) column collapse,
 mu = 0.5, tangential spring with history, semi-implicit Euler at dt = 1e-3,
 the total energy recorded every step (fused into the force pass) and a 64^3
 cloud-in-cell deposit after every step whose absolute number is a multiple of
-40 (kept on the device).  One "step" is one take_step of the whole pipeline.
+40 (kept on the device).  One "step" is one take_step of the whole pipeline:
+a fused step (the force pass, which also counts its grains into the next
+step's cells; the cell scan, whose last blocks reduce the pass's partials;
+the reorder) or, for ensembles, the classic six-launch sequence.
 Exactly --steps K steps are timed, in a window in the middle of the
 configured 5,000-step run: the handle is first advanced untimed to step
 S0 = 2480 - ceil(min(K, 40) / 2) (2480 is the multiple of 40 nearest step
@@ -37,9 +40,11 @@ Secondary entries of the same JSON object:
             window, extended to about ten seconds of work, plus one thread.
 
 `roofline` is the force pass of the headline window: algorithmic bytes per
-grain (56 B of state and cell key, plus 8 B of spring counts and 32 B per
-contact for the stored springs; Z, the mean contact count, is read from the
-device counter over the window) times grains over its mean launch time,
+grain (56 B of state and cell key, 8 B of the fused count (the cell
+counter's increment and the grain's bucket entry or slot), plus 8 B of
+spring counts and 32 B per contact for the stored springs; Z, the mean
+contact count, is read from the device counter over the window) times grains
+over its mean launch time,
 measured by replaying the same window on a second handle with HIP events
 stamped by the kernels on the engine's stream.  `traffic` is the force
 pass's HBM bytes per launch over the same window, from the committed
@@ -65,6 +70,9 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 # Algorithmic bytes per grain of the force pass: read state 24 + material 4,
 # write state 24 (the force + integrate pass) and the next cell key 4.
 B_FORCE = 56.0
+# A fused step's force pass also counts its grains for the next bin: the cell
+# counter's increment (4 B) and the grain's bucket entry or slot (4 B).
+B_COUNT = 8.0
 # Algorithmic bytes per grain-step of the whole semi-implicit Euler step: cell
 # key (read position 12 + write key 4), stable reorder (read key 4 + state 24
 # + material 4, write state 24 + material 4 + permutation 4), fused force +
@@ -195,6 +203,7 @@ def window(cfg, k: int, s0: int, every: int, energy: bool, pin=None, profile: bo
         "stable": sim.is_stable(), "overflow": sim.grid_overflow_count(),
         "variant": variant, "scan_tile": tile, "binning": binning, "contacts_per_grain": z, "history": hist,
         "eager": cfg.kernel_timing != 0,
+        "fused": sim.fused_steps(),
         "tuned": sim.force_variant() == tuple(variant),
     }
     if keep_state:
@@ -261,7 +270,8 @@ def traffic_of(workload: str, s0: int, k: int, variant) -> float | None:
 def roofline_of(run: dict, workload: str) -> dict:
     n = run["n"]
     ms = run["kernels_ms"].get("k_force", 0.0)
-    bpg = B_FORCE + (history_bytes(run["contacts_per_grain"]) if run["history"] else 0.0)
+    bpg = B_FORCE + (B_COUNT if run["fused"] else 0.0)
+    bpg += history_bytes(run["contacts_per_grain"]) if run["history"] else 0.0
     achieved = bpg * n / (ms * 1e-3) if ms > 0 else 0.0
     return {"bound": "hbm", "kernel": "k_force", "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
@@ -284,7 +294,8 @@ def window_config(run: dict, every: int) -> dict:
                                                       "lanes_per_grain": run["variant"][1]},
             "scan_tile_cells": run["scan_tile"],
             "binning": "buckets" if run["binning"] == 2 else "scatter",
-            "tuned_before_window": True, "launch": "eager" if run["eager"] else "graphs captured before the window"}
+            "tuned_before_window": True, "launch": "eager" if run["eager"] else "graphs captured before the window",
+            "step_sequence": "fused: force + count, scan + reduction, reorder" if run["fused"] else "classic: six launches"}
 
 
 def runs32k_leg(steps: int = 5000) -> dict:
@@ -525,19 +536,23 @@ def save_window_state(steps: int = 4950) -> None:
 
 
 def restored_steps(k: int = 20) -> None:
-    """k eager steps of the 1M column from the saved window state, with the
-    one-lane force pass pinned.  The tangential springs are not part of
-    the saved state: they restart at zero and are rebuilt in the first
-    step, so the later of the k launches are the ones to read."""
+    """k steps of the 1M column from the saved window state, launched
+    eagerly (no graph replay), with the force variant, scan tile and binning
+    the tuner keeps in the window pinned (one lane per grain, 4,096-cell
+    tiles, scatter binning).  The tangential springs are not part of the
+    saved state: they restart at zero and are rebuilt in the first step, so
+    the later of the k launches are the ones to read."""
     import numpy as np
-    from world_amd import workloads
+    from world_amd import _capi, workloads
     from world_amd.sim import ParticleDynamics
 
     cfg = workloads.column_collapse_1m()
-    cfg.kernel_timing = 1
     with ParticleDynamics(cfg) as sim:
-        sim.set_state(np.load(WINDOW_STATE))
+        sim.set_graph_replay(False)
         sim.set_force_variant(64, 1)
+        sim.set_scan_tile(4096)
+        sim.set_binning(_capi.WD_BINNING_SCATTER)
+        sim.set_state(np.load(WINDOW_STATE))
         for _ in range(k):
             sim.take_step(1)
         sim.sync()

@@ -354,6 +354,12 @@ int wd_get_binning(const wd_sim* sim, int* mode_out); /* WD_BINNING_SCATTER or _
  * with a synchronization after each (src/particle_dynamics.cu:264-281,
  * src/find_neighbors_kernel.cu:190-223). */
 int wd_set_fused_steps(wd_sim* sim, int on);
+
+/* Graph replay of steps (on by default): 0 launches every kernel of every
+ * step eagerly on the handle's stream (same kernels, same results), for
+ * tools that follow kernel launches one by one.  The reference launches
+ * eagerly and synchronizes after each kernel (src/kernel.cu:36). */
+int wd_set_graph_replay(wd_sim* sim, int on);
 int wd_get_fused_steps(const wd_sim* sim, int* on_out); /* 1 when steps run fused */
 int wd_fused_misses(wd_sim* sim, unsigned long long* misses_out);
 

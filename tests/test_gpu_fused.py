@@ -162,3 +162,19 @@ def test_overflow_totals_count_the_steps_taken(picard):
     assert totals[False][0][-1] > 0
     assert totals[True][0] == totals[False][0]
     np.testing.assert_array_equal(totals[True][1], totals[False][1])
+
+
+def test_eager_launches_equal_graph_replay():
+    """Fused steps launched kernel by kernel (graph replay off) give the
+    replayed steps' bits."""
+    c = workloads.column_collapse_4k(friction=0.5)
+    st, _ = jittered(c, sigma_x=0.002, sigma_v=0.3, seed=66)
+    want = run(c, st, 25, True, energy=True)
+    with ParticleDynamics(c) as sim:
+        sim.set_graph_replay(False)
+        sim.enable_energy_log(25)
+        sim.set_state(st)
+        sim.take_step(25)
+        np.testing.assert_array_equal(sim.state(), want["state"])
+        np.testing.assert_array_equal(sim.energy_log(), want["energy"])
+        assert_springs_equal(sim.history(), want["hist"])

@@ -137,6 +137,7 @@ _SIGNATURES = [
     ("wd_set_binning", C.c_int, [C.c_void_p, C.c_int]),
     ("wd_get_binning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_set_fused_steps", C.c_int, [C.c_void_p, C.c_int]),
+    ("wd_set_graph_replay", C.c_int, [C.c_void_p, C.c_int]),
     ("wd_get_fused_steps", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("wd_fused_misses", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong)]),
     ("wd_neighbor_rows", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -151,6 +151,7 @@ struct wd_sim {
     hipGraphExec_t graph_pow[kGraphs] = {};
     hipGraphExec_t graph_cic_log = nullptr;  // one logged deposit (captured with the log buffer)
     bool timing = false;
+    bool graphs_on = true;        // wd_set_graph_replay: off launches every step's kernels eagerly
     float stage_ms[4] = {0.f, 0.f, 0.f, 0.f};
     std::vector<TimedStage> pending;
     std::vector<hipEvent_t> spare;
@@ -807,6 +808,8 @@ struct wd_sim {
             else fused_ready = false;  // a classic step leaves S unbinned
             if (timing) {
                 for (int s = 0; s < nsteps; ++s) enqueue_step();
+            } else if (!graphs_on) {
+                for (int s = 0; s < nsteps; ++s) enqueue_any_step();
             } else {
                 // graphs are captured the first time a call needs them
                 auto graph = [&](int b, hipGraphExec_t* out) -> int {
@@ -2329,6 +2332,14 @@ extern "C" int wd_get_binning(const wd_sim* s, int* mode) {
     WD_CHECK_SIM(s);
     if (!mode) return fail(WD_ERR_ARG, "wd_get_binning: null output");
     *mode = s->bucket_bin ? WD_BINNING_BUCKETS : WD_BINNING_SCATTER;
+    return WD_OK;
+}
+
+extern "C" int wd_set_graph_replay(wd_sim* s, int on) {
+    WD_CHECK_SIM(s);
+    if (on != 0 && on != 1) return fail(WD_ERR_ARG, "wd_set_graph_replay: on must be 0 or 1");
+    WD_HIP(hipStreamSynchronize(s->stream));
+    s->graphs_on = on == 1;
     return WD_OK;
 }
 

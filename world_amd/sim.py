@@ -288,6 +288,11 @@ class ParticleDynamics:
         classic sequence.  Results are the same either way."""
         check(lib().wd_set_fused_steps(self._h, 1 if on else 0))
 
+    def set_graph_replay(self, on: bool) -> None:
+        """Steps replay captured graphs (default); False launches their
+        kernels one by one (for tools that follow launches)."""
+        check(lib().wd_set_graph_replay(self._h, 1 if on else 0))
+
     def fused_steps(self) -> bool:
         v = C.c_int(0)
         check(lib().wd_get_fused_steps(self._h, C.byref(v)))
