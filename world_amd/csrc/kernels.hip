@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace wd {
@@ -161,9 +163,8 @@ __device__ __forceinline__ const float* slot_pos(const float4* __restrict__ Sp, 
     return reinterpret_cast<const float*>(reinterpret_cast<const char*>(Sp) + (q << 4));
 }
 
-// Candidate positions for the stencil walk: from the cell-sorted state in
-// global memory, or from a range of it staged in LDS (x, y pairs and z
-// values of slots base, base + 1, ...).
+// Candidate positions for the stencil walk, from the cell-sorted state in
+// global memory.
 struct GlobalPos {
     const float4* __restrict__ Sp;
     __device__ __forceinline__ void operator()(uint32_t q, float& x, float& y, float& z) const {
@@ -173,18 +174,6 @@ struct GlobalPos {
         z = s[2];
     }
 };
-struct StagedPos {
-    const float2* xy;
-    const float* z;
-    uint32_t base;
-    __device__ __forceinline__ void operator()(uint32_t q, float& x, float& y, float& zz) const {
-        const float2 a = xy[q - base];
-        x = a.x;
-        y = a.y;
-        zz = z[q - base];
-    }
-};
-
 // Slot of `partner` in an old spring list (slot-major with stride n, grain
 // index hs, h_old entries), or -1 when it holds no such partner.  The
 // partner ids are loaded eight at a time, all in flight together, rather than
@@ -328,6 +317,102 @@ __device__ __forceinline__ void walk_slice(const float4* __restrict__ Sp, const 
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             if (t0 + k < s_hi) probe(q[k], px[k], py[k], pz[k]);
+        }
+    }
+}
+
+// One dx group of a one-lane-per-grain walk, its three columns c0, c0 + 1,
+// c0 + 2 in the reference's neighbor-row order: column k with no cell over
+// capacity is the run bk + [0, lk); lk = ~0 marks a capped one, whose
+// cell-start quad column(c) is loaded again.  Positions from global memory,
+// BATCH per round (a slot past a run reloads its last candidate), every slot
+// probed without branches as probe(q, x, y, z, valid).
+template <int BATCH, class Column, class Probe>
+__device__ __forceinline__ void walk_columns3(const float4* __restrict__ Sp, uint32_t kcap, int c0, uint32_t b0,
+                                              uint32_t l0, uint32_t b1, uint32_t l1, uint32_t b2, uint32_t l2,
+                                              Column column, Probe probe) {
+    const GlobalPos pos{Sp};
+    auto run = [&](uint32_t b, uint32_t len) __attribute__((always_inline)) {
+        for (uint32_t i = 0; i < len; i += BATCH) {
+            float px[BATCH], py[BATCH], pz[BATCH];
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) pos(b + min(i + k, len - 1), px[k], py[k], pz[k]);
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) probe(b + i + k, px[k], py[k], pz[k], i + k < len);
+        }
+    };
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t b = c == 0 ? b0 : (c == 1 ? b1 : b2);
+        const uint32_t len = c == 0 ? l0 : (c == 1 ? l1 : l2);
+        if (len != ~0u) {
+            run(b, len);
+        } else {
+            const uint4 q = column(c0 + c);
+            run(q.x, min(q.y - q.x, kcap));
+            run(q.y, min(q.z - q.y, kcap));
+            run(q.z, min(q.w - q.z, kcap));
+        }
+    }
+}
+
+// The same walk from positions staged in LDS (x, y, z records of slots
+// base, base + 1, ..., at most `area` of them), with every lane taking part
+// (a lane with nothing to walk has runs of length 0): the wave steps through
+// each run to its longest one, so that the control flow stays uniform.  A
+// round's BATCH reads start at a clamped index and stay in the area; the
+// slots past a run are read but not used.  probe(q, x, y, z, valid, self,
+// nan): `self` false tells the probe that q is not its own grain (only
+// column c = 4, dx = dy = 0, holds it); `nan` false, that no squared
+// distance can be NaN (finite = every staged position and the wave's own
+// positions are finite: a squared distance of finite positions is finite or
+// +inf).
+template <int BATCH, class Column, class Probe>
+__device__ __forceinline__ void walk_columns3_staged(const float* __restrict__ sxyz, uint32_t base, uint32_t area,
+                                                     bool finite, uint32_t kcap, int c0, uint32_t b0,
+                                                     uint32_t l0, uint32_t b1, uint32_t l1, uint32_t b2,
+                                                     uint32_t l2, Column column, Probe probe) {
+    auto run = [&](uint32_t b, uint32_t len, auto self, auto nan) __attribute__((always_inline)) {
+        const uint32_t lmax = wave_max_u_dpp(len);
+#pragma unroll 1
+        for (uint32_t i = 0; i < lmax; i += BATCH) {
+            // a clamped round holds no valid slot
+            const int r = static_cast<int>(min(b + i - base, area - BATCH));
+            __builtin_assume(r >= 0 && r < 4096);  // the area is a few KB: the reads take immediate offsets
+            const float* rec = sxyz + 3 * r;
+            float px[BATCH], py[BATCH], pz[BATCH];
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                px[k] = rec[3 * k];
+                py[k] = rec[3 * k + 1];
+                pz[k] = rec[3 * k + 2];
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; ++k) {
+                probe(b + i + k, px[k], py[k], pz[k], i + k < len, decltype(self)::value, decltype(nan)::value);
+            }
+        }
+    };
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t b = c == 0 ? b0 : (c == 1 ? b1 : b2);
+        const uint32_t len = c == 0 ? l0 : (c == 1 ? l1 : l2);
+        const bool capped = len == ~0u;
+        if (__ballot(capped) == 0ull) {
+            if (!finite) run(b, len, std::true_type{}, std::true_type{});
+            else if (c0 + c == 4) run(b, len, std::true_type{}, std::false_type{});
+            else run(b, len, std::false_type{}, std::false_type{});
+        } else {
+            // a capped lane walks its three cells' first kcap grains; the
+            // others walk their run as the first of three
+            uint4 q = make_uint4(b, b + len, b + len, b + len);
+            if (capped) q = column(c0 + c);
+            const uint32_t m0 = capped ? min(q.y - q.x, kcap) : len;
+            const uint32_t m1 = capped ? min(q.z - q.y, kcap) : 0u;
+            const uint32_t m2 = capped ? min(q.w - q.z, kcap) : 0u;
+            run(q.x, m0, std::true_type{}, std::true_type{});
+            run(q.y, m1, std::true_type{}, std::true_type{});
+            run(q.z, m2, std::true_type{}, std::true_type{});
         }
     }
 }
@@ -1597,7 +1682,7 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     const Params& P = *Pp;
     if (bio.mode) fused_arm(P, ctl);
     // Per lane: cell-sorted slots of listed contacts; with one lane per
-    // grain, row QCAP takes the stores of candidates that are not listed.
+    // grain, row QCAP takes the stores that no list row takes.
     // One lane per grain keeps its LDS small (about 7 KB: five waves per
     // SIMD at the register count it takes without the next round's
     // prefetch): the owner's spring slot and old spring list come by
@@ -1607,7 +1692,7 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     // while the stencil is walked, and the damping table holds the handle's
     // materials only (dynamic LDS, nmat x nmat).
     constexpr int NR = ENERGY ? 11 : 9;         // floats per record (L == 1)
-    constexpr int RB = NR * 64 / 3;             // L == 1: staged slots (x, y pair and z each)
+    constexpr int RB = NR * 64 / 3;             // L == 1: staged slots (x, y, z each)
     __shared__ uint32_t s_list[L == 1 ? QCAP + 1 : QCAP][64];  // L == 1: row QCAP is the spare row
     __shared__ int s_q[L == 1 ? 1 : 64];        // per lane: queue index of its first listed contact
     __shared__ uint8_t s_src[L == 1 ? 1 : QCAP * 64];  // owner lane of each queue entry (L > 1)
@@ -1619,9 +1704,7 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     __shared__ int2 s_gh[L == 1 ? 1 : G];       // per grain: old spring list (index, count)
     __shared__ float s_cpair[L == 1 ? 1 : kMaxMat * kMaxMat];
     extern __shared__ float s_cdyn[];           // L == 1: c_pair of the handle's nmat materials
-    uint32_t* const s_spare = &s_list[L == 1 ? QCAP : 0][0];
-    float2* const s_sxy = reinterpret_cast<float2*>(&s_r[0][0]);  // staged x, y (L == 1)
-    float* const s_sz = reinterpret_cast<float*>(s_sxy + RB);      // staged z (L == 1)
+    float* const s_sxyz = &s_r[0][0];  // staged x, y, z records (L == 1)
     // A lone simulation's pair damping table, in LDS: one lookup per contact.
     const bool lds_tab = P.mtab == nullptr;
     const int nmat = P.nmat;
@@ -1736,15 +1819,21 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     if (L > 1 && sub == 0) s_gh[g] = make_int2(hs, h_old);
 
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
-    int cx = 0, cy = 0, cz = 0;
-    MemberCells mc{};
+    // The grain's cell in its member's padded sub-grid and the index steps
+    // of one cell in x and y: the stencil's cells lie at fixed offsets.
+    uint32_t c_self = 0u, step_x = 0u, step_y = 0u;
     if (live) {
-        mc = member_cells(P, member);
+        const MemberCells mc = member_cells(P, member);
+        int cx, cy, cz;
         grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
+        c_self = mc.index(cx, cy, cz);
+        step_y = static_cast<uint32_t>(mc.g.bz);
+        step_x = static_cast<uint32_t>(mc.g.by) * step_y;
     }
     auto column = [&](int c) __attribute__((always_inline)) {
+        const uint32_t cell = c_self + static_cast<uint32_t>(c / 3 - 1) * step_x + static_cast<uint32_t>(c % 3 - 1) * step_y - 1u;
         uint4 q4;
-        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+        __builtin_memcpy(&q4, start + cell, sizeof(uint4));
         return q4;
     };
     // This lane's slice [s_lo, s_hi) of the grain's candidate sequence (set
@@ -1752,27 +1841,32 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     uint32_t s_lo = 0u, s_hi = 0xFFFFFFFFu;
     int cons = 0;     // this lane's contacts already queued
     int seen = 0;     // touching candidates met by the current scan
+    int rowc = 0;     // one lane per grain: seen - cons (the list row of the next listed one)
     bool nan_f = false, nan_e = false;
+    int nan_n = 0;    // one lane per grain: NaN candidates met
     const float touch2 = P.touch2;
     // One lane per grain: every slot of a batch is probed without branches
-    // (an unlisted candidate's store goes to the spare row); with more lanes
-    // per grain, only the slots in the run, with branches.
-    auto probe = [&](uint32_t q, float px, float py, float pz, bool valid = true) __attribute__((always_inline)) {
+    // (an unlisted candidate's store goes to a row no list entry reads), and
+    // a NaN candidate only marks the lane; with more lanes per grain, only
+    // the slots in the run, with branches.
+    auto probe = [&](uint32_t q, float px, float py, float pz, bool valid = true, bool self = true,
+                     bool nan = true) __attribute__((always_inline)) {
         const float d0 = x[0] - px, d1 = x[1] - py, d2 = x[2] - pz;
         float s2 = 0.f;
         s2 += d0 * d0;
         s2 += d1 * d1;
         s2 += d2 * d2;
         if constexpr (L == 1) {
-            const bool other = valid & (q != static_cast<uint32_t>(p));
+            const bool other = valid & (!self | (q != static_cast<uint32_t>(p)));
             const bool hit = other & (s2 < touch2);
-            const uint32_t li = static_cast<uint32_t>(seen - cons);
-            *(hit && li < static_cast<uint32_t>(QCAP) ? &s_list[li][lane] : &s_spare[lane]) = q;
-            seen += hit ? 1 : 0;
-            if (other & (s2 != s2)) {
-                nan_f = true;
-                if (__float_as_int(Sp[q].w) > id) nan_e = true;
-            }
+            // Every slot stores: into the row this pass's next listed
+            // candidate takes (a later one overwrites it; the rows past the
+            // list are not read) or, for the candidates listed by an earlier
+            // pass or past this pass's rows, into the spare row.
+            const uint32_t li = min(static_cast<uint32_t>(rowc), static_cast<uint32_t>(QCAP));
+            s_list[li][lane] = q;
+            rowc += hit ? 1 : 0;
+            if (nan) nan_n += (other & (s2 != s2)) ? 1 : 0;
         } else if (!(s2 >= touch2) && q != static_cast<uint32_t>(p)) {
             if (s2 < touch2) {
                 if (seen >= cons && seen < cons + QCAP) s_list[seen - cons][lane] = q;
@@ -1799,10 +1893,13 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
     // padded sub-grid rules out) is walked from global memory.
     uint32_t c_first = 0u, c_last = 0u, Sx = 0u, Sy = 0u;
     const bool stage_ok = L == 1 && P.members == 1 && lb * G < n;
+    // every grain of the wave at a finite position
+    const bool own_finite =
+            __ballot(live && !(__builtin_isfinite(x[0]) && __builtin_isfinite(x[1]) && __builtin_isfinite(x[2]))) ==
+            0ull;
     if constexpr (L == 1) {
         if (stage_ok) {
             const MemberCells m0 = member_cells(P, 0);
-            const uint32_t c_self = live ? m0.index(cx, cy, cz) : 0u;
             // lane 0 holds the block's first grain, lane n - 1 - lb G (or
             // 63) its last
             c_first = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(c_self)));
@@ -1811,37 +1908,74 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
             Sy = static_cast<uint32_t>(m0.g.bz);
         }
     }
-    const uint32_t n_cells = P.mcell[1];  // lone simulation: start[] holds n_cells + 1 entries
+    // One lane per grain: the slot ranges [g_lo, g_lo + len) of the three dx
+    // groups (uniform), loaded together before the walk; a group whose cell
+    // range leaves the cell-start array (ruled out by the padded sub-grid)
+    // keeps length ~0 and is walked from global memory.
+    uint32_t g_lo0 = 0u, g_lo1 = 0u, g_lo2 = 0u, g_len0 = ~0u, g_len1 = ~0u, g_len2 = ~0u;
+    if constexpr (L == 1) {
+        if (stage_ok) {
+            const uint32_t n_cells = P.mcell[1];  // lone simulation: start[] holds n_cells + 1 entries
+            auto range = [&](int gx, uint32_t& g_lo, uint32_t& len) __attribute__((always_inline)) {
+                const uint32_t lo = c_first + static_cast<uint32_t>(gx - 1) * Sx - Sy - 1u;
+                const uint32_t hi = c_last + static_cast<uint32_t>(gx - 1) * Sx + Sy + 2u;
+                if (lo <= hi && hi <= n_cells) {
+                    g_lo = start[lo];
+                    len = start[hi] - g_lo;
+                }
+            };
+            range(0, g_lo0, g_len0);
+            range(1, g_lo1, g_len1);
+            range(2, g_lo2, g_len2);
+        }
+    }
     for (int pass = 0;; ++pass) {
         seen = 0;
+        rowc = -cons;
         const bool walking = live && (pass == 0 || cons < total);
         if constexpr (L == 1) {
 #pragma unroll 1
             for (int gx = 0; gx < 3; ++gx) {
-                bool staged = false;
-                uint32_t g_lo = 0u;
-                if (stage_ok) {
-                    const uint32_t lo = c_first + static_cast<uint32_t>(gx - 1) * Sx - Sy - 1u;
-                    const uint32_t hi = c_last + static_cast<uint32_t>(gx - 1) * Sx + Sy + 2u;
-                    const bool inside = lo <= hi && hi <= n_cells;
-                    g_lo = inside ? start[lo] : 0u;
-                    const uint32_t len = inside ? start[hi] - g_lo : 0xFFFFFFFFu;
-                    if (len <= static_cast<uint32_t>(RB)) {
-                        staged = true;
-                        for (uint32_t k = static_cast<uint32_t>(lane); k < len; k += 64u) {
-                            const float4 a = Sp[g_lo + k];
-                            s_sxy[k] = make_float2(a.x, a.y);
-                            s_sz[k] = a.z;
-                        }
+                // The group's column quads are in flight while its positions
+                // are staged: a group waits on one load after the first.
+                uint4 qa0{}, qa1{}, qa2{};
+                if (walking) {
+                    qa0 = column(3 * gx);
+                    qa1 = column(3 * gx + 1);
+                    qa2 = column(3 * gx + 2);
+                }
+                // a column as its run (begin, length), ~0: a cell over
+                // capacity; (0, 0) for a lane with nothing to walk
+                auto run_of = [&](const uint4& q, uint32_t& b, uint32_t& l) __attribute__((always_inline)) {
+                    b = walking ? q.x : 0u;
+                    l = !walking ? 0u : (column_capped(q, kcap) ? ~0u : q.w - q.x);
+                };
+                const uint32_t g_lo = gx == 0 ? g_lo0 : (gx == 1 ? g_lo1 : g_lo2);
+                const uint32_t len = gx == 0 ? g_len0 : (gx == 1 ? g_len1 : g_len2);
+                // (at most RB - 4 slots: a round of staged reads clamped into
+                // the area then holds no slot of a run)
+                const bool staged = len <= static_cast<uint32_t>(RB - 4);
+                bool bad = false;  // a staged position that is not finite
+                if (staged) {
+                    for (uint32_t k = static_cast<uint32_t>(lane); k < len; k += 64u) {
+                        const float4 a = Sp[g_lo + k];
+                        s_sxyz[3 * k] = a.x;
+                        s_sxyz[3 * k + 1] = a.y;
+                        s_sxyz[3 * k + 2] = a.z;
+                        bad |= !(__builtin_isfinite(a.x) && __builtin_isfinite(a.y) && __builtin_isfinite(a.z));
                     }
                 }
+                const bool finite = own_finite && __ballot(bad) == 0ull;
                 __syncthreads();
-                if (walking) {
-                    if (staged) {
-                        walk_stencil<4, false, true>(StagedPos{s_sxy, s_sz, g_lo}, kcap, s_lo, s_hi, column, probe, gx, gx + 1);
-                    } else {
-                        walk_stencil<4, false, true>(GlobalPos{Sp}, kcap, s_lo, s_hi, column, probe, gx, gx + 1);
-                    }
+                uint32_t b0, l0, b1, l1, b2, l2;
+                run_of(qa0, b0, l0);
+                run_of(qa1, b1, l1);
+                run_of(qa2, b2, l2);
+                if (staged) {
+                    walk_columns3_staged<3>(s_sxyz, g_lo, static_cast<uint32_t>(RB), finite, kcap, 3 * gx, b0, l0,
+                                            b1, l1, b2, l2, column, probe);
+                } else if (walking) {
+                    walk_columns3<4>(Sp, kcap, 3 * gx, b0, l0, b1, l1, b2, l2, column, probe);
                 }
                 __syncthreads();  // the staged range is read before the next one is written
             }
@@ -1873,7 +2007,7 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
             }
         }
         if (pass == 0) {
-            total = seen;
+            total = L == 1 ? rowc : seen;  // cons is 0
             int b = wc;
 #pragma unroll
             for (int k = 1; k < L; ++k) {
@@ -2131,6 +2265,25 @@ __global__ __launch_bounds__(64, kForceQWaves<L>) void k_force_q(
         }
         cons += cnt;
         if (__ballot(cons < total) == 0ull) break;
+    }
+    if constexpr (L == 1) {
+        // The walk only counted the NaN candidates a lane met: its force is
+        // NaN, and its energy is when such a candidate has the higher id (a
+        // pair's energy is counted from its lower id), which a second walk
+        // of the stencil finds (a diverged state only).
+        nan_f = nan_n != 0;
+        if (ENERGY && nan_f) {
+            walk_stencil<4, false, true>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column,
+                                         [&](uint32_t q, float px, float py, float pz, bool valid)
+                                                 __attribute__((always_inline)) {
+                const float d0 = x[0] - px, d1 = x[1] - py, d2 = x[2] - pz;
+                float s2 = 0.f;
+                s2 += d0 * d0;
+                s2 += d1 * d1;
+                s2 += d2 * d2;
+                if (valid && q != static_cast<uint32_t>(p) && s2 != s2 && __float_as_int(Sp[q].w) > id) nan_e = true;
+            });
+        }
     }
     const unsigned long long gmask = ((L == 1) ? 1ull : ((1ull << L) - 1ull)) << (g * L);
     if (__ballot(nan_f) & gmask) {
