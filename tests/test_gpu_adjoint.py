@@ -196,6 +196,34 @@ def test_batched_adjoint_equals_single_adjoints():
             np.testing.assert_array_equal(gp[m], single[m][1][0])
 
 
+def test_one_and_four_lane_passes_give_the_same_bits():
+    """The contact and gather passes run four lanes per grain on handles of up
+    to 65,536 grains and one lane on larger ones, with a grain's share summed
+    in the same parts either way.  A batch past that size (one lane) against
+    its members alone (four lanes each), from overlapping jittered states so
+    that every dx group holds contacts: the same gradients, bit for bit."""
+    cs = [workloads.column(28, 28, 28, friction=0.3, seed=s) for s in (1, 2, 3)]
+    sts = [jittered(c, 20 + k)[0] for k, c in enumerate(cs)]
+    assert sum(len(st) for st in sts) > 65536 >= max(len(st) for st in sts)
+    nodes = nodes_of(cs[0])
+    rng = np.random.default_rng(8)
+    Ws = [rng.normal(size=4 * nodes).astype(np.float32) for _ in cs]
+    single = []
+    for c, st, W in zip(cs, sts, Ws):
+        with ParticleDynamics(c) as sim:
+            sim.set_state(st)
+            sim.adjoint_record(24, 8)
+            single.append(sim.adjoint_backward(W))
+    with Ensemble(cs) as ens:
+        ens.set_state(np.concatenate(sts))
+        ens.adjoint_record(24, 8)
+        gs, gp = ens.adjoint_backward(np.concatenate(Ws))
+        for m, (first, count) in enumerate(ens.ranges):
+            assert np.count_nonzero(single[m][0]) > 0
+            np.testing.assert_array_equal(gs[first:first + count], single[m][0])
+            np.testing.assert_array_equal(gp[m], single[m][1][0])
+
+
 def test_adjoint_rejects_unsupported_windows():
     c = workloads.column(3, 3, 3)
     c.time_integrator = _capi.WD_BACKWARD_EULER_PICARD

@@ -10,16 +10,16 @@
 // piecewise-smooth step.
 //
 // Per step s (state s -> s+1), with lambda = dL/d(state s+1) given:
-//   k_adj_contacts  one lane per grain in cell order: replays every contact
-//                   of the forward pass, applies the contact's vector-Jacobian
+//   k_adj_contacts  grains in cell order (one or four lanes each): replays
+//                   every contact of the forward pass, applies the contact's vector-Jacobian
 //                   product to (dL/dF_i, dL/dxi_new), keeps the grain's own
 //                   share, writes dL/dxi_old into the previous spring
 //                   generation, and stores each pair contact's (d, v_r)
 //                   cotangents for the partner;
-//   k_adj_gather    one lane per grain: adds the partners' shares (the pair
+//   k_adj_gather    per grain: adds the partners' shares (the pair
 //                   law depends on x_i - x_j and v_i - v_j only, so the
-//                   partner's share is the negative of the stored one), in the
-//                   grain's own contact order: no atomics, deterministic.
+//                   partner's share is the negative of the stored one), in a
+//                   fixed order of the grain's contacts: no atomics, deterministic.
 // CIC cotangents enter through k_adj_seed.  Everything is accumulated in FP32
 // except the parameter gradients (per-grain double sums over the window).
 //
@@ -314,9 +314,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_seed(const Params* __restrict__ 
     lv[id] = make_float4(gv[0], gv[1], gv[2], 0.f);
 }
 
-// Grains per block of the per-grain adjoint passes: one wave, so that a few
-// thousand grains spread over as many CUs as there are waves (the passes use
-// no block-wide step).
+// The per-grain adjoint passes run one-wave blocks of L lanes per grain.  A
+// grain's contacts fall into the stencil's three dx groups, and its share
+// is summed per group: with L = 4, lanes 0, 1, 2 of the grain take dx
+// groups 0, 1, 2 (lane 3 none) and the lanes' parts are added in lane
+// order; with L = 1 the lane takes the groups one after another and adds
+// their parts in the same order, so both give the same bits.  Four lanes
+// run a grain's chains of dependent loads side by side, which pays while
+// the grid is small: below kAdjLanesUpTo grains the passes run L = 4
+// (8,192 grains: 33 -> 20 us for the contacts, 15 -> 9 us for the gather),
+// above it L = 1 (a 171,425-grain batch has 2.6 one-lane waves per SIMD
+// already, and four lanes made its backward step 11% slower).
+constexpr int kAdjLanes = 4;
+constexpr int kAdjLanesUpTo = 65536;
 constexpr int kAdjBlock = 64;
 
 // Pass A of the adjoint of one SIE step (see the file comment).  Grains in
@@ -324,6 +334,14 @@ constexpr int kAdjBlock = 64;
 // forward step read; lxi_next[k][id] = dL/d(spring slot k written by the
 // step); lxi_prev[t][id] receives dL/d(spring slot t read by the step).  The
 // cotangent lists are indexed by original id.
+// A dx group's touching candidates are counted before they are evaluated,
+// which fixes every contact's spring slot and pair index (walls first, then
+// the groups in order).  The share of the grain's force cotangent and of its
+// parameters is summed per dx group (walls and the integrator's terms in
+// group 0's part), and the parts are added in group order.  Every slot of
+// the springs the step read is written once: a matched contact's
+// dL/dxi_old, or zero.
+template <int L>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
         const Params* __restrict__ Pp, Ctl* ctl, const float4* __restrict__ Sp,
         const float4* __restrict__ Sv, const uint32_t* __restrict__ start,
@@ -334,10 +352,17 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
         float4* __restrict__ lxi_prev, float4* __restrict__ own_x, float4* __restrict__ own_v,
         float4* __restrict__ cg_d, float4* __restrict__ cg_v, int* __restrict__ cg_cnt,
         int* __restrict__ cg_p, double2* __restrict__ gparam, int n) {
+    static_assert(L == 1 || L == kAdjLanes, "adjoint pass variants");
+    constexpr int G = kAdjBlock / L;  // grains per block
     const Params& P = *Pp;
     __shared__ uint32_t s_list[kContactList][kAdjBlock];
-    const int p = logical_block() * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const int lane = threadIdx.x;
+    const int sub = lane & (L - 1);
+    const int first = logical_block() * G;
+    if (first >= n) return;  // the whole wave is past the last grain
+    // Lanes past the last grain mirror it and write nothing.
+    const bool live = first + lane / L < n;
+    const int p = live ? first + lane / L : n - 1;
     const float4 pi = Sp[p];
     const float4 vi4 = Sv[p];
     const float x[3] = {pi.x, pi.y, pi.z};
@@ -354,71 +379,80 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
     float gF[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) gF[a] = P.dt * lvt[a] / m;
-    float gx[3] = {lx4.x, lx4.y, lx4.z};
-    float gv[3] = {lvt[0], lvt[1], lvt[2]};
+    // the current dx group's part of the grain's share (group 0's starts
+    // with the integrator's terms)
+    float gx[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
+    if (sub == 0) {
+        gx[0] = lx4.x; gx[1] = lx4.y; gx[2] = lx4.z;
+        gv[0] = lvt[0]; gv[1] = lvt[1]; gv[2] = lvt[2];
+    }
     float gk = 0.f, gmu = 0.f;
 
     const int hs = hidx[p];  // the grain's index in the springs the step read
     const int h_old = h_cnt[hs];
-    for (int t = 0; t < h_old; ++t) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(0.f, 0.f, 0.f, 0.f);
-    int h_new = 0, kc = 0;
-    // Stored spring of `partner` and its slot (-1: a new contact, zero spring).
-    // The stored list is in contact order, so its next entry (prefetched one
-    // contact ahead) normally matches; a miss searches the whole list.
-    int h_cur = 0;
-    float4 h_next = h_in[hs];  // slot 0, whether or not in use
-    auto h_find = [&](int partner, float xi[3]) __attribute__((always_inline)) {
-        int found = -1;
-        if (h_cur < h_old && __float_as_int(h_next.w) == partner) {
-            xi[0] = h_next.x; xi[1] = h_next.y; xi[2] = h_next.z;
-            found = h_cur;
-        } else {
-            xi[0] = xi[1] = xi[2] = 0.f;
-            found = spring_slot(h_in, n, hs, h_old, partner);
-            if (found < 0) return -1;
-            const float4 h = h_in[static_cast<size_t>(found) * n + hs];
+    unsigned written = 0u;   // slots of the springs read that this lane wrote
+    // Stored spring of `partner` and its slot (-1: a new contact, zero
+    // spring).  Lists keep their order, so slot `guess` (the contact's slot
+    // in the new list) normally holds it; otherwise the old list is searched.
+    auto h_find = [&](int partner, int guess, float xi[3]) __attribute__((always_inline)) {
+        const float4 hg = h_in[static_cast<size_t>(min(guess, kHist - 1)) * n + hs];
+        if (guess < h_old && __float_as_int(hg.w) == partner) {
+            xi[0] = hg.x; xi[1] = hg.y; xi[2] = hg.z;
+            return guess;
+        }
+        xi[0] = xi[1] = xi[2] = 0.f;
+        const int t = spring_slot(h_in, n, hs, h_old, partner);
+        if (t >= 0) {
+            const float4 h = h_in[static_cast<size_t>(t) * n + hs];
             xi[0] = h.x; xi[1] = h.y; xi[2] = h.z;
         }
-        h_cur = found + 1;
-        // the next slot, loaded unconditionally (slots below kHist exist)
-        h_next = h_in[static_cast<size_t>(min(h_cur, kHist - 1)) * n + hs];
-        return found;
+        return t;
     };
-    auto g_next = [&](float g[3]) __attribute__((always_inline)) {
-        if (h_new < kHist) {
-            const float4 q = lxi_next[static_cast<size_t>(h_new) * n + p];
+    auto g_next = [&](int slot, float g[3]) __attribute__((always_inline)) {
+        if (slot < kHist) {
+            const float4 q = lxi_next[static_cast<size_t>(slot) * n + p];
             g[0] = q.x; g[1] = q.y; g[2] = q.z;
         } else {
             g[0] = g[1] = g[2] = 0.f;
         }
     };
+    auto keep_xi = [&](int t, const float g_xi[3]) __attribute__((always_inline)) {
+        if (t >= 0 && live) {
+            lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+            written |= 1u << t;
+        }
+    };
 
-    // Walls, lower then upper per axis, as the forward.
+    // Walls, lower then upper per axis, as the forward, in group 0's part;
+    // every lane counts them (they take the first spring slots).
     const float c_wall = mat_cwall(P, member, mat);
+    int wc = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const float dist = side == 0 ? x[a] - P.wlo[a] - P.radius : P.whi[a] - x[a] - P.radius;
             if (dist < 0.f) {
-                const float raw = P.kneg * dist;
-                const float fw = fmaxf(0.f, fminf(P.max_force, raw));
-                float nrm[3] = {0.f, 0.f, 0.f};
-                nrm[a] = side == 0 ? 1.f : -1.f;
-                float vn = 0.f;
-                vn += v[0] * nrm[0];
-                vn += v[1] * nrm[1];
-                vn += v[2] * nrm[2];
-                float xi[3], gX[3], g_n[3], g_vr[3], g_xi[3], g_dist;
-                const int t = h_find(wall_partner(a, side), xi);
-                g_next(gX);
-                contact_vjp(P, mu, nrm, v, vn, fw, fw == raw, dist, c_wall, xi, gF, gX, g_n, g_dist,
-                            g_vr, g_xi, gk, gmu);
-                if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
-                gx[a] += side == 0 ? g_dist : -g_dist;
+                if (sub == 0) {
+                    const float raw = P.kneg * dist;
+                    const float fw = fmaxf(0.f, fminf(P.max_force, raw));
+                    float nrm[3] = {0.f, 0.f, 0.f};
+                    nrm[a] = side == 0 ? 1.f : -1.f;
+                    float vn = 0.f;
+                    vn += v[0] * nrm[0];
+                    vn += v[1] * nrm[1];
+                    vn += v[2] * nrm[2];
+                    float xi[3], gX[3], g_n[3], g_vr[3], g_xi[3], g_dist;
+                    const int t = h_find(wall_partner(a, side), wc, xi);
+                    g_next(wc, gX);
+                    contact_vjp(P, mu, nrm, v, vn, fw, fw == raw, dist, c_wall, xi, gF, gX, g_n, g_dist,
+                                g_vr, g_xi, gk, gmu);
+                    keep_xi(t, g_xi);
+                    gx[a] += side == 0 ? g_dist : -g_dist;
 #pragma unroll
-                for (int b = 0; b < 3; ++b) gv[b] += g_vr[b];
-                ++h_new;
+                    for (int b = 0; b < 3; ++b) gv[b] += g_vr[b];
+                }
+                ++wc;
             }
         }
     }
@@ -429,26 +463,99 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
     int cx, cy, cz;
     grain_cell(P, x[0], x[1], x[2], cx, cy, cz);
     const uint32_t kcap = static_cast<uint32_t>(P.kcap);
-    int listed = 0;
-    auto flush = [&]() __attribute__((always_inline)) {
+    // The forward pass's candidate walk (L = 4: this lane's dx group; L = 1:
+    // the three groups in order, each listed entry tagged with its group in
+    // bits 28-29): touching candidates (squared distance below touch2, exactly
+    // the contact law's dist < 0) listed in LDS in stencil order; a lane with
+    // more than kContactList of them walks again past the ones evaluated.
+    int seen = 0, consumed = 0, listed = 0;
+    uint32_t wtag = 0u;  // L = 1: the group being walked, in bits 28-29
+    auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
+        const float e0 = x[0] - px, e1 = x[1] - py, e2 = x[2] - pz;
+        float s2 = 0.f;
+        s2 += e0 * e0;
+        s2 += e1 * e1;
+        s2 += e2 * e2;
+        if (s2 < P.touch2 && q != static_cast<uint32_t>(p)) {
+            if (seen >= consumed && listed < kContactList) {
+                s_list[listed][lane] = q | wtag;
+                ++listed;
+            }
+            ++seen;
+        }
+    };
+    auto column = [&](int c) __attribute__((always_inline)) {
+        uint4 q4;
+        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
+        return q4;
+    };
+    auto walk = [&]() __attribute__((always_inline)) {
+        seen = 0;
+        listed = 0;
+        if (!live) return;
+        if constexpr (L == 1) {
+#pragma unroll 1
+            for (int g = 0; g < 3; ++g) {
+                wtag = static_cast<uint32_t>(g) << 28;
+                walk_stencil<12, true>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe, g, g + 1);
+            }
+        } else if (sub < 3) {
+            walk_stencil<12, true>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe, sub, sub + 1);
+        }
+    };
+    walk();
+    const int mine = seen;  // this lane's contacts
+    int before = 0;         // L = 4: the grain's contacts in the lanes before this one
+    int pairs = mine;       // the grain's pair contacts
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int k = 1; k < L; ++k) {
+            const int o = __shfl(mine, lane - k, 64);
+            if (sub >= k) before += o;
+        }
+        pairs = __shfl(before + mine, lane | (L - 1), 64);
+    }
+    // L = 1: the parts of the groups before group `grp` (whose part the
+    // accumulators hold), summed in order.
+    float tx[3] = {0.f, 0.f, 0.f}, tv[3] = {0.f, 0.f, 0.f};
+    float tk = 0.f, tmu = 0.f;
+    int grp = 0;
+    auto fold = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            tx[a] = grp == 0 ? gx[a] : tx[a] + gx[a];
+            tv[a] = grp == 0 ? gv[a] : tv[a] + gv[a];
+            gx[a] = gv[a] = 0.f;
+        }
+        tk = grp == 0 ? gk : tk + gk;
+        tmu = grp == 0 ? gmu : tmu + gmu;
+        gk = gmu = 0.f;
+        ++grp;
+    };
+    constexpr uint32_t kSlotBits = 0x0FFFFFFFu;
+    for (;;) {
         uint32_t qn = 0;
         float4 pjn = make_float4(0.f, 0.f, 0.f, 0.f), vjn = pjn;
         if (listed > 0) {
-            qn = s_list[0][threadIdx.x];
-            pjn = Sp[qn];
-            vjn = Sv[qn];
+            qn = s_list[0][lane];
+            pjn = Sp[qn & kSlotBits];
+            vjn = Sv[qn & kSlotBits];
         }
         for (int k = 0; k < listed; ++k) {
             const float4 pj = pjn, vj = vjn;
-            const uint32_t qk = qn;
+            const uint32_t qk = qn & kSlotBits;
+            if constexpr (L == 1) {
+                while (grp < static_cast<int>(qn >> 28)) fold();
+            }
             // the next contact's loads, unconditional (the last one's again)
-            qn = s_list[min(k + 1, listed - 1)][threadIdx.x];
-            pjn = Sp[qn];
-            vjn = Sv[qn];
+            qn = s_list[min(k + 1, listed - 1)][lane];
+            pjn = Sp[qn & kSlotBits];
+            vjn = Sv[qn & kSlotBits];
+            const int kc = before + consumed + k;  // the contact's pair index
+            const int slot = wc + kc;              // its spring slot in the new list
             float d0, d1, d2;
             const float norm = pair_norm(x, pj.x, pj.y, pj.z, d0, d1, d2);
             const float dist = norm - P.radius - P.radius;
-            if (!(dist < 0.f)) continue;
             const int jd = __float_as_int(pj.w);
             const float nrm[3] = {d0 / norm, d1 / norm, d2 / norm};
             const float raw = P.kneg * dist;
@@ -460,11 +567,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
             vn += vr[1] * nrm[1];
             vn += vr[2] * nrm[2];
             float xi[3], gX[3], g_n[3], g_vr[3], g_xi[3], g_dist;
-            const int t = h_find(jd, xi);
-            g_next(gX);
+            const int t = h_find(jd, slot, xi);
+            g_next(slot, gX);
             contact_vjp(P, mu, nrm, vr, vn, fp, fp == raw, dist, c, xi, gF, gX, g_n, g_dist, g_vr,
                         g_xi, gk, gmu);
-            if (t >= 0) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(g_xi[0], g_xi[1], g_xi[2], 0.f);
+            keep_xi(t, g_xi);
             // n = d / |d|, dist = |d| - 2r
             const float gnn = dot3(g_n, nrm);
             float gd[3];
@@ -475,63 +582,67 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_contacts(
                 gx[a] += gd[a];
                 gv[a] += g_vr[a];
             }
-            if (kc < kAdjContacts) {
+            if (live && kc < kAdjContacts) {
                 cg_d[static_cast<size_t>(kc) * n + p] = make_float4(gd[0], gd[1], gd[2], __int_as_float(jd));
                 cg_v[static_cast<size_t>(kc) * n + p] = make_float4(g_vr[0], g_vr[1], g_vr[2], 0.f);
                 cg_p[static_cast<size_t>(p) * kAdjContacts + kc] = static_cast<int>(qk);
-            } else {
-                ctl->error = kAdjErrContacts;
             }
-            ++kc;
-            ++h_new;
         }
-        listed = 0;
-    };
-    // The forward pass's candidate walk: touching candidates (squared
-    // distance below touch2, exactly the contact law's dist < 0) listed in
-    // LDS in stencil order and drained by flush; a grain with more than
-    // kContactList of them walks again past the ones drained.
-    int consumed = 0, seen = 0;
-    auto probe = [&](uint32_t q, float px, float py, float pz) __attribute__((always_inline)) {
-        const float e0 = x[0] - px, e1 = x[1] - py, e2 = x[2] - pz;
-        float s2 = 0.f;
-        s2 += e0 * e0;
-        s2 += e1 * e1;
-        s2 += e2 * e2;
-        if (s2 < P.touch2 && q != static_cast<uint32_t>(p)) {
-            if (seen >= consumed && listed < kContactList) {
-                s_list[listed][threadIdx.x] = q;
-                ++listed;
-            }
-            ++seen;
-        }
-    };
-    auto column = [&](int c) __attribute__((always_inline)) {
-        uint4 q4;
-        __builtin_memcpy(&q4, start + mc.index(cx + c / 3 - 1, cy + c % 3 - 1, cz - 1), sizeof(uint4));
-        return q4;
-    };
-    for (;;) {
-        seen = 0;
-        walk_stencil<4, false>(GlobalPos{Sp}, kcap, 0u, 0xFFFFFFFFu, column, probe);
         consumed += listed;
-        flush();
-        if (seen <= consumed) break;
+        if (consumed >= mine) break;
+        walk();
     }
-    cg_cnt[p] = min(kc, kAdjContacts);
-    own_x[p] = make_float4(gx[0], gx[1], gx[2], 0.f);
-    own_v[p] = make_float4(gv[0], gv[1], gv[2], 0.f);
-    double2 gp = gparam[id];
-    gp.x += static_cast<double>(gmu);
-    gp.y += static_cast<double>(gk);
-    gparam[id] = gp;
+    // The grain's share: the groups' parts in order (a fourth, zero part
+    // stands for lane 3 of four, which takes no group).
+    float sx[3], sv[3], sk, smu;
+    unsigned w_all = written;
+    if constexpr (L == 1) {
+        while (grp < 3) fold();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            sx[a] = tx[a] + 0.f;
+            sv[a] = tv[a] + 0.f;
+        }
+        sk = tk + 0.f;
+        smu = tmu + 0.f;
+    } else {
+        auto grain_sum = [&](float a) __attribute__((always_inline)) {
+            const float a1 = __shfl(a, lane + 1, 64), a2 = __shfl(a, lane + 2, 64), a3 = __shfl(a, lane + 3, 64);
+            return ((a + a1) + a2) + a3;
+        };
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            sx[a] = grain_sum(gx[a]);
+            sv[a] = grain_sum(gv[a]);
+        }
+        sk = grain_sum(gk);
+        smu = grain_sum(gmu);
+        w_all |= __shfl(written, lane + 1, 64) | __shfl(written, lane + 2, 64) | __shfl(written, lane + 3, 64);
+    }
+    if (sub == 0 && live) {
+        for (int t = 0; t < h_old; ++t) {
+            if (!((w_all >> t) & 1u)) lxi_prev[static_cast<size_t>(t) * n + hs] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (pairs > kAdjContacts) ctl->error = kAdjErrContacts;
+        cg_cnt[p] = min(pairs, kAdjContacts);
+        own_x[p] = make_float4(sx[0], sx[1], sx[2], 0.f);
+        own_v[p] = make_float4(sv[0], sv[1], sv[2], 0.f);
+        double2 gp = gparam[id];
+        gp.x += static_cast<double>(smu);
+        gp.y += static_cast<double>(sk);
+        gparam[id] = gp;
+    }
 }
 
 // Pass B: dL/d(state s) of the grain at cell-sorted position j = its own
-// share + minus the stored share of every pair contact (i, j) that grain i
-// evaluated, in j's contact order, written at j's position in the previous
-// step's order (hidx[j]).  Partner lists are grain-major
-// (cg_p[i * kAdjContacts + t]) so that a search reads four partners per load.
+// share minus the stored shares of every pair contact (i, j) that grain i
+// evaluated, written at j's position in the previous step's order
+// (hidx[j]).  j's partners are summed in four parts, part k taking partners
+// k, k + 4, ... (lane k of four per grain, or one lane part after part);
+// the parts are added in order and the sum subtracted from the own share.
+// Partner lists are grain-major (cg_p[i * kAdjContacts + t]) so that a search
+// reads four partners per load.
+template <int L>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(const float4* __restrict__ own_x,
                                                        const float4* __restrict__ own_v,
                                                        const float4* __restrict__ cg_d,
@@ -541,16 +652,24 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(const float4* __restri
                                                        const int* __restrict__ hidx,
                                                        float4* __restrict__ lx, float4* __restrict__ lv,
                                                        int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    float4 gx = own_x[j], gv = own_v[j];
+    static_assert(L == 1 || L == kAdjLanes, "adjoint pass variants");
+    constexpr int G = kAdjBlock / L;
+    const int lane = threadIdx.x;
+    const int sub = lane & (L - 1);
+    const int first = blockIdx.x * G;
+    if (first >= n) return;
+    const bool live = first + lane / L < n;
+    const int j = live ? first + lane / L : n - 1;
     const int cj = cg_cnt[j];
-    const int4* pj = reinterpret_cast<const int4*>(cg_p + static_cast<size_t>(j) * kAdjContacts);
-    for (int k0 = 0; k0 < cj; k0 += 4) {
-        const int4 mine = pj[k0 >> 2];
-        const int part[4] = {mine.x, mine.y, mine.z, mine.w};
-        for (int u = 0; u < 4 && k0 + u < cj; ++u) {
-            const int i = part[u];
+    const int* pj = cg_p + static_cast<size_t>(j) * kAdjContacts;
+    float e[3] = {0.f, 0.f, 0.f}, f[3] = {0.f, 0.f, 0.f};  // the current part
+    float te[3] = {0.f, 0.f, 0.f}, tf[3] = {0.f, 0.f, 0.f};  // the parts before it (L = 1)
+#pragma unroll 1
+    for (int k = L == 1 ? 0 : sub; k < (L == 1 ? kAdjLanes : sub + 1); ++k) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) e[a] = f[a] = 0.f;
+        for (int u = k; u < cj; u += kAdjLanes) {
+            const int i = pj[u];
             const int ci = cg_cnt[i];
             const int4* pi = reinterpret_cast<const int4*>(cg_p + static_cast<size_t>(i) * kAdjContacts);
             int t = -1;
@@ -563,16 +682,37 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(const float4* __restri
                 }
             }
             if (t >= 0) {
-                const float4 e = cg_d[static_cast<size_t>(t) * n + i];
-                const float4 f = cg_v[static_cast<size_t>(t) * n + i];
-                gx.x -= e.x; gx.y -= e.y; gx.z -= e.z;
-                gv.x -= f.x; gv.y -= f.y; gv.z -= f.z;
+                const float4 d = cg_d[static_cast<size_t>(t) * n + i];
+                const float4 w = cg_v[static_cast<size_t>(t) * n + i];
+                e[0] += d.x; e[1] += d.y; e[2] += d.z;
+                f[0] += w.x; f[1] += w.y; f[2] += w.z;
+            }
+        }
+        if constexpr (L == 1) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                te[a] = k == 0 ? e[a] : te[a] + e[a];
+                tf[a] = k == 0 ? f[a] : tf[a] + f[a];
             }
         }
     }
-    const int dst = hidx[j];
-    lx[dst] = gx;
-    lv[dst] = gv;
+    if constexpr (L > 1) {
+        auto grain_sum = [&](float a) __attribute__((always_inline)) {
+            const float a1 = __shfl(a, lane + 1, 64), a2 = __shfl(a, lane + 2, 64), a3 = __shfl(a, lane + 3, 64);
+            return ((a + a1) + a2) + a3;
+        };
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            te[a] = grain_sum(e[a]);
+            tf[a] = grain_sum(f[a]);
+        }
+    }
+    if (sub == 0 && live) {
+        const float4 gx = own_x[j], gv = own_v[j];
+        const int dst = hidx[j];
+        lx[dst] = make_float4(gx.x - te[0], gx.y - te[1], gx.z - te[2], gx.w);
+        lv[dst] = make_float4(gv.x - tf[0], gv.y - tf[1], gv.z - tf[2], gv.w);
+    }
 }
 
 // dL/d(state) and the spring cotangents from original order (a checkpoint's

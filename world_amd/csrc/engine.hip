@@ -1919,7 +1919,10 @@ void adj_recompute_body(wd_sim* s, int set, int steps, int par) {
 void adj_sweep_body(wd_sim* s, int set, int steps, int cur, hipStream_t st) {
     auto* A = s->adj;
     const int blocks = static_cast<int>((s->n + kBlock - 1) / kBlock);
-    const int agb = grain_blocks(s->n, kAdjBlock);
+    // four lanes per grain below kAdjLanesUpTo grains, one above (the same bits)
+    const bool four = s->n <= kAdjLanesUpTo;
+    const int agb = grain_blocks(s->n, four ? kAdjBlock / kAdjLanes : kAdjBlock);
+    const int ggb = four ? (s->n + kAdjBlock / kAdjLanes - 1) / (kAdjBlock / kAdjLanes) : (s->n + kAdjBlock - 1) / kAdjBlock;
     // The cotangents arrive in original order (the checkpoint's); the last
     // step's contact adjoint reads them in that step's cell order.
     hipLaunchKernelGGL(k_adj_to_cells, dim3(blocks), dim3(kBlock), 0, st,
@@ -1929,7 +1932,7 @@ void adj_sweep_body(wd_sim* s, int set, int steps, int cur, hipStream_t st) {
     for (int k = steps - 1; k >= 0; --k) {
         const wd_sim::BinSlot& b = A->at(set, k);
         const bool last = k == steps - 1;
-        hipLaunchKernelGGL(k_adj_contacts, dim3(agb), dim3(kAdjBlock), 0, st, s->dp, s->ctl,
+        hipLaunchKernelGGL(four ? k_adj_contacts<kAdjLanes> : k_adj_contacts<1>, dim3(agb), dim3(kAdjBlock), 0, st, s->dp, s->ctl,
                            (const float4*)b.Sp, (const float4*)b.Sv, (const uint32_t*)b.start,
                            (const int*)b.hidx, (const float4*)b.H, (const int*)b.Hc,
                            (const int*)b.mbox, (const uint32_t*)b.mcell,
@@ -1937,7 +1940,7 @@ void adj_sweep_body(wd_sim* s, int set, int steps, int cur, hipStream_t st) {
                            (const float4*)A->lxi[cur], A->lxi[cur ^ 1], A->own_x, A->own_v, A->cg_d,
                            A->cg_v, A->cg_cnt, A->cg_p, A->gparam, s->n);
         // dL/d(state before step k), at the previous step's positions
-        hipLaunchKernelGGL(k_adj_gather, dim3((s->n + kAdjBlock - 1) / kAdjBlock), dim3(kAdjBlock), 0, st, A->own_x,
+        hipLaunchKernelGGL(four ? k_adj_gather<kAdjLanes> : k_adj_gather<1>, dim3(ggb), dim3(kAdjBlock), 0, st, A->own_x,
                            A->own_v, A->cg_d, A->cg_v, A->cg_cnt, A->cg_p, (const int*)b.hidx, A->lx,
                            A->lv, s->n);
         cur ^= 1;
