@@ -303,6 +303,19 @@ int wd_adjoint_record(wd_sim* sim, int nsteps, int checkpoint_every);
 int wd_adjoint_backward(wd_sim* sim, const float* grid_cotangent, const float* state_cotangent,
                         float* grad_state0, double* grad_params);
 
+/* ---- device buffers (extension; the reference has no counterpart) ---------
+ * wd_set_state, wd_batch_density_grid and wd_adjoint_backward with the
+ * caller's buffers in device memory of the handle's GPU (torch tensors, say)
+ * instead of host memory, for a differentiable caller whose data stays on the
+ * GPU; grad_params stays a host array.  Each call waits for the device first
+ * (the caller's producers, on any stream) and returns with its results
+ * complete.  WD_ERR_ARG for a pointer that is not device memory of the
+ * handle's GPU. */
+int wd_set_state_device(wd_sim* sim, const float* state_in);
+int wd_batch_density_grid_device(wd_sim* sim, int member, float* grid_out);
+int wd_adjoint_backward_device(wd_sim* sim, const float* grid_cotangent, const float* state_cotangent,
+                               float* grad_state0, double* grad_params);
+
 /* ---- launch configuration (extension) -----------------------------------
  * The force pass has six variants: one lane per grain with 64, 128 or 256
  * grains per block (lanes 0), or a one-wave block whose contacts are spread

@@ -85,3 +85,38 @@ def test_two_forwards_then_one_backward_and_broadcast_friction():
         assert float(mu.grad) == pytest.approx(float(sep[0][1].sum() + sep[1][1].sum()), rel=1e-12)
     finally:
         win.close()
+
+
+def test_device_tensors_give_the_host_path_bits():
+    """Initial positions on the GPU: the grids and gradients come back on the
+    GPU (device buffers across the C ABI) with the same bits as the host path;
+    a host pointer given to a device entry point is refused."""
+    import torch
+
+    from world_amd import _capi
+
+    c = workloads.column(6, 6, 7, friction=0.3)
+    win = DensityWindow(c, 40)
+    try:
+        x_init = win.x0.copy()
+        target = torch.zeros(1, win.nodes, dtype=torch.float32)
+        grads, grids = [], []
+        for dev in ("cpu", "cuda"):
+            x0 = torch.tensor(x_init, device=dev, requires_grad=True)
+            mu = torch.tensor(0.3, dtype=torch.float64, device=dev, requires_grad=True)
+            k_n = torch.tensor(150.0, device=dev, requires_grad=True)
+            rho = density_window(x0, mu, k_n, win)
+            assert rho.device.type == dev
+            ((rho[:, 0] - target.to(dev)) ** 2).sum().backward()
+            assert x0.grad.device.type == dev
+            grids.append(rho.detach().cpu().numpy())
+            grads.append((x0.grad.cpu().numpy(), float(mu.grad), float(k_n.grad)))
+        np.testing.assert_array_equal(grids[0], grids[1])
+        np.testing.assert_array_equal(grads[0][0], grads[1][0])
+        assert grads[0][1:] == grads[1][1:]
+        assert np.count_nonzero(grads[0][0]) > 0
+        host = np.zeros((win.sim.n, 6), np.float32)
+        with pytest.raises(_capi.WdError, match="not device memory"):
+            _capi.check(_capi.lib().wd_set_state_device(win.sim._h, _capi.ptr(host)))
+    finally:
+        win.close()

@@ -160,6 +160,10 @@ _SIGNATURES = [
     ("wd_contact_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     ("wd_adjoint_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
+    ("wd_set_state_device", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("wd_batch_density_grid_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("wd_adjoint_backward_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]

@@ -9,8 +9,11 @@ checkpoints every K steps (K = ceil(sqrt(window)) by default).
 
 The reference solver has no differentiation; its surrogate side
 (surrogate/) consumes CIC grids written by the dataset driver, which is the
-quantity differentiated here.  Host buffers cross the C ABI; torch tensors
-are plumbing on either side.
+quantity differentiated here.  Initial positions on the GPU stay there: the
+state, the grids and the gradients cross the C ABI as device buffers of the
+handle's GPU (wd_set_state_device, wd_batch_density_grid_device,
+wd_adjoint_backward_device); only the friction and stiffness, scalars, go
+through the host.  CPU tensors take host buffers.
 
 A window handle holds one recorded trajectory at a time.  Each forward
 through it is numbered; a backward whose forward is no longer the window's
@@ -70,6 +73,35 @@ class DensityWindow:
         gs, gp = self.sim.adjoint_backward(np.asarray(grid_cotangent, np.float32).reshape(-1))
         return gs[:, :3].copy(), gp[:, 0].copy(), float(gp[:, 1].sum())
 
+    def run_device(self, x0, mu, k_n: float):
+        """run() with x0 a torch tensor [n, 3] on the GPU: the grids come back
+        as a device tensor [members, 4, nodes]; nothing crosses the host but
+        the scalars."""
+        import torch
+
+        mu = np.broadcast_to(np.asarray(mu, np.float64), (self.members,))
+        if getattr(self, "_v0_dev", None) is None or self._v0_dev.device != x0.device:
+            self._v0_dev = torch.from_numpy(self.v0).to(x0.device)
+        st = torch.cat([x0.detach().to(torch.float32).reshape(-1, 3), self._v0_dev], dim=1).contiguous()
+        for m in range(self.members):
+            self.sim.set_friction(float(mu[m]), m)
+        self.sim.set_normal_stiffness(float(k_n))
+        self.sim.set_state_device(st)  # also clears the contact history
+        self.sim.adjoint_record(self.nsteps, self.every)
+        self.generation += 1
+        grids = torch.empty(self.members * 4 * self.nodes, dtype=torch.float32, device=x0.device)
+        self.sim.density_grids_device(grids)
+        return grids.reshape(self.members, 4, self.nodes)
+
+    def backward_device(self, grid_cotangent):
+        """backward() with the cotangent a device tensor: dL/dx0 [n, 3] comes
+        back on the device."""
+        import torch
+
+        g = grid_cotangent.detach().to(torch.float32).contiguous().reshape(-1)
+        gs, gp = self.sim.adjoint_backward_device(g)
+        return gs[:, :3], gp[:, 0].copy(), float(gp[:, 1].sum())
+
 
 def density_window_function():
     """The torch.autograd.Function class (torch is imported on first use)."""
@@ -78,14 +110,19 @@ def density_window_function():
     class DensityWindowFunction(torch.autograd.Function):
         @staticmethod
         def forward(ctx, x0, mu, k_n, window: DensityWindow):
-            inputs = (x0.detach().cpu().numpy().copy(), mu.detach().cpu().numpy().copy(),
-                      float(k_n.detach().cpu()))
-            grids = window.run(*inputs)
+            ctx.on_device = x0.is_cuda
+            params = (mu.detach().cpu().numpy().copy(), float(k_n.detach().cpu()))
+            if ctx.on_device:
+                inputs = (x0.detach().clone(),) + params
+                grids = window.run_device(*inputs)
+            else:
+                inputs = (x0.detach().cpu().numpy().copy(),) + params
+                grids = torch.from_numpy(window.run(*inputs))
             ctx.window = window
             ctx.inputs = inputs
             ctx.generation = window.generation
             ctx.meta = [(t.dtype, t.shape, t.device) for t in (x0, mu, k_n)]
-            return torch.from_numpy(grids).to(x0.device)
+            return grids
 
         @staticmethod
         def backward(ctx, g):
@@ -93,14 +130,18 @@ def density_window_function():
             if win.generation != ctx.generation:
                 # another forward ran through the window since: record this
                 # one's trajectory again from its inputs
-                win.run(*ctx.inputs)
+                (win.run_device if ctx.on_device else win.run)(*ctx.inputs)
                 ctx.generation = win.generation
-            gx, gmu, gk = win.backward(g.detach().cpu().numpy())
+            if ctx.on_device:
+                gx, gmu, gk = win.backward_device(g)
+            else:
+                gx, gmu, gk = win.backward(g.detach().cpu().numpy())
             if ctx.meta[1][1].numel() == 1:
                 gmu = np.asarray(gmu).sum()  # one friction broadcast to every member
             out = []
             for val, (dtype, shape, dev) in zip((gx, gmu, np.array(gk)), ctx.meta):
-                out.append(torch.as_tensor(np.asarray(val)).reshape(shape).to(device=dev, dtype=dtype))
+                val = val if isinstance(val, torch.Tensor) else torch.as_tensor(np.asarray(val))
+                out.append(val.reshape(shape).to(device=dev, dtype=dtype))
             return out[0], out[1], out[2], None
 
     return DensityWindowFunction

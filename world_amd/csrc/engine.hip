@@ -1135,11 +1135,12 @@ int allocate(wd_sim* s) {
 
 // Uploads an original-order state (and optionally materials) and rebuilds
 // the working layout.
-int ingest(wd_sim* s, const float* st, const int* mat) {
+int ingest(wd_sim* s, const float* st, const int* mat, bool st_on_device = false) {
     s->fused_ready = false;
     if (s->n == 0) return WD_OK;
     WD_HIP(hipStreamSynchronize(s->stream));
-    WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n), hipMemcpyHostToDevice));
+    WD_HIP(hipMemcpy(s->st6, st, sizeof(float) * 6 * static_cast<size_t>(s->n),
+                     st_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
     if (mat) {
         std::vector<int> code(static_cast<size_t>(s->n));
         for (int m = 0; m < s->members; ++m) {
@@ -2157,9 +2158,12 @@ extern "C" int wd_adjoint_record(wd_sim* s, int nsteps, int checkpoint_every) {
     return adj_run_record(s);
 }
 
-extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float* state_cot,
-                                   float* grad_state0, double* grad_params) {
-    WD_CHECK_SIM(s);
+namespace {
+
+// wd_adjoint_backward with the cotangents and dL/d(initial state) in host
+// memory (dev false) or in device memory of the handle's GPU (dev true).
+int adjoint_backward(wd_sim* s, const float* grid_cot, const float* state_cot, float* grad_state0,
+                     double* grad_params, bool dev) {
     auto* A = s->adj;
     if (!A) return fail(WD_ERR_STATE, "wd_adjoint_backward: no recorded window (wd_adjoint_record)");
     unsigned long long over = 0;
@@ -2170,8 +2174,9 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
                                   "the neighbor lists are not symmetric");
     }
     const size_t n = static_cast<size_t>(s->n);
-    if (grid_cot) WD_HIP(hipMemcpy(A->grid_cot, grid_cot, sizeof(float) * s->cic_count, hipMemcpyHostToDevice));
-    if (state_cot) WD_HIP(hipMemcpy(A->state_cot, state_cot, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    const hipMemcpyKind in_kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (grid_cot) WD_HIP(hipMemcpy(A->grid_cot, grid_cot, sizeof(float) * s->cic_count, in_kind));
+    if (state_cot) WD_HIP(hipMemcpy(A->state_cot, state_cot, sizeof(float) * 6 * n, in_kind));
     WD_HIP(hipMemsetAsync(A->lxi[0], 0, sizeof(float4) * kHist * n, s->stream));
     WD_HIP(hipMemsetAsync(A->lxi[1], 0, sizeof(float4) * kHist * n, s->stream));
     WD_HIP(hipMemsetAsync(A->gparam, 0, sizeof(double2) * n, s->stream));
@@ -2234,7 +2239,13 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
     WD_HIP(hipGetLastError());
     rc = s->sync();
     if (rc != WD_OK) return rc;
-    if (grad_state0) {
+    if (grad_state0 && dev) {
+        // x and v of each grain's float4s into its six floats
+        WD_HIP(hipMemcpy2D(grad_state0, 6 * sizeof(float), A->lx, sizeof(float4), 3 * sizeof(float), n,
+                           hipMemcpyDeviceToDevice));
+        WD_HIP(hipMemcpy2D(grad_state0 + 3, 6 * sizeof(float), A->lv, sizeof(float4), 3 * sizeof(float), n,
+                           hipMemcpyDeviceToDevice));
+    } else if (grad_state0) {
         std::vector<float4> x(n), v(n);
         WD_HIP(hipMemcpy(x.data(), A->lx, sizeof(float4) * n, hipMemcpyDeviceToHost));
         WD_HIP(hipMemcpy(v.data(), A->lv, sizeof(float4) * n, hipMemcpyDeviceToHost));
@@ -2258,6 +2269,74 @@ extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float
         }
     }
     return WD_OK;
+}
+
+// WD_OK when p is device memory of the handle's GPU.
+int device_buffer(const wd_sim* s, const void* p, const char* what) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice || a.device != s->device) {
+        (void)hipGetLastError();
+        return fail(WD_ERR_ARG, std::string(what) + ": not device memory of the handle's GPU");
+    }
+    return WD_OK;
+}
+
+}  // namespace
+
+extern "C" int wd_adjoint_backward(wd_sim* s, const float* grid_cot, const float* state_cot,
+                                   float* grad_state0, double* grad_params) {
+    WD_CHECK_SIM(s);
+    return adjoint_backward(s, grid_cot, state_cot, grad_state0, grad_params, false);
+}
+
+// ---------------------------------------------------------------------------
+// Device-memory variants (extension): the same calls with the caller's
+// buffers in device memory of the handle's GPU (torch tensors, say), so that
+// a differentiable caller's data stays on the GPU.  Each waits for the
+// device first (the caller's producers, on any stream) and returns with its
+// results complete.
+
+extern "C" int wd_set_state_device(wd_sim* s, const float* in) {
+    WD_CHECK_SIM(s);
+    if (!in) return fail(WD_ERR_ARG, "wd_set_state_device: null input");
+    if (s->n == 0) return WD_OK;
+    int rc = device_buffer(s, in, "wd_set_state_device");
+    if (rc != WD_OK) return rc;
+    WD_HIP(hipDeviceSynchronize());
+    return ingest(s, in, nullptr, true);
+}
+
+extern "C" int wd_batch_density_grid_device(wd_sim* s, int member, float* out) {
+    WD_CHECK_SIM(s);
+    if (!out || member < -1 || member >= s->members) {
+        return fail(WD_ERR_ARG, "wd_batch_density_grid_device: bad argument");
+    }
+    int rc = device_buffer(s, out, "wd_batch_density_grid_device");
+    if (rc != WD_OK) return rc;
+    WD_HIP(hipDeviceSynchronize());
+    hipEvent_t a = s->stage_begin();
+    s->enqueue_cic(s->cic_out);
+    s->stage_end(WD_STAGE_DENSITY_GRID, a);
+    WD_HIP(hipGetLastError());
+    const size_t per = 4 * s->nodes;
+    const float* src = s->cic_out + (member < 0 ? 0 : per * static_cast<size_t>(member));
+    const size_t count = member < 0 ? s->cic_count : per;
+    WD_HIP(hipMemcpyAsync(out, src, sizeof(float) * count, hipMemcpyDeviceToDevice, s->stream));
+    return s->sync();
+}
+
+extern "C" int wd_adjoint_backward_device(wd_sim* s, const float* grid_cot, const float* state_cot,
+                                          float* grad_state0, double* grad_params) {
+    WD_CHECK_SIM(s);
+    for (const void* p : {static_cast<const void*>(grid_cot), static_cast<const void*>(state_cot),
+                          static_cast<const void*>(grad_state0)}) {
+        if (p) {
+            const int rc = device_buffer(s, p, "wd_adjoint_backward_device");
+            if (rc != WD_OK) return rc;
+        }
+    }
+    WD_HIP(hipDeviceSynchronize());
+    return adjoint_backward(s, grid_cot, state_cot, grad_state0, grad_params, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -359,6 +359,46 @@ class ParticleDynamics:
                                         None if sc is None else ptr(sc), ptr(gs), ptr(gp)))
         return gs[: K_STATE_STRIDE * self.n].reshape(self.n, K_STATE_STRIDE), gp.reshape(members, 2)
 
+    # -- device buffers: torch tensors on the handle's GPU ----------------
+    def _device_tensor(self, t, size: int, what: str):
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise TypeError(f"{what} must be a contiguous float32 tensor on the GPU")
+        if t.numel() != size:
+            raise ValueError(f"{what} must hold {size} floats")
+        return C.c_void_p(t.data_ptr())
+
+    def set_state_device(self, state) -> None:
+        """The grain state from a device tensor of 6n floats (original
+        order), without a host copy."""
+        check(lib().wd_set_state_device(self._h, self._device_tensor(state, K_STATE_STRIDE * self.n, "state")))
+
+    def density_grids_device(self, out) -> None:
+        """Every member's CIC grids of the current state into a device tensor
+        of members * 4 * nodes floats."""
+        members = getattr(self, "members", 1)
+        size = members * K_CHANNELS * self.density_grid_nodes()
+        check(lib().wd_batch_density_grid_device(self._h, -1, self._device_tensor(out, size, "out")))
+
+    def adjoint_backward_device(self, grid_cotangent=None, state_cotangent=None):
+        """adjoint_backward with device tensors: the cotangents in, dL/d(initial
+        state) [n, 6] out on the device; [dL/dmu, dL/dk_n] per member (host)."""
+        import torch
+
+        members = getattr(self, "members", 1)
+        g = None if grid_cotangent is None else self._device_tensor(
+            grid_cotangent, members * K_CHANNELS * self.density_grid_nodes(), "grid cotangent")
+        sc = None if state_cotangent is None else self._device_tensor(
+            state_cotangent, K_STATE_STRIDE * self.n, "state cotangent")
+        if grid_cotangent is None and state_cotangent is None:
+            raise ValueError("adjoint_backward_device needs a grid or a state cotangent")
+        dev = (grid_cotangent if grid_cotangent is not None else state_cotangent).device
+        gs = torch.empty((self.n, K_STATE_STRIDE), dtype=torch.float32, device=dev)
+        gp = np.zeros(2 * members, np.float64)
+        check(lib().wd_adjoint_backward_device(self._h, g, sc, C.c_void_p(gs.data_ptr()), ptr(gp)))
+        return gs, gp.reshape(members, 2)
+
     def rhs(self) -> np.ndarray:
         out = np.zeros(K_STATE_STRIDE * max(self.n, 1), np.float32)
         check(lib().wd_rhs(self._h, ptr(out)))
