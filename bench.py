@@ -391,21 +391,19 @@ def ensemble_leg(steps: int) -> dict:
 
 def adjoint_window(make, cot_of, window_steps: int, every: int):
     """Forward, record and backward cost of one window, all over the same
-    steps and all with the force variant tuned and pinned and the graphs
-    captured beforehand.  A handle tunes (TUNE_STEPS steps), pins, captures the step
-    graphs (31 steps) and runs one window untimed, then times a forward step
-    over the next window; a second handle does the same and records and
-    differentiates its first window untimed (allocations, adjoint graph
-    captures) and then the same second window.  Returns seconds (forward,
-    record, backward) for the window."""
-    def settle(h):
-        h.take_step(TUNE_STEPS)
-        h.set_force_variant(*h.force_variant())
-        h.take_step(CAPTURE_STEPS)
-        h.sync()
-
+    steps, with the same tuned choices and the graphs captured beforehand.  A
+    handle tunes (TUNE_STEPS steps), pins its force variant, scan tile and
+    binning, captures the step graphs (31 steps) and runs one window untimed,
+    then times a forward step over the next window; a second handle takes the
+    same choices (so the record and the recomputed steps run the forward's
+    variant), runs the same steps, records and differentiates its first window
+    untimed (allocations, adjoint graph captures) and then the same second
+    window.  Returns seconds (forward, record, backward) for the window."""
     with make() as a:
-        settle(a)
+        a.take_step(TUNE_STEPS)
+        choice = (a.force_variant(), a.scan_tile(), a.binning())
+        a.set_force_variant(*choice[0])
+        a.take_step(CAPTURE_STEPS)
         a.take_step(window_steps)
         a.sync()
         t = time.perf_counter()
@@ -414,7 +412,11 @@ def adjoint_window(make, cot_of, window_steps: int, every: int):
         fwd = time.perf_counter() - t
     with make() as h:
         cot = cot_of(h)
-        settle(h)
+        h.set_force_variant(*choice[0])
+        h.set_scan_tile(choice[1])
+        h.set_binning(choice[2])
+        h.take_step(TUNE_STEPS + CAPTURE_STEPS)
+        h.sync()
         rec = bwd = 0.0
         for _ in range(2):
             t = time.perf_counter()
