@@ -87,36 +87,57 @@ def test_two_forwards_then_one_backward_and_broadcast_friction():
         win.close()
 
 
+DEVICE_SCRIPT = r"""
+import numpy as np
+import torch
+
+torch.cuda.init()  # before the first world_amd call: the library binds to torch's HIP runtime
+from world_amd import _capi, workloads
+from world_amd.adjoint import DensityWindow, density_window
+
+c = workloads.column(6, 6, 7, friction=0.3)
+win = DensityWindow(c, 40)
+x_init = win.x0.copy()
+target = torch.zeros(1, win.nodes, dtype=torch.float32)
+grads, grids = [], []
+for dev in ("cpu", "cuda"):
+    x0 = torch.tensor(x_init, device=dev, requires_grad=True)
+    mu = torch.tensor(0.3, dtype=torch.float64, device=dev, requires_grad=True)
+    k_n = torch.tensor(150.0, device=dev, requires_grad=True)
+    rho = density_window(x0, mu, k_n, win)
+    assert rho.device.type == dev
+    ((rho[:, 0] - target.to(dev)) ** 2).sum().backward()
+    assert x0.grad.device.type == dev
+    grids.append(rho.detach().cpu().numpy())
+    grads.append((x0.grad.cpu().numpy(), float(mu.grad), float(k_n.grad)))
+np.testing.assert_array_equal(grids[0], grids[1])
+np.testing.assert_array_equal(grads[0][0], grads[1][0])
+assert grads[0][1:] == grads[1][1:]
+assert np.count_nonzero(grads[0][0]) > 0
+host = np.zeros((win.sim.n, 6), np.float32)
+try:
+    _capi.check(_capi.lib().wd_set_state_device(win.sim._h, _capi.ptr(host)))
+    raise AssertionError("a host buffer was accepted")
+except _capi.WdError as e:
+    assert "not device memory" in str(e)
+win.close()
+print("device path: the host path's bits", flush=True)
+"""
+
+
 def test_device_tensors_give_the_host_path_bits():
     """Initial positions on the GPU: the grids and gradients come back on the
     GPU (device buffers across the C ABI) with the same bits as the host path;
-    a host pointer given to a device entry point is refused."""
-    import torch
+    a host pointer given to a device entry point is refused.  It runs in a
+    process of its own that sets up torch's GPU support before the library
+    loads (the order the device path needs: see world_amd/adjoint.py)."""
+    import subprocess
+    import sys
+    from pathlib import Path
 
-    from world_amd import _capi
-
-    c = workloads.column(6, 6, 7, friction=0.3)
-    win = DensityWindow(c, 40)
-    try:
-        x_init = win.x0.copy()
-        target = torch.zeros(1, win.nodes, dtype=torch.float32)
-        grads, grids = [], []
-        for dev in ("cpu", "cuda"):
-            x0 = torch.tensor(x_init, device=dev, requires_grad=True)
-            mu = torch.tensor(0.3, dtype=torch.float64, device=dev, requires_grad=True)
-            k_n = torch.tensor(150.0, device=dev, requires_grad=True)
-            rho = density_window(x0, mu, k_n, win)
-            assert rho.device.type == dev
-            ((rho[:, 0] - target.to(dev)) ** 2).sum().backward()
-            assert x0.grad.device.type == dev
-            grids.append(rho.detach().cpu().numpy())
-            grads.append((x0.grad.cpu().numpy(), float(mu.grad), float(k_n.grad)))
-        np.testing.assert_array_equal(grids[0], grids[1])
-        np.testing.assert_array_equal(grads[0][0], grads[1][0])
-        assert grads[0][1:] == grads[1][1:]
-        assert np.count_nonzero(grads[0][0]) > 0
-        host = np.zeros((win.sim.n, 6), np.float32)
-        with pytest.raises(_capi.WdError, match="not device memory"):
-            _capi.check(_capi.lib().wd_set_state_device(win.sim._h, _capi.ptr(host)))
-    finally:
-        win.close()
+    root = Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, "-c", DEVICE_SCRIPT], cwd=root, capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "the host path's bits" in r.stdout
