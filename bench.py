@@ -37,7 +37,8 @@ Secondary entries of the same JSON object:
             deceleration against -mu g = -4.905 m/s^2;
   cpu_baseline  the CPU oracle (the reference has no CPU path) on the
             host's cores from the GPU's state and springs at S0: the same
-            window, extended to about ten seconds of work, plus one thread.
+            window, extended to about ten seconds of work, plus one thread;
+            with column_1m, also the 1M window (column_1m.cpu_baseline).
 
 `roofline` is the force pass of the headline window: algorithmic bytes per
 grain (56 B of state and cell key, 8 B of the fused count (the cell
@@ -308,9 +309,12 @@ def runs32k_leg(steps: int = 5000) -> dict:
     return out
 
 
-def large_leg(k: int, warmup: int, pin, full_steps: int, eager: bool = False) -> dict:
+def large_leg(k: int, warmup: int, pin, full_steps: int, eager: bool = False,
+              cpu_seconds: float | None = None) -> dict:
     """The 1,048,576-grain column: a window of whole deposit periods (at
-    least k steps) around step 5,000, and the configured run."""
+    least k steps) around step 5,000, and the configured run; with
+    cpu_seconds, the CPU oracle over the same window (from the GPU's state and
+    springs at its start, its end state checked against the GPU's)."""
     from world_amd import workloads
 
     cfg = workloads.column_collapse_1m()
@@ -318,7 +322,7 @@ def large_leg(k: int, warmup: int, pin, full_steps: int, eager: bool = False) ->
         cfg.kernel_timing = 1
     kl = 100 * max(1, -(-k // 100))
     s0 = window_start(5000, kl, 100, warmup)
-    r = window(cfg, kl, s0, 100, False, pin)
+    r = window(cfg, kl, s0, 100, False, pin, keep_state=cpu_seconds is not None)
     v = r["n"] * kl / r["elapsed"]
     out = {"workload": "column_collapse_1m", "grains": r["n"], "steps": kl, "value": v,
            "unit": "grain-steps/s", "ms_per_step": 1e3 * r["elapsed"] / kl,
@@ -328,6 +332,8 @@ def large_leg(k: int, warmup: int, pin, full_steps: int, eager: bool = False) ->
     out["cell_scale_tuned"] = cell_scale_window(cfg, kl, s0, 100, r["variant"])
     if full_steps > 0:
         out["full_run"] = full_run(cfg, full_steps, 100, False)
+    if cpu_seconds is not None:
+        out["cpu_baseline"] = cpu_baseline(cfg, r, cpu_seconds)
     return out
 
 
@@ -727,7 +733,7 @@ def main() -> int:
         progress("column_32k_runs")
     if "large" in legs:
         out["column_1m"] = large_leg(k, args.warmup, pin_of(args.pin_1m), args.large_full_steps,
-                                     args.eager)
+                                     args.eager, args.cpu_seconds if "cpu" in legs else None)
         progress("column_1m")
     if "ensemble" in legs:
         out["uneven_ensemble_64"] = ensemble_leg(args.ensemble_steps)
