@@ -525,7 +525,8 @@ def copy_probe(mib: int = 256) -> None:
     torch.cuda.synchronize()
 
 
-WINDOW_STATE = ROOT / "gpurun_out" / "column_1m_step4950.npy"
+# The saved 1M window state lives in the temporary directory (24 MiB, not part of the tree).
+WINDOW_STATE = Path(os.environ.get("TMPDIR") or "/tmp") / "world_amd_column_1m_step4950.npy"
 
 
 def save_window_state(steps: int = 4950) -> None:
