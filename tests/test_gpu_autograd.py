@@ -91,7 +91,7 @@ DEVICE_SCRIPT = r"""
 import numpy as np
 import torch
 
-torch.cuda.init()  # before the first world_amd call: the library binds to torch's HIP runtime
+torch.cuda.init()  # torch (imported first) holds the HIP runtime the library binds to
 from world_amd import _capi, workloads
 from world_amd.adjoint import DensityWindow, density_window
 
@@ -129,8 +129,8 @@ def test_device_tensors_give_the_host_path_bits():
     """Initial positions on the GPU: the grids and gradients come back on the
     GPU (device buffers across the C ABI) with the same bits as the host path;
     a host pointer given to a device entry point is refused.  It runs in a
-    process of its own that sets up torch's GPU support before the library
-    loads (the order the device path needs: see world_amd/adjoint.py)."""
+    process of its own that imports torch before the library loads (the
+    order the device path needs: see world_amd/adjoint.py)."""
     import subprocess
     import sys
     from pathlib import Path

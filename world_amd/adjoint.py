@@ -9,10 +9,10 @@ checkpoints every K steps (K = ceil(sqrt(window)) by default).
 
 The reference solver has no differentiation; its surrogate side
 (surrogate/) consumes CIC grids written by the dataset driver, which is the
-quantity differentiated here.  Initial positions on the GPU stay there (torch's
-GPU support must be set up, torch.cuda.init(), before the first world_amd
-call in the process: torch ships its own copy of the HIP runtime, and the
-library then binds to it rather than loading a second one): the
+quantity differentiated here.  Initial positions on the GPU stay there (torch
+must be imported before the first world_amd call in the process: torch ships
+its own copy of the HIP runtime, and the library then binds to it rather
+than loading a second one): the
 state, the grids and the gradients cross the C ABI as device buffers of the
 handle's GPU (wd_set_state_device, wd_batch_density_grid_device,
 wd_adjoint_backward_device); only the friction and stiffness, scalars, go
@@ -84,9 +84,9 @@ class DensityWindow:
 
         if not torch.cuda.is_available():
             raise RuntimeError(
-                "torch sees no GPU in this process: the engine library was loaded before torch set up its "
-                "HIP runtime, and torch bundles a second copy of that runtime.  Call torch.cuda.init() before "
-                "the first world_amd call so that the library binds to torch's runtime.")
+                "torch sees no GPU in this process: the engine library was loaded before torch, which "
+                "bundles a second copy of the HIP runtime.  Import torch before the first world_amd call "
+                "so that the library binds to torch's runtime.")
         mu = np.broadcast_to(np.asarray(mu, np.float64), (self.members,))
         if getattr(self, "_v0_dev", None) is None or self._v0_dev.device != x0.device:
             self._v0_dev = torch.from_numpy(self.v0).to(x0.device)
