@@ -373,6 +373,8 @@ struct wd_sim {
             rf.slices = fin_slices;
             rf.energy_log = energy_log;
             rf.flags = fin_flags;
+            rf.box_out = bin_box_out;
+            rf.cell_out = bin_cell_out;
             (void)fin_gpp;
         }
         auto scan = [&](auto kern) {
@@ -579,6 +581,18 @@ struct wd_sim {
             if (last) enqueue_bin(1, nullptr, nullptr, NP[0], NV[0], fin[0], fin[1], fin[2]);
             else enqueue_bin(2, NP[b], NV[b], NP[b ^ 1], NV[b ^ 1], fin[0], fin[1], fin[2]);
         }
+    }
+
+    // A fused semi-implicit Euler step whose bin writes where `redirect`
+    // points the bin's outputs once the force pass is enqueued (the
+    // adjoint's recompute: the next step's slot).
+    template <class Redirect>
+    void enqueue_step_fused_into(Redirect redirect) {
+        int fin[3] = {0, 0, 0};
+        if (energy_on) launch_force<kModeSIE, true>(nullptr, nullptr, nullptr, true, true, fin);
+        else launch_force<kModeSIE, false>(nullptr, nullptr, nullptr, true, true, fin);
+        redirect();
+        enqueue_bin(0, nullptr, nullptr, nullptr, nullptr, fin[0], fin[1], fin[2]);
     }
 
     // The step the handle runs now (fused steps need a primed state).
@@ -1898,18 +1912,37 @@ int adj_run_record(wd_sim* s) {
 // segments (eagerly with kernel timing on).
 void adj_recompute_body(wd_sim* s, int set, int steps, int par) {
     auto* A = s->adj;
-    for (int k = 0; k < steps; ++k) {
+    auto bin_into = [&](int k) {
         wd_sim::BinSlot& b = A->at(set, k);
-        s->bin_box_out = b.mbox;  // k_count keeps the step's boxes in the slot
+        s->bin_box_out = b.mbox;  // the bin keeps the step's grid in the slot
         s->bin_cell_out = b.mcell;
         s->Sp = b.Sp;
         s->Sv = b.Sv;
         s->start = b.start;
         s->hidx[0] = b.hidx;
-        s->H[par] = b.H;
-        s->Hc[par] = b.Hc;
+    };
+    auto springs = [&](int k) {
+        s->H[par] = A->at(set, k).H;
+        s->Hc[par] = A->at(set, k).Hc;
         s->H[par ^ 1] = A->at(set, k + 1).H;
         s->Hc[par ^ 1] = A->at(set, k + 1).Hc;
+    };
+    if (s->use_fused()) {
+        // Fused steps (the classic sequence's bits in three launches): the
+        // checkpoint's state is binned into slot 0; step k's force pass then
+        // reads slot k and its bin fills slot k + 1.
+        bin_into(0);
+        s->enqueue_bin_current(true);
+        for (int k = 0; k < steps; ++k) {
+            springs(k);
+            s->enqueue_step_fused_into([&]() { bin_into(k + 1); });
+            par ^= 1;
+        }
+        return;
+    }
+    for (int k = 0; k < steps; ++k) {
+        bin_into(k);
+        springs(k);
         s->enqueue_step();
         par ^= 1;
     }
@@ -2005,7 +2038,7 @@ int adj_segment(wd_sim* s, const wd_sim::Snap& ck, int steps, int par, int cur) 
     if (s->timing) {
         adj_segment_body(s, steps, par, cur);
     } else {
-        const std::array<int, 4> key{steps, par, cur, s->force_var};
+        const std::array<int, 4> key{steps, par, cur, 2 * s->force_var + (s->use_fused() ? 1 : 0)};
         hipGraphExec_t exec = nullptr;
         for (auto& g : A->seg_graphs) {
             if (g.first == key) exec = g.second;
@@ -2068,7 +2101,7 @@ int adj_recompute(wd_sim* s, const wd_sim::Snap& ck, int set, int steps, int par
     s->Hc[par] = A->at(set, 0).Hc;
     snap_load(s, ck);
     hipGraphExec_t exec = nullptr;
-    const int rc = adj_graph(A->rec_graphs, {steps, par, set, s->force_var}, A->rstream,
+    const int rc = adj_graph(A->rec_graphs, {steps, par, set, 2 * s->force_var + (s->use_fused() ? 1 : 0)}, A->rstream,
                              [&]() { adj_recompute_body(s, set, steps, par); }, &exec);
     if (rc != WD_OK) return rc;
     if (hipGraphLaunch(exec, A->rstream) != hipSuccess) return fail(WD_ERR_HIP, "adjoint recompute launch");

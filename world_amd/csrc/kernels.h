@@ -132,6 +132,8 @@ struct ReorderFuse {
     const FinSlice* slices;
     double* energy_log;
     int flags;
+    int* box_out;         // when set: the grid the bin used (box and cell offsets), as k_count
+    uint32_t* cell_out;   // keeps it for a recomputed step's slot
 };
 
 // What k_finalize publishes.

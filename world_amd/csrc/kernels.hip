@@ -1287,6 +1287,12 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
             for (int k = 0; k < 6; ++k) P.mbox[k] = P.nbox[k];
             P.mcell[0] = P.ncell[0];
             P.mcell[1] = P.ncell[1];
+            if (rf.box_out) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) rf.box_out[k] = P.nbox[k];
+                rf.cell_out[0] = P.ncell[0];
+                rf.cell_out[1] = P.ncell[1];
+            }
             ctl->bin_miss = 0;
         }
         slice_combine(P, ctl, rf.slices, rf.energy_log, rf.flags, nullptr);
