@@ -395,7 +395,7 @@ def ensemble_leg(steps: int) -> dict:
             "speedup": tl / tb, "stable": stable, "includes_online_tuning": True}
 
 
-def adjoint_window(make, cot_of, window_steps: int, every: int):
+def adjoint_window(make, cot_of, window_steps: int, every: int, pin=None):
     """Forward, record and backward cost of one window, all over the same
     steps, with the same tuned choices and the graphs captured beforehand.  A
     handle tunes (TUNE_STEPS steps), pins its force variant, scan tile and
@@ -406,6 +406,8 @@ def adjoint_window(make, cot_of, window_steps: int, every: int):
     untimed (allocations, adjoint graph captures) and then the same second
     window.  Returns seconds (forward, record, backward) for the window."""
     with make() as a:
+        if pin:
+            a.set_force_variant(*pin)  # no tuning: the default scan tile and binning
         a.take_step(TUNE_STEPS)
         choice = (a.force_variant(), a.scan_tile(), a.binning())
         a.set_force_variant(*choice[0])
@@ -435,7 +437,7 @@ def adjoint_window(make, cot_of, window_steps: int, every: int):
     return fwd, rec, bwd
 
 
-def adjoint_leg(window_steps: int = 200, every: int = 20) -> dict:
+def adjoint_leg(window_steps: int = 200, every: int = 20, pin=None) -> dict:
     """Backward-step cost of the 8,192-grain density-loss window against a
     forward step over the same window."""
     import numpy as np
@@ -447,7 +449,7 @@ def adjoint_leg(window_steps: int = 200, every: int = 20) -> dict:
     fwd, rec, bwd = adjoint_window(
         lambda: ParticleDynamics(c),
         lambda h: np.random.default_rng(0).normal(size=4 * h.density_grid_nodes()).astype(np.float32),
-        window_steps, every)
+        window_steps, every, pin)
     return {"workload": "column_8k_window", "grains": 8192, "window_steps": window_steps,
             "checkpoint_every": every, "forward_ms_per_step": 1e3 * fwd / window_steps,
             "record_ms_per_step": 1e3 * rec / window_steps,
@@ -668,6 +670,9 @@ def main() -> int:
                     help="first run one 256 MiB device copy (counter calibration)")
     ap.add_argument("--pin-32k", default="", metavar="BLOCK/LANES",
                     help="fix the 32,768-grain force-pass variant instead of tuning it before the window")
+    ap.add_argument("--pin-8k", default="", metavar="BLOCK/LANES",
+                    help="the same for the 8,192-grain adjoint window (for profiler runs, whose "
+                         "serialized dispatches mislead the tuner)")
     ap.add_argument("--pin-1m", default="", metavar="BLOCK/LANES",
                     help="the same for the 1,048,576-grain column")
     args = ap.parse_args()
@@ -740,7 +745,7 @@ def main() -> int:
         out["uneven_ensemble_64"] = ensemble_leg(args.ensemble_steps)
         progress("uneven_ensemble_64")
     if "adjoint" in legs:
-        out["adjoint"] = adjoint_leg()
+        out["adjoint"] = adjoint_leg(pin=pin_of(args.pin_8k))
         progress("adjoint")
     if "batched_adjoint" in legs:
         out["batched_adjoint"] = batched_adjoint_leg()
